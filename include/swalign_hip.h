@@ -1,0 +1,167 @@
+/*
+ * swalign_hip.h -- C-ABI of the MI355X (gfx950) SWalign library, libswalign_hip.so.
+ *
+ * Optimal local alignment of two byte sequences with affine gaps (Smith-Waterman-Gotoh),
+ * as computed by Sterbic/Projekt-2012 ("SWalign" v0.6.3), with the forward pass,
+ * checkpoint rows/columns, end-cell search and traceback run as hand-written HIP
+ * kernels.  Every function returns an int status (SWH_OK = 0); nothing in the library
+ * calls exit().  All sizes are int64.  Inputs are borrowed for the duration of a call;
+ * results are owned by the library until swh_result_free().
+ *
+ * Orientation follows the reference (src/SWutils.cpp:84-88): the longer sequence
+ * becomes the rows ("A"), the shorter the columns ("B"); with equal lengths the
+ * argument order is kept.  Coordinates in results are 0-based [row in A, column in B].
+ *
+ * Scoring: s(a,b) = match if a == b (byte equality, no alphabet mapping) else mismatch;
+ * a gap of length L costs gap_open + (L-1)*gap_extend (the reference's "first" and
+ * "extension", src/SWutils.h:15-20).  Valid iff match >= 1 and the three penalties
+ * are <= -1 (src/SWutils.cpp:36).
+ *
+ * Canonical tie rules (the reference's own order depends on its launch configuration):
+ *   end cell : the first cell in row-major order holding the maximal score;
+ *   traceback: greedy backward walk, H: zero (start) > diagonal > vertical gap >
+ *              horizontal gap; gaps: open > extend.
+ * CIGAR ops: '=' match, 'X' mismatch, 'D' base of A against a gap (vertical move),
+ *            'I' base of B against a gap (horizontal move).
+ */
+#ifndef SWALIGN_HIP_H_
+#define SWALIGN_HIP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------- */
+enum {
+    SWH_OK = 0,
+    SWH_ERR_INVALID_ARG = 1,     /* null pointer, negative length, bad option            */
+    SWH_ERR_SCORING = 2,         /* scoring values not usable (SWutils.cpp:36-37)        */
+    SWH_ERR_NO_DEVICE = 3,       /* no HIP device (SWutils.cpp:323-324)                  */
+    SWH_ERR_HIP = 4,             /* a HIP runtime call failed                            */
+    SWH_ERR_OUT_OF_MEMORY = 5,   /* device allocation failed or budget too small         */
+    SWH_ERR_BUDGET = 6,          /* request rejected: exceeds the HBM admission budget   */
+    SWH_ERR_INTERNAL = 7,        /* traceback inconsistency (should never happen)        */
+    SWH_ERR_NOT_FOUND = 8        /* unknown job id / nothing to poll                     */
+};
+
+/* Mirrors `scoring` (src/SWutils.h:15-20) field for field. */
+typedef struct {
+    int32_t match;
+    int32_t mismatch;
+    int32_t gap_open;     /* "first"     */
+    int32_t gap_extend;   /* "extension" */
+} swh_scoring;
+
+/* Options; zero-initialise for defaults. */
+typedef struct {
+    int64_t checkpoint_rows;   /* K: checkpoint row spacing (multiple of 256), 0 = auto     */
+    int64_t checkpoint_cols;   /* Kc: checkpoint column spacing (power of two >= 64), 0 = K */
+    int64_t hbm_budget_bytes;  /* cap on checkpoint memory, 0 = half of free HBM            */
+    int32_t score_only;        /* 1: score + end cell only (no traceback)                   */
+    int32_t precision;         /* 0 = auto, 32 = int32; batch path: 16 = packed int16 first  */
+    int32_t want_xpoints;      /* 1: return the path nodes on checkpoint rows                */
+    int32_t reserved[5];
+} swh_opts;
+
+/* Path node on a checkpoint row (the analogue of the reference's crosspoints,
+ * src/Crosspointer.cpp:104-108, printed by src/main.cu:245-248). */
+typedef struct {
+    int64_t row, col;
+    int32_t score;   /* forward value of the path at that node */
+    int32_t layer;   /* 0 = H (match/mismatch state), 2 = inside a vertical gap */
+} swh_xpoint;
+
+typedef struct {
+    int32_t score;
+    int32_t swapped;          /* 1 if seq2 was longer and became the rows            */
+    int64_t end_row, end_col; /* 0-based; -1 when score == 0                         */
+    int64_t start_row, start_col;
+    int64_t m, n;             /* rows (A) and columns (B) after orientation          */
+    char* cigar;              /* NUL-terminated run-length CIGAR, NULL if score-only */
+    int64_t cigar_len;
+    int64_t n_ops;            /* alignment columns                                   */
+    swh_xpoint* xpoints;      /* start -> end, NULL unless want_xpoints             */
+    int64_t n_xpoints;
+    int64_t checkpoint_rows, checkpoint_cols;  /* K, Kc actually used            */
+    double ms_forward;        /* stage 1 (forward pass with checkpoints)          */
+    double ms_end_search;     /* end-cell column search                           */
+    double ms_traceback;      /* block recomputation + walk                       */
+    double ms_total;          /* device work of the call                          */
+    int32_t precision_used;   /* 32 or 16                                        */
+    int32_t _pad;
+} swh_result;
+
+typedef struct swh_ctx swh_ctx;
+
+const char* swh_strerror(int status);
+
+/* Context = one device + one stream + cached device buffers.  Replaces the
+ * reference's device discovery and per-run buffers (src/SWutils.cpp:300-338,
+ * src/SWutils.cpp:144-173).  One context is not re-entrant. */
+int swh_ctx_create(int device, swh_ctx** out);
+int swh_ctx_destroy(swh_ctx* ctx);
+/* Device name, CU count and clock (kHz) of the context's device. */
+int swh_device_info(swh_ctx* ctx, char* name, int64_t name_cap, int32_t* cus, int32_t* clock_khz,
+                    int64_t* hbm_bytes);
+
+/* Full alignment of two host sequences: stage 1 forward with checkpoints, end-cell
+ * search and traceback.  Replaces the stage-1 driver and the crosspoint search of
+ * src/main.cu:158-248 and src/Crosspointer.cpp:104-338. */
+int swh_align_pair(swh_ctx* ctx, const uint8_t* seq1, int64_t len1, const uint8_t* seq2,
+                   int64_t len2, const swh_scoring* sc, const swh_opts* opts, swh_result* out);
+
+/* Same, for sequences already resident in device memory (d_seq1/d_seq2 are device
+ * pointers on the context's device).  No host<->device copy of the inputs. */
+int swh_align_pair_device(swh_ctx* ctx, const uint8_t* d_seq1, int64_t len1, const uint8_t* d_seq2,
+                          int64_t len2, const swh_scoring* sc, const swh_opts* opts,
+                          swh_result* out);
+
+void swh_result_free(swh_result* r);
+
+/* ---- stage-level entry points (tests and tools) ------------------------------------ */
+
+/* Stage 1 only: score, end cell and per-row maxima; optionally copies the checkpoint
+ * rows {H,F} (rows K-1, 2K-1, ... < m) and checkpoint columns {H,E} (columns Kc-1,
+ * 2Kc-1, ... < n) to host buffers.  The analogue of shortPhase/longPhase + RowBuilder
+ * (src/FindAlignment.cuh:9-278, src/Builders.cpp:27-104).  A/B are already oriented
+ * (m rows, n columns).  rowmax: m int32 (nullable); srow: (m/K)*n*2 int32 (nullable);
+ * scol: (n/Kc)*m*2 int32 (nullable). */
+int swh_forward(swh_ctx* ctx, const uint8_t* A, int64_t m, const uint8_t* B, int64_t n,
+                const swh_scoring* sc, int64_t K, int64_t Kc, int32_t* score, int64_t* end_row,
+                int64_t* end_col, int32_t* rowmax, int32_t* srow, int32_t* scol);
+
+/* ---- batched pairs (many pairs of uneven length in one persistent launch) -------- */
+typedef struct {
+    const uint8_t* seq1;
+    int64_t len1;
+    const uint8_t* seq2;
+    int64_t len2;
+} swh_pair;
+
+/* Aligns npairs pairs; out[npairs] receives one result each (free each with
+ * swh_result_free).  Per-pair errors are reported in statuses[npairs] (nullable)
+ * without failing the batch. */
+int swh_batch_align(swh_ctx* ctx, const swh_pair* pairs, int64_t npairs, const swh_scoring* sc,
+                    const swh_opts* opts, swh_result* out, int32_t* statuses);
+
+/* ---- synthetic workloads ------------------------------------------------------------ */
+/* Generates the synthetic pair used by the benchmarks (seeded std::mt19937_64):
+ * two uniform (or GC-biased) sequences of lengths len1, len2 sharing a homologous
+ * region of length hom_len with substitution rate p_sub and indel rate p_indel
+ * (geometric lengths, mean 3).  gc < 0 means uniform ACGT; repeat_frac > 0 plants
+ * tandem repeats (period 2..50) over that fraction of each sequence.  Buffers are
+ * allocated by the library; free with swh_free. */
+int swh_synth_pair(uint64_t seed, int64_t len1, int64_t len2, int64_t hom_len, double p_sub,
+                   double p_indel, double gc, double repeat_frac, uint8_t** seq1, uint8_t** seq2);
+/* Mutates `src` (substitutions p_sub, indels p_indel) and trims/extends both flanks by
+ * up to flank_frac of the length: the query model of the batched workload. */
+int swh_synth_mutate(uint64_t seed, const uint8_t* src, int64_t len, double p_sub, double p_indel,
+                     double flank_frac, uint8_t** out, int64_t* out_len);
+void swh_free(void* p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

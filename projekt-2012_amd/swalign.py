@@ -1,0 +1,287 @@
+"""ctypes binding of libswalign_hip.so (include/swalign_hip.h).
+
+The library is the HIP (gfx950) implementation of the SWalign hot path; this module
+only marshals arguments.  It raises if the built library is missing: there is no
+CPU fallback.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libswalign_hip.so")
+_LIB = None
+
+SWH_OK = 0
+SWH_ERR_INVALID_ARG = 1
+SWH_ERR_SCORING = 2
+SWH_ERR_NO_DEVICE = 3
+SWH_ERR_HIP = 4
+SWH_ERR_OUT_OF_MEMORY = 5
+SWH_ERR_BUDGET = 6
+SWH_ERR_INTERNAL = 7
+SWH_ERR_NOT_FOUND = 8
+
+
+class Scoring(C.Structure):
+    _fields_ = [("match", C.c_int32), ("mismatch", C.c_int32),
+                ("gap_open", C.c_int32), ("gap_extend", C.c_int32)]
+
+
+class Opts(C.Structure):
+    _fields_ = [("checkpoint_rows", C.c_int64), ("checkpoint_cols", C.c_int64),
+                ("hbm_budget_bytes", C.c_int64), ("score_only", C.c_int32),
+                ("precision", C.c_int32), ("want_xpoints", C.c_int32),
+                ("reserved", C.c_int32 * 5)]
+
+
+class XPoint(C.Structure):
+    _fields_ = [("row", C.c_int64), ("col", C.c_int64), ("score", C.c_int32), ("layer", C.c_int32)]
+
+
+class Result(C.Structure):
+    _fields_ = [("score", C.c_int32), ("swapped", C.c_int32),
+                ("end_row", C.c_int64), ("end_col", C.c_int64),
+                ("start_row", C.c_int64), ("start_col", C.c_int64),
+                ("m", C.c_int64), ("n", C.c_int64),
+                ("cigar", C.c_char_p), ("cigar_len", C.c_int64), ("n_ops", C.c_int64),
+                ("xpoints", C.POINTER(XPoint)), ("n_xpoints", C.c_int64),
+                ("checkpoint_rows", C.c_int64), ("checkpoint_cols", C.c_int64),
+                ("ms_forward", C.c_double), ("ms_end_search", C.c_double),
+                ("ms_traceback", C.c_double), ("ms_total", C.c_double),
+                ("precision_used", C.c_int32), ("_pad", C.c_int32)]
+
+
+class Pair(C.Structure):
+    _fields_ = [("seq1", C.c_void_p), ("len1", C.c_int64), ("seq2", C.c_void_p), ("len2", C.c_int64)]
+
+
+# (name, restype, argtypes) of every exported function declared in include/swalign_hip.h
+EXPORTS = [
+    ("swh_strerror", C.c_char_p, [C.c_int]),
+    ("swh_ctx_create", C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    ("swh_ctx_destroy", C.c_int, [C.c_void_p]),
+    ("swh_device_info", C.c_int, [C.c_void_p, C.c_char_p, C.c_int64, C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
+    ("swh_align_pair", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
+                                 C.POINTER(Scoring), C.POINTER(Opts), C.POINTER(Result)]),
+    ("swh_align_pair_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
+                                        C.POINTER(Scoring), C.POINTER(Opts), C.POINTER(Result)]),
+    ("swh_result_free", None, [C.POINTER(Result)]),
+    ("swh_forward", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
+                              C.POINTER(Scoring), C.c_int64, C.c_int64, C.POINTER(C.c_int32),
+                              C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.c_void_p, C.c_void_p,
+                              C.c_void_p]),
+    ("swh_batch_align", C.c_int, [C.c_void_p, C.POINTER(Pair), C.c_int64, C.POINTER(Scoring),
+                                  C.POINTER(Opts), C.POINTER(Result), C.POINTER(C.c_int32)]),
+    ("swh_synth_pair", C.c_int, [C.c_uint64, C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_double,
+                                 C.c_double, C.c_double, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
+    ("swh_synth_mutate", C.c_int, [C.c_uint64, C.c_void_p, C.c_int64, C.c_double, C.c_double,
+                                   C.c_double, C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]),
+    ("swh_free", None, [C.c_void_p]),
+]
+
+
+class SWError(RuntimeError):
+    def __init__(self, status, what=""):
+        self.status = status
+        msg = lib().swh_strerror(status).decode()
+        super().__init__("%s (status %d)%s" % (msg, status, (": " + what) if what else ""))
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libswalign_hip.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in EXPORTS:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _bytes(seq):
+    if isinstance(seq, str):
+        seq = seq.encode()
+    return bytes(seq)
+
+
+def scoring(match, mismatch, gap_open, gap_extend):
+    return Scoring(match, mismatch, gap_open, gap_extend)
+
+
+def _result_dict(r):
+    d = dict(score=r.score, swapped=bool(r.swapped), end=(r.end_row, r.end_col),
+             start=(r.start_row, r.start_col), m=r.m, n=r.n,
+             cigar=r.cigar.decode() if r.cigar else None, n_ops=r.n_ops,
+             checkpoint_rows=r.checkpoint_rows, checkpoint_cols=r.checkpoint_cols,
+             ms_forward=r.ms_forward, ms_end_search=r.ms_end_search, ms_traceback=r.ms_traceback,
+             ms_total=r.ms_total, precision=r.precision_used)
+    if r.xpoints and r.n_xpoints:
+        d["xpoints"] = [(r.xpoints[k].row, r.xpoints[k].col, r.xpoints[k].score, r.xpoints[k].layer)
+                        for k in range(r.n_xpoints)]
+    else:
+        d["xpoints"] = []
+    return d
+
+
+def make_opts(K=0, Kc=0, score_only=False, xpoints=False, budget=0, precision=0):
+    o = Opts()
+    o.checkpoint_rows = K
+    o.checkpoint_cols = Kc
+    o.hbm_budget_bytes = budget
+    o.score_only = 1 if score_only else 0
+    o.want_xpoints = 1 if xpoints else 0
+    o.precision = precision
+    return o
+
+
+class Context:
+    """One HIP device + stream + cached device buffers (swh_ctx)."""
+
+    def __init__(self, device=0):
+        self._h = C.c_void_p()
+        rc = lib().swh_ctx_create(device, C.byref(self._h))
+        if rc != SWH_OK:
+            raise SWError(rc, "swh_ctx_create")
+
+    def close(self):
+        if self._h:
+            lib().swh_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def device_info(self):
+        name = C.create_string_buffer(256)
+        cus = C.c_int32()
+        clk = C.c_int32()
+        hbm = C.c_int64()
+        lib().swh_device_info(self._h, name, 256, C.byref(cus), C.byref(clk), C.byref(hbm))
+        return dict(name=name.value.decode(), cus=cus.value, clock_khz=clk.value, hbm_bytes=hbm.value)
+
+    def align(self, seq1, seq2, sc, **kw):
+        s1, s2 = _bytes(seq1), _bytes(seq2)
+        r = Result()
+        o = make_opts(**kw)
+        rc = lib().swh_align_pair(self._h, s1, len(s1), s2, len(s2), C.byref(sc), C.byref(o), C.byref(r))
+        try:
+            if rc != SWH_OK:
+                raise SWError(rc, "swh_align_pair")
+            return _result_dict(r)
+        finally:
+            lib().swh_result_free(C.byref(r))
+
+    def align_device(self, ptr1, len1, ptr2, len2, sc, **kw):
+        """Sequences already in device memory (e.g. torch uint8 tensors' data_ptr())."""
+        r = Result()
+        o = make_opts(**kw)
+        rc = lib().swh_align_pair_device(self._h, C.c_void_p(ptr1), len1, C.c_void_p(ptr2), len2,
+                                         C.byref(sc), C.byref(o), C.byref(r))
+        try:
+            if rc != SWH_OK:
+                raise SWError(rc, "swh_align_pair_device")
+            return _result_dict(r)
+        finally:
+            lib().swh_result_free(C.byref(r))
+
+    def forward(self, A, B, sc, K, Kc, checkpoints=True):
+        """Stage 1 on oriented inputs (A rows, B columns); returns score, end, rowmax and
+        the checkpoint rows/columns as numpy arrays."""
+        import numpy as np
+        A, B = _bytes(A), _bytes(B)
+        m, n = len(A), len(B)
+        score = C.c_int32()
+        er = C.c_int64()
+        ec = C.c_int64()
+        rowmax = np.zeros(max(m, 1), np.int32)
+        nsr, nsc = m // K, n // Kc
+        srow = np.zeros((max(nsr, 1), n, 2), np.int32) if checkpoints else None
+        scol = np.zeros((max(nsc, 1), m, 2), np.int32) if checkpoints else None
+        p = lambda a: a.ctypes.data_as(C.c_void_p) if a is not None else None
+        rc = lib().swh_forward(self._h, A, m, B, n, C.byref(sc), K, Kc, C.byref(score), C.byref(er),
+                               C.byref(ec), p(rowmax), p(srow), p(scol))
+        if rc != SWH_OK:
+            raise SWError(rc, "swh_forward")
+        out = dict(score=score.value, end=(er.value, ec.value), rowmax=rowmax[:m])
+        if checkpoints:
+            out["srow"] = srow[:nsr]
+            out["scol"] = scol[:nsc]
+        return out
+
+    def batch(self, pairs, sc, **kw):
+        """pairs: list of (seq1, seq2).  Returns (results, statuses)."""
+        keep = []
+        arr = (Pair * len(pairs))()
+        for k, (a, b) in enumerate(pairs):
+            a, b = _bytes(a), _bytes(b)
+            ba = C.create_string_buffer(a, len(a))
+            bb = C.create_string_buffer(b, len(b))
+            keep += [ba, bb]
+            arr[k] = Pair(C.cast(ba, C.c_void_p), len(a), C.cast(bb, C.c_void_p), len(b))
+        res = (Result * len(pairs))()
+        st = (C.c_int32 * len(pairs))()
+        o = make_opts(**kw)
+        rc = lib().swh_batch_align(self._h, arr, len(pairs), C.byref(sc), C.byref(o), res, st)
+        if rc != SWH_OK:
+            raise SWError(rc, "swh_batch_align")
+        out = []
+        for k in range(len(pairs)):
+            out.append(_result_dict(res[k]) if st[k] == SWH_OK else None)
+            lib().swh_result_free(C.byref(res[k]))
+        return out, list(st)
+
+
+def synth_pair(seed, len1, len2, hom_len, p_sub, p_indel, gc=-1.0, repeat_frac=0.0):
+    a = C.c_void_p()
+    b = C.c_void_p()
+    rc = lib().swh_synth_pair(seed, len1, len2, hom_len, p_sub, p_indel, gc, repeat_frac, C.byref(a),
+                              C.byref(b))
+    if rc != SWH_OK:
+        raise SWError(rc, "swh_synth_pair")
+    try:
+        return C.string_at(a, len1), C.string_at(b, len2)
+    finally:
+        lib().swh_free(a)
+        lib().swh_free(b)
+
+
+def synth_mutate(seed, src, p_sub, p_indel, flank_frac):
+    src = _bytes(src)
+    out = C.c_void_p()
+    ln = C.c_int64()
+    rc = lib().swh_synth_mutate(seed, src, len(src), p_sub, p_indel, flank_frac, C.byref(out), C.byref(ln))
+    if rc != SWH_OK:
+        raise SWError(rc, "swh_synth_mutate")
+    try:
+        return C.string_at(out, ln.value)
+    finally:
+        lib().swh_free(out)
+
+
+# Synthetic workloads (seed = 2012 + workload index).
+def workload_pair(index):
+    """The single-pair workloads: 0 = 10 kbp x 10 kbp (3 kbp homology, 10%/2%),
+    1 = 1 Mbp x 1 Mbp (200 kbp homology, 8%/1.5%), 2 = 10 Mbp x 10 Mbp, GC 40%, tandem
+    repeats over 1%, 2 Mbp homology at 5%/1%."""
+    seed = 2012 + index
+    if index == 0:
+        return synth_pair(seed, 10_000, 10_000, 3_000, 0.10, 0.02), scoring(1, -3, -5, -2)
+    if index == 1:
+        return synth_pair(seed, 1_000_000, 1_000_000, 200_000, 0.08, 0.015), scoring(1, -3, -5, -2)
+    if index == 2:
+        return (synth_pair(seed, 10_000_000, 10_000_000, 2_000_000, 0.05, 0.01, gc=0.40, repeat_frac=0.01),
+                scoring(2, -3, -5, -1))
+    raise ValueError(index)

@@ -1,0 +1,113 @@
+"""Parity of the HIP library with the CPU oracle (run on an MI355X).
+
+Integer arithmetic: every comparison is exact (score, end cell, start cell, CIGAR,
+checkpoint rows/columns, row maxima).
+"""
+import json
+import os
+import random
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, fasta_bytes, orient
+
+pytestmark = pytest.mark.gpu
+
+
+def _cmp(got, ref, keys=("score", "end", "start", "cigar")):
+    for k in keys:
+        assert got[k] == ref[k], (k, got[k], ref[k])
+
+
+def test_smoke_small_pair(sw, orc, ctx):
+    (a, b), sc = sw.synth_pair(2012, 3000, 2400, 1200, 0.1, 0.02), sw.scoring(1, -3, -5, -2)
+    got = ctx.align(a, b, sc, K=512, Kc=256)
+    ref = orc.align_full(*orient(a, b), orc.scoring(1, -3, -5, -2))
+    _cmp(got, ref)
+
+
+def test_golden_small_pairs(sw, ctx):
+    pairs = json.load(open(os.path.join(GOLDEN, "small_pairs.json")))
+    for p in pairs:
+        got = ctx.align(p["a"], p["b"], sw.scoring(*p["scoring"]))
+        ref = dict(score=p["score"], end=tuple(p["end"]), start=tuple(p["start"]), cigar=p["cigar"])
+        if ref["score"] == 0:
+            assert got["score"] == 0
+            continue
+        _cmp(got, ref)
+
+
+@pytest.mark.parametrize("m,n", [(1, 1), (1, 7), (7, 1), (63, 64), (64, 64), (65, 63), (255, 100), (256, 256),
+                                 (257, 129), (300, 5), (513, 700), (1000, 1000), (2047, 64), (64, 2047)])
+def test_shapes(sw, orc, ctx, m, n):
+    rng = random.Random(m * 1000 + n)
+    a = bytes(rng.choice(b"ACGT") for _ in range(m))
+    b = bytes(rng.choice(b"ACGT") for _ in range(n))
+    if m > 30 and n > 30:
+        b = b[: n // 3] + a[m // 4: m // 4 + n // 3] + b[2 * n // 3:]
+        b = b[:n]
+    sc = sw.scoring(2, -3, -5, -1)
+    got = ctx.align(a, b, sc, K=256, Kc=64)
+    ref = orc.align_full(*orient(a, b), orc.scoring(2, -3, -5, -1))
+    if ref["score"] == 0:
+        assert got["score"] == 0
+    else:
+        _cmp(got, ref)
+
+
+def test_config0_golden(sw, ctx):
+    g = json.load(open(os.path.join(GOLDEN, "config0.json")))
+    (a, b), sc = sw.workload_pair(0)
+    got = ctx.align(a, b, sc)
+    _cmp(got, dict(score=g["score"], end=tuple(g["end"]), start=tuple(g["start"]), cigar=g["cigar"]))
+
+
+@pytest.mark.parametrize("K,Kc", [(256, 64), (512, 1024), (1024, 256), (8192, 8192)])
+def test_checkpoint_spacing_independent(sw, orc, ctx, K, Kc):
+    (a, b), sc = sw.synth_pair(77, 6000, 5000, 3000, 0.08, 0.015), sw.scoring(1, -3, -5, -2)
+    got = ctx.align(a, b, sc, K=K, Kc=Kc, xpoints=True)
+    A, B = orient(a, b)
+    ref = orc.align_linear(A, B, orc.scoring(1, -3, -5, -2), K, Kc)
+    _cmp(got, ref)
+    assert got["xpoints"] == [tuple(x) for x in ref["xpoints"]]
+
+
+def test_forward_checkpoints_bitexact(sw, orc, ctx):
+    (a, b), sc = sw.synth_pair(91, 3000, 2000, 1500, 0.1, 0.02), sw.scoring(1, -3, -5, -2)
+    A, B = orient(a, b)
+    K, Kc = 512, 128
+    got = ctx.forward(A, B, sc, K, Kc)
+    (s, i, j), rowmax = orc.forward(A, B, orc.scoring(1, -3, -5, -2), rowmax=True)
+    srow, scol = orc.checkpoints(A, B, orc.scoring(1, -3, -5, -2), K, Kc)
+    assert got["score"] == s and got["end"] == (i, j)
+    assert np.array_equal(got["rowmax"], rowmax)
+    assert np.array_equal(got["srow"], srow)
+    assert np.array_equal(got["scol"], scol)
+
+
+@pytest.mark.parametrize("name", ["test1", "test2", "test3"])
+def test_known_answers_small(sw, orc, ctx, name):
+    k = [x for x in json.load(open(os.path.join(GOLDEN, "kats.json"))) if x["name"] == name][0]
+    a, b = fasta_bytes(k["a"]), fasta_bytes(k["b"])
+    got = ctx.align(a, b, sw.scoring(*k["scoring"]))
+    assert got["score"] == k["score"]
+    assert (got["end"][0] + 1, got["end"][1] + 1) == tuple(k["end"])
+    A, B = orient(a, b)
+    if len(A) * len(B) <= 4_000_000:
+        _cmp(got, orc.align_full(A, B, orc.scoring(*k["scoring"])))
+    # the CIGAR re-scores to the score and ends at the end cell
+    s, end = orc.rescore(A, B, orc.scoring(*k["scoring"]), got["start"], got["cigar"])
+    assert s == got["score"] and end == got["end"]
+
+
+@pytest.mark.parametrize("name", ["test4", "test5"])
+def test_known_answers_large(sw, orc, ctx, name):
+    k = [x for x in json.load(open(os.path.join(GOLDEN, "kats.json"))) if x["name"] == name][0]
+    a, b = fasta_bytes(k["a"]), fasta_bytes(k["b"])
+    got = ctx.align(a, b, sw.scoring(*k["scoring"]))
+    assert got["score"] == k["score"]
+    assert (got["end"][0] + 1, got["end"][1] + 1) == tuple(k["end"])
+    A, B = orient(a, b)
+    s, end = orc.rescore(A, B, orc.scoring(*k["scoring"]), got["start"], got["cigar"])
+    assert s == got["score"] and end == got["end"]
