@@ -1,0 +1,157 @@
+"""Benchmark: SWalign on one MI355X.
+
+A step is one full alignment of one synthetic pair: forward sweep with checkpoints,
+end-cell search and traceback to the CIGAR.  The default workload is the 1 Mbp x 1 Mbp
+pair (200 kbp homologous region at 8% substitutions / 1.5% indels, scoring
++1/-3/-5/-2, checkpoint rows every 8192 rows).  Inputs are resident in device memory
+before timing starts.  value = m*n / (seconds per step) / 1e9 (GCUPS).
+
+Extra JSON fields:
+  roofline      stage-1 sweep kernel against the integer-VALU peak of the device
+                (12 integer ops per cell; duration from HIP events around the launch on
+                the library's stream);
+  cpu_baseline  the CPU oracle's forward pass over a bounded row band of the same pair,
+                on the host's cores.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "projekt-2012_amd"))
+
+OPS_PER_CELL = 12           # 2 substitution + 3 E + 3 F + 4 H integer ops per cell
+LANES_PER_CU_CLK = 64       # 4 SIMD x 16 lanes; one 32-bit VALU op per lane per clock
+PEAK_CUS = 256
+PEAK_CLOCK_GHZ = 2.4
+PEAK_INT32_TOPS = PEAK_CUS * LANES_PER_CU_CLK * PEAK_CLOCK_GHZ * 1e9 / 1e12   # 39.32
+
+WORKLOADS = {
+    0: "10 kbp x 10 kbp pair, 3 kbp homology (10% sub, 2% indel), +1/-3/-5/-2, full traceback",
+    1: "1 Mbp x 1 Mbp pair, 200 kbp homology (8% sub, 1.5% indel), +1/-3/-5/-2, "
+       "checkpoint rows every 8192, full traceback",
+    2: "10 Mbp x 10 Mbp pair, GC 40%, tandem repeats over 1%, 2 Mbp homology (5% sub, 1% indel), "
+       "+2/-3/-5/-1, full traceback",
+}
+
+
+def cpu_baseline(a, b, budget_s=15.0):
+    """Times the oracle's multithreaded forward pass over the first rows of the pair."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("pyoracle", os.path.join(ROOT, "oracle", "pyoracle.py"))
+    orc = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(orc)
+    A, B = (a, b) if len(a) >= len(b) else (b, a)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    sc = orc.scoring(1, -3, -5, -2)
+    # calibrate on a small band, then size the sample to about budget_s
+    rows = 512
+    t0 = time.perf_counter()
+    orc.forward(A[:rows], B, sc, threads=threads)
+    dt = time.perf_counter() - t0
+    rate = rows * len(B) / max(dt, 1e-6)
+    rows = int(min(len(A), max(1024, rate * budget_s / len(B))))
+    t0 = time.perf_counter()
+    orc.forward(A[:rows], B, sc, threads=threads)
+    dt = time.perf_counter() - t0
+    return dict(value=round(rows * len(B) / dt / 1e9, 3), unit="GCUPS", cores=threads, kind="port",
+                sample="oracle forward pass (score + end cell, no traceback) over the first %d rows x %d "
+                       "columns of the same pair, %d host threads, %.1f s" % (rows, len(B), threads, dt))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", type=int, default=1, choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one MI355X; the pair alignment is not sharded across "
+              "GPUs (see DESIGN.md for how it would shard). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import swalign as sw
+
+    (a, b), sc = sw.workload_pair(args.workload)
+    K = 8192
+    ctx = sw.Context(0)
+    info = ctx.device_info()
+    dev = torch.device("cuda:0")
+    da = torch.frombuffer(bytearray(a), dtype=torch.uint8).to(dev)
+    db = torch.frombuffer(bytearray(b), dtype=torch.uint8).to(dev)
+    torch.cuda.synchronize()
+
+    def step():
+        return ctx.align_device(da.data_ptr(), len(a), db.data_ptr(), len(b), sc, K=K)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    results = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        results.append(step())
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+
+    r0 = results[0]
+    for r in results[1:]:
+        assert (r["score"], r["end"], r["start"], r["cigar"]) == (r0["score"], r0["end"], r0["start"], r0["cigar"])
+    m, n = r0["m"], r0["n"]
+    cells = m * n
+    ms_step = elapsed * 1000.0 / args.steps
+    gcups = cells / (elapsed / args.steps) / 1e9
+    fwd_ms = statistics.mean(r["ms_forward"] for r in results)
+    achieved_tops = cells * OPS_PER_CELL / (fwd_ms / 1000.0) / 1e12
+    clock_ghz = info["clock_khz"] / 1e6 if info["clock_khz"] else PEAK_CLOCK_GHZ
+    line = {
+        "metric": "GCUPS on 1 MI355X (single long pair, batched mix) + served p50/p99 vs plain",
+        "value": round(gcups, 2),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (seeded std::mt19937_64, seed %d)" % (2012 + args.workload),
+        "config": {
+            "workload": WORKLOADS[args.workload],
+            "m": m, "n": n, "cells": cells,
+            "checkpoint_rows": r0["checkpoint_rows"], "checkpoint_cols": r0["checkpoint_cols"],
+            "score": r0["score"], "end": list(r0["end"]), "start": list(r0["start"]),
+            "cigar_ops": r0["n_ops"],
+            "stage_ms": {
+                "forward": round(fwd_ms, 3),
+                "end_search": round(statistics.mean(r["ms_end_search"] for r in results), 3),
+                "traceback": round(statistics.mean(r["ms_traceback"] for r in results), 3),
+            },
+            "forward_gcups": round(cells / (fwd_ms / 1000.0) / 1e9, 2),
+            "device": info["name"], "cus": info["cus"], "clock_mhz": round(clock_ghz * 1000),
+        },
+        "roofline": {
+            "bound": "valu",
+            "kernel": "k_sweep<4, kStage1> (forward sweep with checkpoints)",
+            "achieved": round(achieved_tops, 3),
+            "peak": round(PEAK_INT32_TOPS, 2),
+            "unit": "TOP/s",
+            "frac": round(achieved_tops / PEAK_INT32_TOPS, 4),
+            "traffic": None,
+        },
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(a, b)
+    print(json.dumps(line))
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -36,7 +36,7 @@ struct swh_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipDeviceProp_t props{};
-    DevBuf seqA, seqB, bnd, prog, rowmax, srow, scol, dirs, tiles, key, found, walk, ops, xps, ticket;
+    DevBuf seqA, seqB, bnd, prog, rowmax, srow, scol, dirs, tiles, key, found, walk, ops, xps, ticket, err;
     int occ_stage1 = 0, occ_dirs = 0, occ_search = 0;
     hipEvent_t ev[8] = {};
     hipError_t last = hipSuccess;
@@ -92,21 +92,41 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
-// Runs the sweep over `tiles` (already in host memory).
-int run_sweep(swh_ctx* c, int mode, SweepParams p, const std::vector<Tile>& tiles) {
+// Runs the sweep over `tiles` (already in host memory).  Optional events are recorded
+// immediately before and after the kernel launch.
+int run_sweep(swh_ctx* c, int mode, SweepParams p, const std::vector<Tile>& tiles, hipEvent_t ev_before = nullptr,
+              hipEvent_t ev_after = nullptr) {
     HCK(c, ensure(c->tiles, sizeof(Tile) * tiles.size()));
     HCK(c, hipMemcpyAsync(c->tiles.p, tiles.data(), sizeof(Tile) * tiles.size(), hipMemcpyHostToDevice,
                           c->stream));
     HCK(c, ensure(c->ticket, 64));
     HCK(c, hipMemsetAsync(c->ticket.p, 0, 8, c->stream));
+    HCK(c, ensure(c->err, 64));
+    HCK(c, hipMemsetAsync(c->err.p, 0, 32, c->stream));
     int64_t total = 0;
     for (auto& t : tiles) total += t.nstripes;
     p.tiles = (const Tile*)c->tiles.p;
     p.ntiles = (int32_t)tiles.size();
     p.total_stripes = total;
     p.ticket = (unsigned long long*)c->ticket.p;
+    p.err = (int32_t*)c->err.p;
+    if (p.spin_limit == 0) p.spin_limit = kDefaultSpinLimit;
     const int occ = mode == kStage1 ? c->occ_stage1 : (mode == kDirs ? c->occ_dirs : c->occ_search);
+    if (ev_before) HCK(c, hipEventRecord(ev_before, c->stream));
     HCK(c, launch_sweep(mode, p, grid_for(c, occ, total), c->stream));
+    if (ev_after) HCK(c, hipEventRecord(ev_after, c->stream));
+    return SWH_OK;
+}
+
+// Reads the sweep's failure record (after the stream has been synchronised).
+int check_sweep(swh_ctx* c) {
+    int32_t e[5] = {0, 0, 0, 0, 0};
+    HCK(c, hipMemcpy(e, c->err.p, sizeof e, hipMemcpyDeviceToHost));
+    if (e[0] != 0) {
+        fprintf(stderr, "swalign: stripe wait exceeded its bound (tile %d, stripe %d, need %d, seen %d)\n", e[1],
+                e[2], e[3], e[4]);
+        return SWH_ERR_INTERNAL;
+    }
     return SWH_OK;
 }
 
@@ -183,9 +203,9 @@ int align_core(swh_ctx* c, const uint8_t* dA, int64_t m, const uint8_t* dB, int6
     p.K = pl.K;
     p.scol = pl.nsc ? scol : nullptr;
     p.Kc = pl.Kc;
+    p.spin_limit = (o && o->wait_spin_limit > 0) ? (uint32_t)o->wait_spin_limit : kDefaultSpinLimit;
 
     // ---- stage 1 ------------------------------------------------------------------
-    HCK(c, hipEventRecord(c->ev[0], c->stream));
     HCK(c, hipMemsetAsync(c->prog.p, 0, sizeof(int32_t) * nstripes, c->stream));
     {
         Tile t{};
@@ -198,15 +218,15 @@ int align_core(swh_ctx* c, const uint8_t* dA, int64_t m, const uint8_t* dB, int6
         t.bnd = (int2*)c->bnd.p;
         t.prog = (int32_t*)c->prog.p;
         std::vector<Tile> tiles{t};
-        rc = run_sweep(c, kStage1, p, tiles);
+        rc = run_sweep(c, kStage1, p, tiles, c->ev[0], c->ev[1]);
         if (rc) return rc;
     }
     HCK(c, hipMemsetAsync(c->key.p, 0, 8, c->stream));
     HCK(c, launch_argmax_rows((const int32_t*)c->rowmax.p, m, (unsigned long long*)c->key.p, c->stream));
-    HCK(c, hipEventRecord(c->ev[1], c->stream));
     unsigned long long key = 0;
     HCK(c, hipMemcpyAsync(&key, c->key.p, 8, hipMemcpyDeviceToHost, c->stream));
     HCK(c, hipStreamSynchronize(c->stream));
+    if ((rc = check_sweep(c))) return rc;
     out->ms_forward = elapsed(c->ev[0], c->ev[1]);
     const int32_t best = (int32_t)(key >> 32);
     const int64_t bi = (int64_t)(0xFFFFFFFFu - (uint32_t)(key & 0xFFFFFFFFu));
@@ -216,8 +236,8 @@ int align_core(swh_ctx* c, const uint8_t* dA, int64_t m, const uint8_t* dB, int6
         return SWH_OK;
     }
 
-    // ---- end-cell column search -----------------------------------------------------
-    HCK(c, hipEventRecord(c->ev[2], c->stream));
+    // ---- end-cell column search (the argmax above is counted here) ---------------------
+    const hipEvent_t ev_search0 = c->ev[1];
     const int64_t br0 = (bi / pl.K) * pl.K;
     const int64_t band_rows = bi - br0 + 1;
     const int64_t sp = ceil_div(band_rows, kStripeRows);
@@ -253,11 +273,12 @@ int align_core(swh_ctx* c, const uint8_t* dA, int64_t m, const uint8_t* dB, int6
         HCK(c, hipMemcpyAsync(&bj, c->found.p, 4, hipMemcpyDeviceToHost, c->stream));
         HCK(c, hipEventRecord(c->ev[3], c->stream));
         HCK(c, hipStreamSynchronize(c->stream));
+        if ((rc = check_sweep(c))) return rc;
         if (bj == INT32_MAX || bj < 0) return SWH_ERR_INTERNAL;
         out->end_row = bi;
         out->end_col = bj;
     }
-    out->ms_end_search = elapsed(c->ev[2], c->ev[3]);
+    out->ms_end_search = elapsed(ev_search0, c->ev[3]);
     if (!want_tb) {
         out->ms_total = out->ms_forward + out->ms_end_search;
         return SWH_OK;
@@ -324,6 +345,7 @@ int align_core(swh_ctx* c, const uint8_t* dA, int64_t m, const uint8_t* dB, int6
         HCK(c, launch_walk(wa, c->stream));
         HCK(c, hipMemcpyAsync(&st, c->walk.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
         HCK(c, hipStreamSynchronize(c->stream));
+        if ((rc = check_sweep(c))) return rc;
         if (!st.done && (st.i >= r0 && st.j >= c0)) return SWH_ERR_INTERNAL;  // no progress
     }
     HCK(c, hipEventRecord(c->ev[5], c->stream));
@@ -419,7 +441,7 @@ int swh_ctx_destroy(swh_ctx* c) {
     if (!c) return SWH_ERR_INVALID_ARG;
     hipSetDevice(c->device);
     DevBuf* bufs[] = {&c->seqA, &c->seqB, &c->bnd, &c->prog, &c->rowmax, &c->srow, &c->scol, &c->dirs,
-                      &c->tiles, &c->key, &c->found, &c->walk, &c->ops, &c->xps, &c->ticket};
+                      &c->tiles, &c->key, &c->found, &c->walk, &c->ops, &c->xps, &c->ticket, &c->err};
     if (c->stream) hipStreamSynchronize(c->stream);
     for (DevBuf* b : bufs)
         if (b->p) hipFree(b->p);

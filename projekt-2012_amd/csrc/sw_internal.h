@@ -56,8 +56,14 @@ struct SweepParams {
     // kSearch
     int64_t target_row;
     int32_t target;
-    int32_t _pad2;
+    // Bounded waits: a wave that polls a predecessor's progress more than spin_limit
+    // times records {1, tile, stripe, need, seen} in err[0..4] and abandons its work;
+    // waves that see err[0] != 0 stop as well.  err must be zeroed before the launch.
+    uint32_t spin_limit;
+    int32_t* err;
 };
+
+constexpr uint32_t kDefaultSpinLimit = 1u << 20;
 
 struct WalkState {
     int64_t i, j;              // current cell (absolute)

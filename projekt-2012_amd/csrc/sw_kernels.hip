@@ -35,12 +35,12 @@ __device__ __forceinline__ int shr1(int old, int src) {
     return __builtin_amdgcn_update_dpp(old, src, kDppWaveShr1, 0xF, 0xF, false);
 }
 
-// Copies a wave-uniform value into a VGPR the compiler cannot see through, so that
-// per-step selects keep both operands in registers instead of re-materialising them.
+// Copies a wave-uniform value into a VGPR the compiler cannot see through (a DPP
+// row_mirror of a uniform value is the value itself), so that per-step selects keep
+// both operands in registers instead of re-materialising them every step.
+constexpr int kDppRowMirror = 0x140;
 __device__ __forceinline__ int to_vgpr(int v) {
-    int r;
-    asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "s"(v));
-    return r;
+    return __builtin_amdgcn_mov_dpp(v, kDppRowMirror, 0xF, 0xF, false);
 }
 
 __device__ __forceinline__ uint64_t pack2(int x, int y) {
@@ -57,13 +57,43 @@ __device__ __forceinline__ void store_coherent(int2* p, int x, int y) {
     __hip_atomic_store((uint64_t*)p, pack2(x, y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Waits until stripe `sl-1` of tile T has published at least `need` columns.
-__device__ __forceinline__ void wait_prog(const Tile& T, int64_t sl, int64_t need) {
-    if (sl == 0) return;
-    const int32_t* p = T.prog + (sl - 1);
-    while (__hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < need)
+// Wave-uniform reads of the cross-wave flags: every lane takes lane 0's value, so all
+// lanes leave a wait loop on the same iteration.
+__device__ __forceinline__ int32_t load_flag_uniform(const int32_t* q) {
+    return (int32_t)__builtin_amdgcn_readfirstlane(
+        (uint32_t)__hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+__device__ __forceinline__ bool sweep_failed(const SweepParams& p) { return load_flag_uniform(p.err) != 0; }
+
+// Waits until stripe `sl-1` of tile T has published at least `need` columns.  The wait
+// is bounded: after p.spin_limit polls (or once another wave has failed) it records the
+// failure and returns false.  Every lane reads the same values, so the outcome is
+// uniform across the wave.
+__device__ __forceinline__ bool wait_prog(const SweepParams& p, const Tile& T, int tile_idx, int64_t sl,
+                                          int64_t need) {
+    if (sl == 0) return true;
+    const int32_t* q = T.prog + (sl - 1);
+    uint32_t spins = 0;
+    int32_t seen;
+    while ((seen = load_flag_uniform(q)) < need) {
+        if (++spins > p.spin_limit || sweep_failed(p)) {
+            if (threadIdx.x == 0) {
+                int32_t expected = 0;
+                if (__hip_atomic_compare_exchange_strong(p.err, &expected, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT)) {
+                    p.err[1] = tile_idx;
+                    p.err[2] = (int32_t)sl;
+                    p.err[3] = (int32_t)need;
+                    p.err[4] = seen;
+                }
+            }
+            return false;
+        }
         __builtin_amdgcn_s_sleep(2);
+    }
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    return true;
 }
 
 // Publishes `cols` columns of stripe sl after all of this wave's boundary stores have
@@ -181,9 +211,19 @@ __device__ __forceinline__ void chunk_steps(const SweepParams& p, const Tile& T,
     }
 }
 
+// Per-wave LDS of the sweep: lane 0's top-boundary inputs and the column bytes of the
+// current and next chunk, and lane 63's bottom-row outputs of the current chunk.
+struct SweepLds {
+    int2 in[2][64];
+    uint32_t b[2][128];
+    int2 out[64];
+};
+
 template <int R, int MODE>
-__device__ void process_stripe(const SweepParams& p, const Tile& T, int64_t sl, int2 (*s_in)[64],
-                               uint32_t (*s_b)[128], int2* s_out) {
+__device__ void process_stripe(const SweepParams& p, const Tile& T, int tile_idx, int64_t sl, SweepLds& lds) {
+    int2 (*s_in)[64] = lds.in;
+    uint32_t (*s_b)[128] = lds.b;
+    int2* s_out = lds.out;
     const int lane = threadIdx.x;
     const int O = p.sc.O;
     constexpr int64_t HR = 64 * R;
@@ -221,18 +261,22 @@ __device__ void process_stripe(const SweepParams& p, const Tile& T, int64_t sl, 
         if (brow < p.m && (brow + 1) % p.K == 0) srow_idx = (brow + 1) / p.K - 1;
     }
 
-    wait_prog(T, sl, min((int64_t)64, w));
+    if (!wait_prog(p, T, tile_idx, sl, min((int64_t)64, w))) return;
     int2 in_reg;
     uint32_t b0, b1;
     load_chunk_inputs(p, T, sl, 0, w, in_reg, b0, b1);
     s_in[0][lane] = in_reg;
     s_b[0][lane] = b0;
     s_b[0][64 + lane] = b1;
+    // LDS handoffs between lanes of the wave (here, after lane 63's last output of a
+    // chunk and after the next chunk's inputs) need no hardware barrier, the wave runs
+    // in lockstep, but must not be reordered by the compiler.
+    __builtin_amdgcn_wave_barrier();
 
     for (int64_t c = 0; c < nchunks; ++c) {
         const int cur = (int)(c & 1);
         if (c + 1 < nchunks) {
-            wait_prog(T, sl, min(64 * (c + 2), w));
+            if (!wait_prog(p, T, tile_idx, sl, min(64 * (c + 2), w))) return;
             load_chunk_inputs(p, T, sl, c + 1, w, in_reg, b0, b1);
         }
         const bool interior = c >= 1 && c <= last_interior;
@@ -254,6 +298,7 @@ __device__ void process_stripe(const SweepParams& p, const Tile& T, int64_t sl, 
                                              s_in[cur], s_b[cur], s_out, found_col);
         }
         // Bottom row of the stripe for columns 64c-63+lane.
+        __builtin_amdgcn_wave_barrier();
         {
             const int2 o = s_out[lane];
             const int64_t col = 64 * c + lane - 63;
@@ -269,6 +314,7 @@ __device__ void process_stripe(const SweepParams& p, const Tile& T, int64_t sl, 
             s_b[cur ^ 1][lane] = b0;
             s_b[cur ^ 1][64 + lane] = b1;
         }
+        __builtin_amdgcn_wave_barrier();
     }
     if (sl + 1 < T.nstripes) publish(T, sl, w);
     if (MODE == kStage1) {
@@ -281,18 +327,16 @@ __device__ void process_stripe(const SweepParams& p, const Tile& T, int64_t sl, 
     }
 }
 
+// Persistent sweep: each wave takes stripes by ticket until none are left.
 template <int R, int MODE>
-__global__ __launch_bounds__(64) void k_sweep(SweepParams p) {
-    __shared__ int2 s_in[2][64];
-    __shared__ uint32_t s_b[2][128];
-    __shared__ int2 s_out[64];
+__device__ void sweep_main(const SweepParams& p, SweepLds& lds) {
     while (true) {
         unsigned long long g = 0;
         if (threadIdx.x == 0) g = atomicAdd(p.ticket, 1ull);
         const uint32_t glo = __builtin_amdgcn_readfirstlane((uint32_t)g);
         const uint32_t ghi = __builtin_amdgcn_readfirstlane((uint32_t)(g >> 32));
         const int64_t gi = (int64_t)(((uint64_t)ghi << 32) | glo);
-        if (gi >= p.total_stripes) break;
+        if (gi >= p.total_stripes || sweep_failed(p)) break;
         int lo = 0, hi = p.ntiles - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -300,12 +344,17 @@ __global__ __launch_bounds__(64) void k_sweep(SweepParams p) {
             else hi = mid - 1;
         }
         const Tile T = p.tiles[lo];
-        process_stripe<R, MODE>(p, T, gi - T.stripe0, s_in, s_b, s_out);
+        process_stripe<R, MODE>(p, T, lo, gi - T.stripe0, lds);
     }
 }
 
-__global__ __launch_bounds__(256) void k_argmax_rows(const int32_t* __restrict__ rowmax, int64_t m,
-                                                     unsigned long long* key) {
+template <int R, int MODE>
+__global__ __launch_bounds__(64) void k_sweep(SweepParams p) {
+    __shared__ SweepLds lds;
+    sweep_main<R, MODE>(p, lds);
+}
+
+__device__ void argmax_rows_main(const int32_t* __restrict__ rowmax, int64_t m, unsigned long long* key) {
     unsigned long long best = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
         // larger score first, then smaller row
@@ -320,14 +369,27 @@ __global__ __launch_bounds__(256) void k_argmax_rows(const int32_t* __restrict__
     if ((threadIdx.x & 63) == 0) atomicMax(key, best);
 }
 
+__global__ __launch_bounds__(256) void k_argmax_rows(const int32_t* __restrict__ rowmax, int64_t m,
+                                                     unsigned long long* key) {
+    argmax_rows_main(rowmax, m, key);
+}
+
 // Greedy backward walk over one recomputed region (see WalkArgs).  The region's
 // direction codes are staged through LDS in 128x128-cell windows; lane 0 walks.
+constexpr int kWalkWinRows = 128, kWalkWinCols = 128;
+
 template <int R>
-__global__ __launch_bounds__(64) void k_walk(WalkArgs a) {
-    constexpr int WR = 128, WC = 128, NG = WR / R;
+struct WalkLds {
+    uint16_t win[kWalkWinRows / R][kWalkWinCols];
+    WalkState sst;
+};
+
+template <int R>
+__device__ void walk_main(const WalkArgs& a, WalkLds<R>& lds) {
+    constexpr int WR = kWalkWinRows, WC = kWalkWinCols, NG = WR / R;
     constexpr int64_t HR = 64 * R;
-    __shared__ uint16_t win[NG][WC];
-    __shared__ WalkState sst;
+    auto& win = lds.win;
+    WalkState& sst = lds.sst;
     const int lane = threadIdx.x;
     if (lane == 0) sst = *a.st;
     __syncthreads();
@@ -408,6 +470,12 @@ __global__ __launch_bounds__(64) void k_walk(WalkArgs a) {
     }
     __syncthreads();
     if (lane == 0) *a.st = sst;
+}
+
+template <int R>
+__global__ __launch_bounds__(64) void k_walk(WalkArgs a) {
+    __shared__ WalkLds<R> lds;
+    walk_main<R>(a, lds);
 }
 
 }  // namespace
