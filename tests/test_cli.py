@@ -1,0 +1,59 @@
+"""The SWalign command line (drop-in for `SWalign <seq1> <seq2> <match> <mismatch>
+<gap_open> <gap_extend>`): argument and input errors behave like the reference
+(src/main.cu:107-117, src/SWutils.cpp:13-17, exit status 255 = -1); on a GPU the known
+answer line of Tests/test1.sh is reproduced."""
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import GOLDEN, ROOT
+
+CLI = os.path.join(ROOT, "projekt-2012_amd", "bin", "SWalign")
+FA = os.path.join(GOLDEN, "fasta")
+
+
+def run(*args):
+    return subprocess.run([CLI, *args], capture_output=True, text=True, timeout=600)
+
+
+def test_wrong_argument_count():
+    p = run("a.fna", "b.fna", "1")
+    assert p.returncode == 255
+    assert "Expected 6 input arguments, not 3!" in p.stdout
+    assert p.stdout.startswith("### Welcome to SWalign v0.6.3")
+
+
+def test_missing_input_file(tmp_path):
+    p = run(str(tmp_path / "nope.fna"), os.path.join(FA, "seq0.fna"), "1", "-3", "-5", "-2")
+    assert p.returncode == 255
+    assert "ERROR\nAn error has occured while loading input sequences.\n" in p.stdout
+
+
+def test_header_without_marker(tmp_path):
+    f = tmp_path / "bad.fna"
+    f.write_text("ACGT\n")
+    p = run(str(f), os.path.join(FA, "seq0.fna"), "1", "-3", "-5", "-2")
+    assert p.returncode == 255 and "ERROR" in p.stdout
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a HIP device is present")
+def test_no_device_exit_status():
+    p = run(os.path.join(FA, "seq0.fna"), os.path.join(FA, "seq1.fna"), "1", "-3", "-5", "-2")
+    assert p.returncode == 254 and "No HIP capable device detected." in p.stdout
+
+
+@pytest.mark.gpu
+def test_known_answer_line_test1():
+    p = run(os.path.join(FA, "A00142.fna"), os.path.join(FA, "A00145.fna"), "5", "-5", "-5", "-5", "--json")
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert "\t>Alignment score: 480 at [623, 561]" in p.stdout
+    assert re.search(r"\t>CIGAR: [0-9=XID]+\n", p.stdout)
+
+
+@pytest.mark.gpu
+def test_invalid_scoring_after_device_setup():
+    p = run(os.path.join(FA, "seq0.fna"), os.path.join(FA, "seq1.fna"), "1", "3", "-5", "-2")
+    assert p.returncode == 255
+    assert "ERROR\nOne or more scoring values were not usable!" in p.stdout
