@@ -62,8 +62,8 @@ typedef struct {
     int32_t score_only;        /* 1: score + end cell only (no traceback)                   */
     int32_t precision;         /* 0 = auto, 32 = int32; batch path: 16 = packed int16 first  */
     int32_t want_xpoints;      /* 1: return the path nodes on checkpoint rows                */
-    int32_t wait_spin_limit;   /* polls a stripe waits for its predecessor before the call    */
-                               /* fails with SWH_ERR_INTERNAL; 0 = default (about 1e6)        */
+    int32_t tile_cols;         /* columns per sweep tile (rounded up to a multiple of 64);     */
+                               /* 0 = auto (about four tiles per CU across the row, >= 256)   */
     int32_t reserved[4];
 } swh_opts;
 

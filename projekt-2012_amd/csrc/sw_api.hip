@@ -3,6 +3,7 @@
 // (src/main.cu:158-248) and its Crosspointer (src/Crosspointer.cpp:104-338):
 //   1. forward sweep over the whole matrix, keeping checkpoint rows {H,F} every K rows
 //      and checkpoint columns {H,E} every Kc columns in HBM, plus per-row maxima;
+//      a sweep is a sequence of launches, one per anti-diagonal of its tile grid;
 //   2. argmax over the row maxima (smallest row holding the score);
 //   3. end-cell column search: the band of blocks above that row is recomputed exactly
 //      from its checkpoints, all column blocks in parallel;
@@ -36,8 +37,7 @@ struct swh_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipDeviceProp_t props{};
-    DevBuf seqA, seqB, bnd, prog, rowmax, srow, scol, dirs, tiles, key, found, walk, ops, xps, ticket, err;
-    int occ_stage1 = 0, occ_dirs = 0, occ_search = 0;
+    DevBuf seqA, seqB, rbuf, cb, cor, rowmax, srow, scol, dirs, regions, key, found, walk, ops, xps;
     hipEvent_t ev[8] = {};
     hipError_t last = hipSuccess;
 };
@@ -81,52 +81,61 @@ bool scoring_ok(const swh_scoring* s) {
     return s && s->match >= 1 && s->mismatch <= -1 && s->gap_open <= -1 && s->gap_extend <= -1;
 }
 
-int grid_for(swh_ctx* c, int occ, int64_t stripes) {
-    int64_t cap = (int64_t)std::max(occ, 1) * c->props.multiProcessorCount;
-    return (int)std::max<int64_t>(1, std::min<int64_t>(stripes, cap));
-}
-
 float elapsed(hipEvent_t a, hipEvent_t b) {
     float ms = 0.f;
     hipEventElapsedTime(&ms, a, b);
     return ms;
 }
 
-// Runs the sweep over `tiles` (already in host memory).  Optional events are recorded
-// immediately before and after the kernel launch.
-int run_sweep(swh_ctx* c, int mode, SweepParams p, const std::vector<Tile>& tiles, hipEvent_t ev_before = nullptr,
-              hipEvent_t ev_after = nullptr) {
-    HCK(c, ensure(c->tiles, sizeof(Tile) * tiles.size()));
-    HCK(c, hipMemcpyAsync(c->tiles.p, tiles.data(), sizeof(Tile) * tiles.size(), hipMemcpyHostToDevice,
-                          c->stream));
-    HCK(c, ensure(c->ticket, 64));
-    HCK(c, hipMemsetAsync(c->ticket.p, 0, 8, c->stream));
-    HCK(c, ensure(c->err, 64));
-    HCK(c, hipMemsetAsync(c->err.p, 0, 32, c->stream));
-    int64_t total = 0;
-    for (auto& t : tiles) total += t.nstripes;
-    p.tiles = (const Tile*)c->tiles.p;
-    p.ntiles = (int32_t)tiles.size();
-    p.total_stripes = total;
-    p.ticket = (unsigned long long*)c->ticket.p;
-    p.err = (int32_t*)c->err.p;
-    if (p.spin_limit == 0) p.spin_limit = kDefaultSpinLimit;
-    const int occ = mode == kStage1 ? c->occ_stage1 : (mode == kDirs ? c->occ_dirs : c->occ_search);
-    if (ev_before) HCK(c, hipEventRecord(ev_before, c->stream));
-    HCK(c, launch_sweep(mode, p, grid_for(c, occ, total), c->stream));
-    if (ev_after) HCK(c, hipEventRecord(ev_after, c->stream));
-    return SWH_OK;
+// Columns per tile: `want` if given (rounded up to a multiple of 64), else about four
+// tiles per CU across `cols` columns, at least 256.
+int64_t tile_cols(swh_ctx* c, int64_t cols, int64_t want) {
+    if (want > 0) return ceil_div(want, 64) * 64;
+    const int64_t target = 4 * std::max(1, c->props.multiProcessorCount);
+    return std::max<int64_t>(256, ceil_div(ceil_div(cols, target), 64) * 64);
 }
 
-// Reads the sweep's failure record (after the stream has been synchronised).
-int check_sweep(swh_ctx* c) {
-    int32_t e[5] = {0, 0, 0, 0, 0};
-    HCK(c, hipMemcpy(e, c->err.p, sizeof e, hipMemcpyDeviceToHost));
-    if (e[0] != 0) {
-        fprintf(stderr, "swalign: stripe wait exceeded its bound (tile %d, stripe %d, need %d, seen %d)\n", e[1],
-                e[2], e[3], e[4]);
-        return SWH_ERR_INTERNAL;
+// Runs a sweep over `regions` with tiles of kStripeRows x W: one launch per anti-diagonal
+// of the tile grid, all regions side by side.  Optional events are recorded immediately
+// before the first and after the last launch.
+int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions, int64_t W,
+              hipEvent_t ev_before = nullptr, hipEvent_t ev_after = nullptr) {
+    int64_t S = 0, G = 0, cb_elems = 0, cor_elems = 0;
+    for (const Region& r : regions) {
+        const int64_t rows = r.r1 - r.r0 + 1, cols = r.c1 - r.c0 + 1;
+        if (rows <= 0 || cols <= 0) return SWH_ERR_INTERNAL;
+        S = std::max(S, ceil_div(rows, kStripeRows));
+        G = std::max(G, ceil_div(cols, W));
+        cb_elems += 2 * rows;
     }
+    cor_elems = 2 * S * (int64_t)regions.size();
+    HCK(c, ensure(c->cb, sizeof(int2) * cb_elems));
+    HCK(c, ensure(c->cor, sizeof(int32_t) * cor_elems));
+    HCK(c, ensure(c->rbuf, sizeof(int2) * 2 * p.n));
+    int64_t cbo = 0;
+    for (size_t q = 0; q < regions.size(); ++q) {
+        regions[q].cb = (int2*)c->cb.p + cbo;
+        regions[q].cor = (int32_t*)c->cor.p + 2 * S * (int64_t)q;
+        cbo += 2 * (regions[q].r1 - regions[q].r0 + 1);
+    }
+    HCK(c, ensure(c->regions, sizeof(Region) * regions.size()));
+    HCK(c, hipMemcpyAsync(c->regions.p, regions.data(), sizeof(Region) * regions.size(), hipMemcpyHostToDevice,
+                          c->stream));
+    p.regions = (const Region*)c->regions.p;
+    p.nregions = (int32_t)regions.size();
+    p.S = S;
+    p.G = G;
+    p.W = W;
+    p.rbuf = (int2*)c->rbuf.p;
+    if (ev_before) HCK(c, hipEventRecord(ev_before, c->stream));
+    for (int64_t d = 0; d < S + G - 1; ++d) {
+        const int64_t s_lo = std::max<int64_t>(0, d - (G - 1)), s_hi = std::min(S - 1, d);
+        p.d = d;
+        p.s_lo = s_lo;
+        p.per = (int32_t)(s_hi - s_lo + 1);
+        HCK(c, launch_sweep(mode, p, c->stream));
+    }
+    if (ev_after) HCK(c, hipEventRecord(ev_after, c->stream));
     return SWH_OK;
 }
 
@@ -180,9 +189,7 @@ int align_core(swh_ctx* c, const uint8_t* dA, int64_t m, const uint8_t* dB, int6
     out->checkpoint_rows = pl.nsr ? pl.K : 0;
     out->checkpoint_cols = pl.nsc ? pl.Kc : 0;
 
-    const int64_t nstripes = ceil_div(m, kStripeRows);
-    HCK(c, ensure(c->bnd, sizeof(int2) * n));
-    HCK(c, ensure(c->prog, sizeof(int32_t) * std::max<int64_t>(nstripes, 1) + 64));
+    const int64_t want_w = o ? o->tile_cols : 0;
     HCK(c, ensure(c->rowmax, sizeof(int32_t) * m));
     HCK(c, ensure(c->srow, sizeof(int2) * std::max<int64_t>(pl.nsr * n, 1)));
     HCK(c, ensure(c->scol, sizeof(int2) * std::max<int64_t>(pl.nsc * m, 1)));
@@ -203,22 +210,16 @@ int align_core(swh_ctx* c, const uint8_t* dA, int64_t m, const uint8_t* dB, int6
     p.K = pl.K;
     p.scol = pl.nsc ? scol : nullptr;
     p.Kc = pl.Kc;
-    p.spin_limit = (o && o->wait_spin_limit > 0) ? (uint32_t)o->wait_spin_limit : kDefaultSpinLimit;
 
     // ---- stage 1 ------------------------------------------------------------------
-    HCK(c, hipMemsetAsync(c->prog.p, 0, sizeof(int32_t) * nstripes, c->stream));
     {
-        Tile t{};
-        t.r0 = 0;
-        t.r1 = m - 1;
-        t.c0 = 0;
-        t.c1 = n - 1;
-        t.stripe0 = 0;
-        t.nstripes = nstripes;
-        t.bnd = (int2*)c->bnd.p;
-        t.prog = (int32_t*)c->prog.p;
-        std::vector<Tile> tiles{t};
-        rc = run_sweep(c, kStage1, p, tiles, c->ev[0], c->ev[1]);
+        Region r{};
+        r.r0 = 0;
+        r.r1 = m - 1;
+        r.c0 = 0;
+        r.c1 = n - 1;
+        std::vector<Region> regions{r};
+        rc = run_sweep(c, kStage1, p, regions, tile_cols(c, n, want_w), c->ev[0], c->ev[1]);
         if (rc) return rc;
     }
     HCK(c, hipMemsetAsync(c->key.p, 0, 8, c->stream));
@@ -226,7 +227,6 @@ int align_core(swh_ctx* c, const uint8_t* dA, int64_t m, const uint8_t* dB, int6
     unsigned long long key = 0;
     HCK(c, hipMemcpyAsync(&key, c->key.p, 8, hipMemcpyDeviceToHost, c->stream));
     HCK(c, hipStreamSynchronize(c->stream));
-    if ((rc = check_sweep(c))) return rc;
     out->ms_forward = elapsed(c->ev[0], c->ev[1]);
     const int32_t best = (int32_t)(key >> 32);
     const int64_t bi = (int64_t)(0xFFFFFFFFu - (uint32_t)(key & 0xFFFFFFFFu));
@@ -239,41 +239,33 @@ int align_core(swh_ctx* c, const uint8_t* dA, int64_t m, const uint8_t* dB, int6
     // ---- end-cell column search (the argmax above is counted here) ---------------------
     const hipEvent_t ev_search0 = c->ev[1];
     const int64_t br0 = (bi / pl.K) * pl.K;
-    const int64_t band_rows = bi - br0 + 1;
-    const int64_t sp = ceil_div(band_rows, kStripeRows);
     const int64_t nq = ceil_div(n, pl.Kc);
-    HCK(c, ensure(c->prog, sizeof(int32_t) * std::max<int64_t>(nstripes, sp * nq) + 64));
     {
-        std::vector<Tile> tiles;
-        tiles.reserve(nq);
+        // one region per checkpoint-column block: blocks start from their own checkpoints
+        std::vector<Region> regions;
+        regions.reserve(nq);
         for (int64_t q = 0; q < nq; ++q) {
-            Tile t{};
-            t.r0 = br0;
-            t.r1 = bi;
-            t.c0 = q * pl.Kc;
-            t.c1 = std::min(n, t.c0 + pl.Kc) - 1;
-            t.stripe0 = q * sp;
-            t.nstripes = sp;
-            t.bnd = (int2*)c->bnd.p + t.c0;
-            t.prog = (int32_t*)c->prog.p + q * sp;
-            t.top = br0 > 0 ? srow + (br0 / pl.K - 1) * n + t.c0 : nullptr;
-            t.left = t.c0 > 0 ? scol + (t.c0 / pl.Kc - 1) * m : nullptr;
-            t.found = (int32_t*)c->found.p;
-            tiles.push_back(t);
+            Region r{};
+            r.r0 = br0;
+            r.r1 = bi;
+            r.c0 = q * pl.Kc;
+            r.c1 = std::min(n, r.c0 + pl.Kc) - 1;
+            r.top = br0 > 0 ? srow + (br0 / pl.K - 1) * n + r.c0 : nullptr;
+            r.left = r.c0 > 0 ? scol + (r.c0 / pl.Kc - 1) * m : nullptr;
+            r.found = (int32_t*)c->found.p;
+            regions.push_back(r);
         }
         const int32_t big = INT32_MAX;
         HCK(c, hipMemcpyAsync(c->found.p, &big, 4, hipMemcpyHostToDevice, c->stream));
-        HCK(c, hipMemsetAsync(c->prog.p, 0, sizeof(int32_t) * sp * nq, c->stream));
         SweepParams ps = p;
         ps.target_row = bi;
         ps.target = best;
-        rc = run_sweep(c, kSearch, ps, tiles);
+        rc = run_sweep(c, kSearch, ps, regions, tile_cols(c, std::min(n, pl.Kc), want_w));
         if (rc) return rc;
         int32_t bj = -1;
         HCK(c, hipMemcpyAsync(&bj, c->found.p, 4, hipMemcpyDeviceToHost, c->stream));
         HCK(c, hipEventRecord(c->ev[3], c->stream));
         HCK(c, hipStreamSynchronize(c->stream));
-        if ((rc = check_sweep(c))) return rc;
         if (bj == INT32_MAX || bj < 0) return SWH_ERR_INTERNAL;
         out->end_row = bi;
         out->end_col = bj;
@@ -305,32 +297,30 @@ int align_core(swh_ctx* c, const uint8_t* dA, int64_t m, const uint8_t* dB, int6
             return SWH_ERR_INTERNAL;
         const int64_t r0 = (st.i / pl.K) * pl.K, c0 = (st.j / pl.Kc) * pl.Kc;
         const int64_t rows = st.i - r0 + 1, cols = st.j - c0 + 1;
-        const int64_t ns = ceil_div(rows, kStripeRows);
-        const int64_t tstride = ceil_div(cols + 63, 64) * 64;
-        HCK(c, ensure(c->dirs, sizeof(uint16_t) * (size_t)(ns * tstride * 64)));
-        HCK(c, ensure(c->prog, sizeof(int32_t) * std::max<int64_t>({nstripes, sp * nq, ns}) + 64));
-        HCK(c, hipMemsetAsync(c->prog.p, 0, sizeof(int32_t) * ns, c->stream));
-        Tile t{};
-        t.r0 = r0;
-        t.r1 = st.i;
-        t.c0 = c0;
-        t.c1 = st.j;
-        t.stripe0 = 0;
-        t.nstripes = ns;
-        t.bnd = (int2*)c->bnd.p;
-        t.prog = (int32_t*)c->prog.p;
-        t.top = r0 > 0 ? srow + (r0 / pl.K - 1) * n + c0 : nullptr;
-        t.left = c0 > 0 ? scol + (c0 / pl.Kc - 1) * m : nullptr;
-        t.dirs = (uint16_t*)c->dirs.p;
-        t.tstride = tstride;
-        std::vector<Tile> tiles{t};
-        rc = run_sweep(c, kDirs, p, tiles);
+        const int64_t W = tile_cols(c, cols, want_w);
+        const int64_t ns = ceil_div(rows, kStripeRows), ng = ceil_div(cols, W);
+        const int64_t tstride = ceil_div(W + 63, 64) * 64;
+        HCK(c, ensure(c->dirs, sizeof(uint16_t) * (size_t)(ns * ng * tstride * 64)));
+        Region r{};
+        r.r0 = r0;
+        r.r1 = st.i;
+        r.c0 = c0;
+        r.c1 = st.j;
+        r.top = r0 > 0 ? srow + (r0 / pl.K - 1) * n + c0 : nullptr;
+        r.left = c0 > 0 ? scol + (c0 / pl.Kc - 1) * m : nullptr;
+        r.dirs = (uint16_t*)c->dirs.p;
+        std::vector<Region> regions{r};
+        SweepParams pd = p;
+        pd.tstride = tstride;
+        rc = run_sweep(c, kDirs, pd, regions, W);
         if (rc) return rc;
         WalkArgs wa{};
         wa.A = dA;
         wa.B = dB;
         wa.dirs = (const uint16_t*)c->dirs.p;
         wa.tstride = tstride;
+        wa.G = ng;
+        wa.W = W;
         wa.r0 = r0;
         wa.c0 = c0;
         wa.rows = rows;
@@ -345,7 +335,6 @@ int align_core(swh_ctx* c, const uint8_t* dA, int64_t m, const uint8_t* dB, int6
         HCK(c, launch_walk(wa, c->stream));
         HCK(c, hipMemcpyAsync(&st, c->walk.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
         HCK(c, hipStreamSynchronize(c->stream));
-        if ((rc = check_sweep(c))) return rc;
         if (!st.done && (st.i >= r0 && st.j >= c0)) return SWH_ERR_INTERNAL;  // no progress
     }
     HCK(c, hipEventRecord(c->ev[5], c->stream));
@@ -430,9 +419,6 @@ int swh_ctx_create(int device, swh_ctx** out) {
         delete c;
         return SWH_ERR_HIP;
     }
-    c->occ_stage1 = sweep_occupancy(kStage1);
-    c->occ_dirs = sweep_occupancy(kDirs);
-    c->occ_search = sweep_occupancy(kSearch);
     *out = c;
     return SWH_OK;
 }
@@ -440,8 +426,8 @@ int swh_ctx_create(int device, swh_ctx** out) {
 int swh_ctx_destroy(swh_ctx* c) {
     if (!c) return SWH_ERR_INVALID_ARG;
     hipSetDevice(c->device);
-    DevBuf* bufs[] = {&c->seqA, &c->seqB, &c->bnd, &c->prog, &c->rowmax, &c->srow, &c->scol, &c->dirs,
-                      &c->tiles, &c->key, &c->found, &c->walk, &c->ops, &c->xps, &c->ticket, &c->err};
+    DevBuf* bufs[] = {&c->seqA, &c->seqB, &c->rbuf, &c->cb, &c->cor, &c->rowmax, &c->srow, &c->scol,
+                      &c->dirs, &c->regions, &c->key, &c->found, &c->walk, &c->ops, &c->xps};
     if (c->stream) hipStreamSynchronize(c->stream);
     for (DevBuf* b : bufs)
         if (b->p) hipFree(b->p);

@@ -21,19 +21,20 @@ struct Score {
     int32_t M, X, O, E;  // match, mismatch, gap open ("first"), gap extension
 };
 
-// One independent rectangle of the DP matrix swept by a set of stripes.
-// Rows [r0, r1] and columns [c0, c1] are absolute (inclusive).
-struct Tile {
+// One independent rectangle of the DP matrix, rows [r0, r1] x columns [c0, c1]
+// (absolute, inclusive).  It is cut into tiles of kStripeRows rows (stripe s) by W
+// columns (segment g); tile (s, g) needs the bottom row of (s-1, g), the right column of
+// (s, g-1) and the bottom-right cell of (s-1, g-1).  All tiles with s + g = d are
+// independent and form one launch; launches run in order of d, so every hand-off between
+// tiles crosses a kernel boundary and no wave ever waits on another.
+struct Region {
     int64_t r0, r1, c0, c1;
-    int64_t stripe0;     // first ticket of this tile
-    int64_t nstripes;
-    int2* bnd;           // (c1-c0+1) {H,F}: bottom row of the last published stripe
-    int32_t* prog;       // nstripes: columns of each stripe's bottom row published
-    const int2* top;     // {H,F} of row r0-1 at columns c0..c1, nullptr = zero boundary
-    const int2* left;    // {H,E} of column c0-1 indexed by absolute row, nullptr = zero
-    uint16_t* dirs;      // kDirs: [stripe][step][lane] packed direction codes
-    int64_t tstride;     // kDirs: steps stored per stripe
-    int32_t* found;      // kSearch: smallest absolute column holding the target value
+    const int2* top;   // {H,F} of row r0-1 at columns c0..c1 (index col-c0); nullptr = zero row
+    const int2* left;  // {H,E} of column c0-1 indexed by absolute row; nullptr = zero column
+    int2* cb;          // 2 x (r1-r0+1) {H,E}: right column of segment g, kept in slot (g+1)&1
+    int32_t* cor;      // 2 x stripes: H(R0-1, C0-1) of the next segment, kept in slot (g+1)&1
+    uint16_t* dirs;    // kDirs: [tile s*G+g][step][lane] packed direction codes
+    int32_t* found;    // kSearch: smallest absolute column holding the target value
 };
 
 struct SweepParams {
@@ -42,11 +43,13 @@ struct SweepParams {
     const uint8_t* B;  // columns
     int64_t n;
     Score sc;
-    const Tile* tiles;
-    int32_t ntiles;
-    int32_t _pad;
-    int64_t total_stripes;
-    unsigned long long* ticket;
+    const Region* regions;
+    int32_t nregions;
+    int32_t per;       // tiles per region in this launch (grid = nregions * per)
+    int64_t S, G, W;   // stripes per region, segments per region (widest), columns per segment
+    int64_t d, s_lo;   // anti-diagonal of this launch and its first stripe
+    int2* rbuf;        // 2 x n {H,F}: bottom row of stripe s at absolute columns, slot (s+1)&1
+    int64_t tstride;   // kDirs: steps stored per tile
     // kStage1 outputs
     int32_t* rowmax;   // m
     int2* srow;        // (m/K) x n {H,F}
@@ -56,14 +59,8 @@ struct SweepParams {
     // kSearch
     int64_t target_row;
     int32_t target;
-    // Bounded waits: a wave that polls a predecessor's progress more than spin_limit
-    // times records {1, tile, stripe, need, seen} in err[0..4] and abandons its work;
-    // waves that see err[0] != 0 stop as well.  err must be zeroed before the launch.
-    uint32_t spin_limit;
-    int32_t* err;
+    int32_t _pad;
 };
-
-constexpr uint32_t kDefaultSpinLimit = 1u << 20;
 
 struct WalkState {
     int64_t i, j;              // current cell (absolute)
@@ -84,8 +81,8 @@ struct XPoint {
 struct WalkArgs {
     const uint8_t* A;
     const uint8_t* B;
-    const uint16_t* dirs;
-    int64_t tstride;
+    const uint16_t* dirs;        // codes of the recomputed region (Region::dirs layout)
+    int64_t tstride, G, W;       // steps per tile, segments per stripe, columns per segment
     int64_t r0, c0, rows, cols;  // recomputed region: rows [r0, r0+rows), cols [c0, c0+cols)
     int64_t K;                   // checkpoint row spacing (crosspoint detection)
     const int2* srow;            // checkpoint rows (crosspoint values), may be null
@@ -97,10 +94,9 @@ struct WalkArgs {
 };
 
 // Launchers (sw_kernels.hip).
-hipError_t launch_sweep(int mode, const SweepParams& p, int grid, hipStream_t s);
+hipError_t launch_sweep(int mode, const SweepParams& p, hipStream_t s);
 hipError_t launch_argmax_rows(const int32_t* rowmax, int64_t m, unsigned long long* key,
                               hipStream_t s);
 hipError_t launch_walk(const WalkArgs& a, hipStream_t s);
-int sweep_occupancy(int mode);  // resident 64-thread workgroups per CU
 
 }  // namespace swk

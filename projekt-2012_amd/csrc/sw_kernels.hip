@@ -1,15 +1,15 @@
 // HIP kernels (gfx950) for the SWalign hot path.
 //
-// Forward sweep.  The DP matrix (rows = A, columns = B) is cut into stripes of
-// kStripeRows = 64*R rows; one 64-lane wavefront owns a stripe and sweeps it left to
-// right.  Lane l holds rows [l*R, l*R+R) of the stripe and, at step t, updates column
-// t-l: the stripe advances along anti-diagonals, so lane l needs the bottom-row {H,F}
-// of lane l-1 from the previous step, which it receives with one DPP wave_shr:1 per
-// value.  Lane 0 receives the stripe's top boundary (the previous stripe's bottom row)
-// from LDS; lane 63's bottom row is staged in LDS and written once per 64 steps with a
-// coalesced store.  Stripes run as a persistent grid: a wave takes the next stripe by
-// ticket, and waits on the predecessor stripe's published column count before it reads
-// the boundary row -- so a stripe only ever waits on a stripe that is already running.
+// Forward sweep.  The DP matrix (rows = A, columns = B) is cut into tiles of
+// kStripeRows = 64*R rows by W columns; one 64-lane wavefront computes one tile, left
+// to right.  Lane l holds rows [l*R, l*R+R) of the tile and, at step t, updates column
+// t-l: the tile advances along anti-diagonals, so lane l needs the bottom-row {H,F} of
+// lane l-1 from the previous step, which it receives with one DPP wave_shr:1 per value.
+// Lane 0 receives the tile's top boundary (the bottom row of the tile above) from LDS;
+// lane 63's bottom row is staged in LDS and written once per 64 steps with a coalesced
+// store.  Tiles are scheduled by anti-diagonal of the tile grid, one kernel launch per
+// diagonal, so a tile's inputs were all written by earlier launches: no wave waits on
+// another wave.
 //
 // Cell update (Sterbic/Projekt-2012 src/FindAlignment.cuh:50-56, zero boundaries as in
 // src/SWutils.cpp:131-136), kept in "plus open" form Ho = H + O so that one add serves
@@ -43,110 +43,47 @@ __device__ __forceinline__ int to_vgpr(int v) {
     return __builtin_amdgcn_mov_dpp(v, kDppRowMirror, 0xF, 0xF, false);
 }
 
-__device__ __forceinline__ uint64_t pack2(int x, int y) {
-    return (uint64_t)(uint32_t)x | ((uint64_t)(uint32_t)y << 32);
-}
-
-// Agent-coherent 64-bit load/store for the stripe-to-stripe boundary row (the producer
-// and consumer waves may sit on different XCDs).
-__device__ __forceinline__ int2 load_coherent(const int2* p) {
-    const uint64_t v = __hip_atomic_load((const uint64_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return make_int2((int)(uint32_t)v, (int)(uint32_t)(v >> 32));
-}
-__device__ __forceinline__ void store_coherent(int2* p, int x, int y) {
-    __hip_atomic_store((uint64_t*)p, pack2(x, y), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Wave-uniform reads of the cross-wave flags: every lane takes lane 0's value, so all
-// lanes leave a wait loop on the same iteration.
-__device__ __forceinline__ int32_t load_flag_uniform(const int32_t* q) {
-    return (int32_t)__builtin_amdgcn_readfirstlane(
-        (uint32_t)__hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-}
-
-__device__ __forceinline__ bool sweep_failed(const SweepParams& p) { return load_flag_uniform(p.err) != 0; }
-
-// Waits until stripe `sl-1` of tile T has published at least `need` columns.  The wait
-// is bounded: after p.spin_limit polls (or once another wave has failed) it records the
-// failure and returns false.  Every lane reads the same values, so the outcome is
-// uniform across the wave.
-__device__ __forceinline__ bool wait_prog(const SweepParams& p, const Tile& T, int tile_idx, int64_t sl,
-                                          int64_t need) {
-    if (sl == 0) return true;
-    const int32_t* q = T.prog + (sl - 1);
-    uint32_t spins = 0;
-    int32_t seen;
-    while ((seen = load_flag_uniform(q)) < need) {
-        if (++spins > p.spin_limit || sweep_failed(p)) {
-            if (threadIdx.x == 0) {
-                int32_t expected = 0;
-                if (__hip_atomic_compare_exchange_strong(p.err, &expected, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                         __HIP_MEMORY_SCOPE_AGENT)) {
-                    p.err[1] = tile_idx;
-                    p.err[2] = (int32_t)sl;
-                    p.err[3] = (int32_t)need;
-                    p.err[4] = seen;
-                }
-            }
-            return false;
-        }
-        __builtin_amdgcn_s_sleep(2);
-    }
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    return true;
-}
-
-// Publishes `cols` columns of stripe sl after all of this wave's boundary stores have
-// completed (vmcnt(0)): data stores are agent-coherent, so completion makes them
-// visible to the consumer's agent-coherent loads.
-__device__ __forceinline__ void publish(const Tile& T, int64_t sl, int64_t cols) {
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_s_waitcnt(0);
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    if (threadIdx.x == 0) __hip_atomic_store(T.prog + sl, (int32_t)cols, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+// One tile of a sweep: rows [R0, R0 + 64R) clipped to r1, columns [C0, C0 + w).
+struct TileCtx {
+    int64_t R0, r1, C0, w;
+    const int2* top;    // {H,F} of row R0-1 at columns C0.. (index col-C0); nullptr = zero row
+    int2* bot;          // receives {H,F} of the tile's last row (index col-C0); nullptr = none
+    int2* srow;         // stage 1: checkpoint row receiving the last row (index col-C0) or nullptr
+    uint16_t* dirs;     // kDirs: [step][lane] codes of this tile
+};
 
 // Loads the inputs of chunk cc (steps [64cc, 64cc+64)): lane 0's top boundary for
 // columns 64cc + lane, and the column bytes 64cc-64+lane and 64cc+lane (0x100 pads
 // never equal a byte).
-__device__ __forceinline__ void load_chunk_inputs(const SweepParams& p, const Tile& T, int64_t sl,
-                                                  int64_t cc, int64_t w, int2& in, uint32_t& b0,
-                                                  uint32_t& b1) {
+__device__ __forceinline__ void load_chunk_inputs(const SweepParams& p, const TileCtx& T, int64_t cc,
+                                                  int2& in, uint32_t& b0, uint32_t& b1) {
     const int lane = threadIdx.x;
     const int64_t col = 64 * cc + lane;
     int H = 0, F = 0;
-    if (col < w) {
-        if (sl == 0) {
-            if (T.top) {
-                const int2 v = T.top[col];
-                H = v.x;
-                F = v.y;
-            }
-        } else {
-            const int2 v = load_coherent(T.bnd + col);
-            H = v.x;
-            F = v.y;
-        }
+    if (col < T.w && T.top) {
+        const int2 v = T.top[col];
+        H = v.x;
+        F = v.y;
     }
     in = make_int2(H + p.sc.O, F + p.sc.E);
     const int64_t cb0 = 64 * cc - 64 + lane, cb1 = 64 * cc + lane;
-    b0 = (cb0 >= 0 && cb0 < w) ? (uint32_t)p.B[T.c0 + cb0] : 0x100u;
-    b1 = (cb1 < w) ? (uint32_t)p.B[T.c0 + cb1] : 0x100u;
+    b0 = (cb0 >= 0 && cb0 < T.w) ? (uint32_t)p.B[T.C0 + cb0] : 0x100u;
+    b1 = (cb1 < T.w) ? (uint32_t)p.B[T.C0 + cb1] : 0x100u;
 }
 
-// 64 steps of one stripe.  EDGE: some (lane, step) pairs fall outside [0, w) and must
+// 64 steps of one tile.  EDGE: some (lane, step) pairs fall outside [0, w) and must
 // leave the lane state untouched.  DUMP: stage-1 chunk containing a checkpoint column.
 template <int R, int MODE, bool EDGE, bool DUMP>
-__device__ __forceinline__ void chunk_steps(const SweepParams& p, const Tile& T, int64_t sl, int64_t c,
-                                            int64_t w, int64_t row0, const uint32_t (&a)[R],
-                                            int (&Ho)[R], int (&E)[R], int (&rmax)[R], int& Ho_up_prev,
-                                            int& Ho_bot, int& Fe_bot, const int2* s_in,
+__device__ __forceinline__ void chunk_steps(const SweepParams& p, const TileCtx& T, int64_t c, int64_t row0,
+                                            const uint32_t (&a)[R], int (&Ho)[R], int (&E)[R], int (&rmax)[R],
+                                            int& Ho_up_prev, int& Ho_bot, int& Fe_bot, const int2* s_in,
                                             const uint32_t* s_b, int2* s_out, int& found_col) {
     const int lane = threadIdx.x;
     const int O = p.sc.O, Ev = p.sc.E;
     const int MO = to_vgpr(p.sc.M - O), XO = to_vgpr(p.sc.X - O);
+    const int64_t w = T.w;
     uint16_t* dcol = nullptr;
-    if (MODE == kDirs) dcol = T.dirs + ((sl * T.tstride) + 64 * c) * 64 + lane;
+    if (MODE == kDirs) dcol = T.dirs + (64 * c) * 64 + lane;
     SWK_UNROLL(UNROLL_STEPS)
     for (int s = 0; s < 64; ++s) {
         const int2 in = s_in[s];
@@ -201,11 +138,12 @@ __device__ __forceinline__ void chunk_steps(const SweepParams& p, const Tile& T,
         if (MODE == kDirs) dcol[s * 64] = (uint16_t)nib;
         if (DUMP) {
             const int64_t x = 64 * c + s - lane;
-            if (x >= 0 && x < w && ((x + 1) & (p.Kc - 1)) == 0) {
-                int2* dst = p.scol + ((x + 1) / p.Kc - 1) * p.m;
+            const int64_t X = T.C0 + x;
+            if (x >= 0 && x < w && ((X + 1) & (p.Kc - 1)) == 0) {
+                int2* dst = p.scol + ((X + 1) / p.Kc - 1) * p.m;
 #pragma unroll
                 for (int k = 0; k < R; ++k)
-                    if (row0 + k < p.m) dst[row0 + k] = make_int2(Ho[k] - O, E[k]);
+                    if (row0 + k <= T.r1) dst[row0 + k] = make_int2(Ho[k] - O, E[k]);
             }
         }
     }
@@ -219,16 +157,19 @@ struct SweepLds {
     int2 out[64];
 };
 
+// Sweeps one tile.  left/left_r0: {H,E} of column C0-1 at absolute row r is
+// left[r - left_r0] (nullptr = zero column); corner = H(R0-1, C0-1); right (nullable)
+// receives the tile's last column {H,E} at index row - right_r0.
 template <int R, int MODE>
-__device__ void process_stripe(const SweepParams& p, const Tile& T, int tile_idx, int64_t sl, SweepLds& lds) {
+__device__ void sweep_tile(const SweepParams& p, const TileCtx& T, const int2* left, int64_t left_r0, int corner,
+                           int2* right, int64_t right_r0, bool first_segment, int32_t* found, SweepLds& lds) {
     int2 (*s_in)[64] = lds.in;
     uint32_t (*s_b)[128] = lds.b;
     int2* s_out = lds.out;
     const int lane = threadIdx.x;
     const int O = p.sc.O;
-    constexpr int64_t HR = 64 * R;
-    const int64_t w = T.c1 - T.c0 + 1;
-    const int64_t row0 = T.r0 + sl * HR + (int64_t)lane * R;
+    const int64_t w = T.w;
+    const int64_t row0 = T.R0 + (int64_t)lane * R;
 
     uint32_t a[R];
     int Ho[R], E[R], rmax[R];
@@ -237,8 +178,8 @@ __device__ void process_stripe(const SweepParams& p, const Tile& T, int tile_idx
         const int64_t row = row0 + k;
         const bool in = row <= T.r1;
         a[k] = in ? (uint32_t)p.A[row] : 0x200u;
-        if (T.left && in) {
-            const int2 v = T.left[row];
+        if (left && in) {
+            const int2 v = left[row - left_r0];
             Ho[k] = v.x + O;
             E[k] = v.y;
         } else {
@@ -247,24 +188,19 @@ __device__ void process_stripe(const SweepParams& p, const Tile& T, int tile_idx
         }
         rmax[k] = 0;
     }
-    int Ho_up_prev = O;  // H(row0-1, c0-1) + O
-    if (T.left && row0 - 1 >= 0 && row0 - 1 < p.m) Ho_up_prev = T.left[row0 - 1].x + O;
+    // H(row0-1, C0-1) + O: lane 0 takes the corner, the other lanes the left column.
+    int Ho_up_prev = corner + O;
+    if (lane > 0) Ho_up_prev = (left && row0 - 1 <= T.r1) ? left[row0 - 1 - left_r0].x + O : O;
     int Ho_bot = O, Fe_bot = p.sc.E;
     int found_col = -1;
 
     const int64_t nsteps = w + 63;
     const int64_t nchunks = (nsteps + 63) / 64;
     const int64_t last_interior = (w - 64) / 64;
-    int64_t srow_idx = -1;
-    if (MODE == kStage1 && p.srow) {
-        const int64_t brow = T.r0 + sl * HR + HR - 1;
-        if (brow < p.m && (brow + 1) % p.K == 0) srow_idx = (brow + 1) / p.K - 1;
-    }
 
-    if (!wait_prog(p, T, tile_idx, sl, min((int64_t)64, w))) return;
     int2 in_reg;
     uint32_t b0, b1;
-    load_chunk_inputs(p, T, sl, 0, w, in_reg, b0, b1);
+    load_chunk_inputs(p, T, 0, in_reg, b0, b1);
     s_in[0][lane] = in_reg;
     s_b[0][lane] = b0;
     s_b[0][64 + lane] = b1;
@@ -275,39 +211,35 @@ __device__ void process_stripe(const SweepParams& p, const Tile& T, int tile_idx
 
     for (int64_t c = 0; c < nchunks; ++c) {
         const int cur = (int)(c & 1);
-        if (c + 1 < nchunks) {
-            if (!wait_prog(p, T, tile_idx, sl, min(64 * (c + 2), w))) return;
-            load_chunk_inputs(p, T, sl, c + 1, w, in_reg, b0, b1);
-        }
+        if (c + 1 < nchunks) load_chunk_inputs(p, T, c + 1, in_reg, b0, b1);
         const bool interior = c >= 1 && c <= last_interior;
         bool dump = false;
         if (MODE == kStage1 && p.scol) {
-            const int64_t lo = max(64 * c - 63, (int64_t)0), hi = min(64 * c + 63, w - 1);
+            const int64_t lo = T.C0 + max(64 * c - 63, (int64_t)0), hi = T.C0 + min(64 * c + 63, w - 1);
             const int64_t x = ((lo + p.Kc) / p.Kc) * p.Kc - 1;  // smallest x >= lo, x = -1 mod Kc
             dump = x <= hi;
         }
         if (!dump) {
             if (interior)
-                chunk_steps<R, MODE, false, false>(p, T, sl, c, w, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot,
-                                                   Fe_bot, s_in[cur], s_b[cur], s_out, found_col);
+                chunk_steps<R, MODE, false, false>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
+                                                   s_in[cur], s_b[cur], s_out, found_col);
             else
-                chunk_steps<R, MODE, true, false>(p, T, sl, c, w, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot,
-                                                  Fe_bot, s_in[cur], s_b[cur], s_out, found_col);
+                chunk_steps<R, MODE, true, false>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
+                                                  s_in[cur], s_b[cur], s_out, found_col);
         } else {
-            chunk_steps<R, MODE, true, true>(p, T, sl, c, w, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
-                                             s_in[cur], s_b[cur], s_out, found_col);
+            chunk_steps<R, MODE, true, true>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot, s_in[cur],
+                                             s_b[cur], s_out, found_col);
         }
-        // Bottom row of the stripe for columns 64c-63+lane.
+        // Bottom row of the tile for columns 64c-63+lane.
         __builtin_amdgcn_wave_barrier();
         {
             const int2 o = s_out[lane];
             const int64_t col = 64 * c + lane - 63;
             if (col >= 0 && col < w) {
-                const int H = o.x - O, F = o.y - p.sc.E;
-                if (sl + 1 < T.nstripes) store_coherent(T.bnd + col, H, F);
-                if (MODE == kStage1 && srow_idx >= 0) p.srow[srow_idx * p.n + T.c0 + col] = make_int2(H, F);
+                const int2 v = make_int2(o.x - O, o.y - p.sc.E);
+                if (T.bot) T.bot[col] = v;
+                if (MODE == kStage1 && T.srow) T.srow[col] = v;
             }
-            if (sl + 1 < T.nstripes) publish(T, sl, min(64 * c + 1, w));
         }
         if (c + 1 < nchunks) {
             s_in[cur ^ 1][lane] = in_reg;
@@ -316,36 +248,66 @@ __device__ void process_stripe(const SweepParams& p, const Tile& T, int tile_idx
         }
         __builtin_amdgcn_wave_barrier();
     }
-    if (sl + 1 < T.nstripes) publish(T, sl, w);
-    if (MODE == kStage1) {
+    if (right) {
 #pragma unroll
         for (int k = 0; k < R; ++k)
-            if (row0 + k < p.m) p.rowmax[row0 + k] = rmax[k];
+            if (row0 + k <= T.r1) right[row0 + k - right_r0] = make_int2(Ho[k] - O, E[k]);
+    }
+    if (MODE == kStage1) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const int64_t row = row0 + k;
+            if (row <= T.r1) p.rowmax[row] = first_segment ? rmax[k] : max(p.rowmax[row], rmax[k]);
+        }
     }
     if (MODE == kSearch) {
-        if (found_col >= 0) atomicMin(T.found, (int32_t)(T.c0 + found_col));
+        if (found_col >= 0) atomicMin(found, (int32_t)(T.C0 + found_col));
     }
 }
 
-// Persistent sweep: each wave takes stripes by ticket until none are left.
+// One workgroup = one tile (s, g) with s + g = p.d of region blockIdx.x / p.per.
 template <int R, int MODE>
 __device__ void sweep_main(const SweepParams& p, SweepLds& lds) {
-    while (true) {
-        unsigned long long g = 0;
-        if (threadIdx.x == 0) g = atomicAdd(p.ticket, 1ull);
-        const uint32_t glo = __builtin_amdgcn_readfirstlane((uint32_t)g);
-        const uint32_t ghi = __builtin_amdgcn_readfirstlane((uint32_t)(g >> 32));
-        const int64_t gi = (int64_t)(((uint64_t)ghi << 32) | glo);
-        if (gi >= p.total_stripes || sweep_failed(p)) break;
-        int lo = 0, hi = p.ntiles - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (p.tiles[mid].stripe0 <= gi) lo = mid;
-            else hi = mid - 1;
-        }
-        const Tile T = p.tiles[lo];
-        process_stripe<R, MODE>(p, T, lo, gi - T.stripe0, lds);
+    constexpr int64_t HR = 64 * R;
+    const int64_t q = blockIdx.x / p.per;
+    const int64_t s = p.s_lo + blockIdx.x % p.per;
+    const int64_t g = p.d - s;
+    const Region Rg = p.regions[q];
+    const int64_t rows = Rg.r1 - Rg.r0 + 1, cols = Rg.c1 - Rg.c0 + 1;
+    const int64_t nstr = (rows + HR - 1) / HR, nseg = (cols + p.W - 1) / p.W;
+    if (s >= nstr || g >= nseg) return;  // a narrower region has fewer segments
+    TileCtx T;
+    T.R0 = Rg.r0 + s * HR;
+    T.r1 = Rg.r1;
+    T.C0 = Rg.c0 + g * p.W;
+    T.w = min(p.W, Rg.c1 - T.C0 + 1);
+    const int64_t coff = T.C0 - Rg.c0;
+    T.top = s == 0 ? (Rg.top ? Rg.top + coff : nullptr) : p.rbuf + (s & 1) * p.n + T.C0;
+    T.bot = s + 1 < nstr ? p.rbuf + ((s + 1) & 1) * p.n + T.C0 : nullptr;
+    T.srow = nullptr;
+    if (MODE == kStage1 && p.srow) {
+        const int64_t brow = T.R0 + HR - 1;
+        if (brow <= Rg.r1 && (brow + 1) % p.K == 0) T.srow = p.srow + ((brow + 1) / p.K - 1) * p.n + T.C0;
     }
+    T.dirs = MODE == kDirs ? Rg.dirs + (s * p.G + g) * p.tstride * 64 : nullptr;
+    // Left column and corner H(R0-1, C0-1).
+    const int2* left;
+    int64_t left_r0;
+    int corner;
+    if (g == 0) {
+        left = Rg.left;
+        left_r0 = 0;
+        corner = (Rg.left && T.R0 >= 1) ? Rg.left[T.R0 - 1].x : 0;
+    } else {
+        left = Rg.cb + (g & 1) * rows;
+        left_r0 = Rg.r0;
+        if (s == 0) corner = Rg.top ? Rg.top[coff - 1].x : 0;
+        else corner = Rg.cor[2 * s + (g & 1)];
+    }
+    // Hand the next segment its corner: the top boundary at this tile's last column.
+    if (s > 0 && g + 1 < nseg && threadIdx.x == 0) Rg.cor[2 * s + ((g + 1) & 1)] = T.top[T.w - 1].x;
+    int2* right = g + 1 < nseg ? Rg.cb + ((g + 1) & 1) * rows : nullptr;
+    sweep_tile<R, MODE>(p, T, left, left_r0, corner, right, Rg.r0, g == 0, Rg.found, lds);
 }
 
 template <int R, int MODE>
@@ -409,8 +371,8 @@ __device__ void walk_main(const WalkArgs& a, WalkLds<R>& lds) {
             const int64_t row = wr0 + (int64_t)g * R, col = wc0 + cc;
             uint16_t v = 0;
             if (row < a.rows && col < a.cols) {
-                const int64_t s = row / HR, l = (row % HR) / R, t = col + l;
-                v = a.dirs[(s * a.tstride + t) * 64 + l];
+                const int64_t s = row / HR, l = (row % HR) / R, sg = col / a.W, t = col % a.W + l;
+                v = a.dirs[((s * a.G + sg) * a.tstride + t) * 64 + l];
             }
             win[g][cc] = v;
         }
@@ -480,28 +442,16 @@ __global__ __launch_bounds__(64) void k_walk(WalkArgs a) {
 
 }  // namespace
 
-hipError_t launch_sweep(int mode, const SweepParams& p, int grid, hipStream_t s) {
+hipError_t launch_sweep(int mode, const SweepParams& p, hipStream_t s) {
     constexpr int R = kRowsPerLane;
+    const dim3 grid((unsigned)(p.nregions * p.per));
     switch (mode) {
-        case kStage1: hipLaunchKernelGGL((k_sweep<R, kStage1>), dim3(grid), dim3(64), 0, s, p); break;
-        case kDirs: hipLaunchKernelGGL((k_sweep<R, kDirs>), dim3(grid), dim3(64), 0, s, p); break;
-        case kSearch: hipLaunchKernelGGL((k_sweep<R, kSearch>), dim3(grid), dim3(64), 0, s, p); break;
+        case kStage1: hipLaunchKernelGGL((k_sweep<R, kStage1>), grid, dim3(64), 0, s, p); break;
+        case kDirs: hipLaunchKernelGGL((k_sweep<R, kDirs>), grid, dim3(64), 0, s, p); break;
+        case kSearch: hipLaunchKernelGGL((k_sweep<R, kSearch>), grid, dim3(64), 0, s, p); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
-}
-
-int sweep_occupancy(int mode) {
-    int nb = 0;
-    hipError_t e = hipErrorInvalidValue;
-    constexpr int R = kRowsPerLane;
-    switch (mode) {
-        case kStage1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep<R, kStage1>, 64, 0); break;
-        case kDirs: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep<R, kDirs>, 64, 0); break;
-        case kSearch: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep<R, kSearch>, 64, 0); break;
-        default: break;
-    }
-    return e == hipSuccess ? nb : 0;
 }
 
 hipError_t launch_argmax_rows(const int32_t* rowmax, int64_t m, unsigned long long* key, hipStream_t s) {
