@@ -62,9 +62,11 @@ typedef struct {
     int32_t score_only;        /* 1: score + end cell only (no traceback)                   */
     int32_t precision;         /* 0 = auto, 32 = int32; batch path: 16 = packed int16 first  */
     int32_t want_xpoints;      /* 1: return the path nodes on checkpoint rows                */
-    int32_t tile_cols;         /* columns per sweep tile (rounded up to a multiple of 64);     */
+    int32_t tile_cols;         /* columns per sweep tile (any width >= 1);                     */
                                /* 0 = auto (about four tiles per CU across the row, >= 256)   */
-    int32_t reserved[4];
+    int32_t tb_lookahead;      /* traceback: checkpoint blocks recomputed ahead of the walk in  */
+                               /* each direction per round; 0 = auto (3), -1 = none            */
+    int32_t reserved[3];
 } swh_opts;
 
 /* Path node on a checkpoint row (the analogue of the reference's crosspoints,

@@ -37,8 +37,9 @@ struct swh_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipDeviceProp_t props{};
-    DevBuf seqA, seqB, rbuf, cb, cor, rowmax, srow, scol, dirs, regions, key, found, walk, ops, xps;
+    DevBuf seqA, seqB, rbuf, cb, cor, rowmax, srow, scol, dirs, regions, key, found, walk, ops, xps, wregs;
     hipEvent_t ev[8] = {};
+    size_t lds_pad = 0;  // dynamic LDS per sweep workgroup: one workgroup per CU
     hipError_t last = hipSuccess;
 };
 
@@ -59,7 +60,7 @@ int map_hip(swh_ctx* c, hipError_t e) {
 hipError_t ensure(DevBuf& b, size_t bytes) {
     if (bytes == 0) bytes = 64;
     if (b.p && b.cap >= bytes) return hipSuccess;
-    if (b.p) hipFree(b.p);
+    if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
     b.cap = 0;
     hipError_t e = hipMalloc(&b.p, bytes);
@@ -69,6 +70,9 @@ hipError_t ensure(DevBuf& b, size_t bytes) {
 }
 
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Direction codes kept per traceback round (bounds the lookahead).
+constexpr int64_t kTracebackDirsCap = INT64_C(2) << 30;
 
 int64_t next_pow2(int64_t v) {
     int64_t p = 64;
@@ -83,16 +87,16 @@ bool scoring_ok(const swh_scoring* s) {
 
 float elapsed(hipEvent_t a, hipEvent_t b) {
     float ms = 0.f;
-    hipEventElapsedTime(&ms, a, b);
+    (void)hipEventElapsedTime(&ms, a, b);
     return ms;
 }
 
-// Columns per tile: `want` if given (rounded up to a multiple of 64), else about four
-// tiles per CU across `cols` columns, at least 256.
+// Columns per tile: `want` if given, else one tile per SIMD (four per CU) across `cols`
+// columns, at least 256 (a tile spends 63 extra steps filling and draining its lanes).
 int64_t tile_cols(swh_ctx* c, int64_t cols, int64_t want) {
-    if (want > 0) return ceil_div(want, 64) * 64;
+    if (want > 0) return want;
     const int64_t target = 4 * std::max(1, c->props.multiProcessorCount);
-    return std::max<int64_t>(256, ceil_div(ceil_div(cols, target), 64) * 64);
+    return std::max<int64_t>(256, ceil_div(cols, target));
 }
 
 // Runs a sweep over `regions` with tiles of kStripeRows x W: one launch per anti-diagonal
@@ -100,23 +104,26 @@ int64_t tile_cols(swh_ctx* c, int64_t cols, int64_t want) {
 // before the first and after the last launch.
 int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions, int64_t W,
               hipEvent_t ev_before = nullptr, hipEvent_t ev_after = nullptr) {
-    int64_t S = 0, G = 0, cb_elems = 0, cor_elems = 0;
+    int64_t S = 0, G = 0, cb_elems = 0, rb_elems = 0, cor_elems = 0;
     for (const Region& r : regions) {
         const int64_t rows = r.r1 - r.r0 + 1, cols = r.c1 - r.c0 + 1;
         if (rows <= 0 || cols <= 0) return SWH_ERR_INTERNAL;
         S = std::max(S, ceil_div(rows, kStripeRows));
         G = std::max(G, ceil_div(cols, W));
         cb_elems += 2 * rows;
+        rb_elems += 2 * cols;
     }
     cor_elems = 2 * S * (int64_t)regions.size();
     HCK(c, ensure(c->cb, sizeof(int2) * cb_elems));
     HCK(c, ensure(c->cor, sizeof(int32_t) * cor_elems));
-    HCK(c, ensure(c->rbuf, sizeof(int2) * 2 * p.n));
-    int64_t cbo = 0;
+    HCK(c, ensure(c->rbuf, sizeof(int2) * rb_elems));
+    int64_t cbo = 0, rbo = 0;
     for (size_t q = 0; q < regions.size(); ++q) {
         regions[q].cb = (int2*)c->cb.p + cbo;
+        regions[q].rb = (int2*)c->rbuf.p + rbo;
         regions[q].cor = (int32_t*)c->cor.p + 2 * S * (int64_t)q;
         cbo += 2 * (regions[q].r1 - regions[q].r0 + 1);
+        rbo += 2 * (regions[q].c1 - regions[q].c0 + 1);
     }
     HCK(c, ensure(c->regions, sizeof(Region) * regions.size()));
     HCK(c, hipMemcpyAsync(c->regions.p, regions.data(), sizeof(Region) * regions.size(), hipMemcpyHostToDevice,
@@ -126,14 +133,13 @@ int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions,
     p.S = S;
     p.G = G;
     p.W = W;
-    p.rbuf = (int2*)c->rbuf.p;
     if (ev_before) HCK(c, hipEventRecord(ev_before, c->stream));
     for (int64_t d = 0; d < S + G - 1; ++d) {
         const int64_t s_lo = std::max<int64_t>(0, d - (G - 1)), s_hi = std::min(S - 1, d);
         p.d = d;
         p.s_lo = s_lo;
         p.per = (int32_t)(s_hi - s_lo + 1);
-        HCK(c, launch_sweep(mode, p, c->stream));
+        HCK(c, launch_sweep(mode, p, c->lds_pad, c->stream));
     }
     if (ev_after) HCK(c, hipEventRecord(ev_after, c->stream));
     return SWH_OK;
@@ -147,7 +153,7 @@ int choose_plan(swh_ctx* c, int64_t m, int64_t n, const swh_opts* o, Plan* pl) {
     int64_t K = o && o->checkpoint_rows > 0 ? ceil_div(o->checkpoint_rows, kStripeRows) * kStripeRows : 0;
     int64_t Kc = o && o->checkpoint_cols > 0 ? next_pow2(o->checkpoint_cols) : 0;
     size_t freeb = 0, totalb = 0;
-    hipMemGetInfo(&freeb, &totalb);
+    (void)hipMemGetInfo(&freeb, &totalb);
     int64_t budget = o && o->hbm_budget_bytes > 0 ? o->hbm_budget_bytes : (int64_t)(freeb / 2);
     auto mem = [&](int64_t k, int64_t kc) { return ((m / k) * n + (n / kc) * m) * (int64_t)sizeof(int2); };
     if (o && o->score_only) {
@@ -291,41 +297,72 @@ int align_core(swh_ctx* c, const uint8_t* dA, int64_t m, const uint8_t* dB, int6
     st.layer = 0;
     st.done = 0;
     HCK(c, hipMemcpyAsync(c->walk.p, &st, sizeof st, hipMemcpyHostToDevice, c->stream));
+    // Rounds: the checkpoint block holding the current cell (cut at that cell) and the
+    // full blocks up to L block-rows above and L block-columns to the left are
+    // recomputed in one sweep (independent regions, each from its own checkpoints), then
+    // the walk follows the path through them until it leaves them.
+    const int64_t Wtb = tile_cols(c, std::min(n, pl.Kc), want_w);
+    const int64_t tstride = ceil_div(Wtb + 63, 64) * 64;
+    const int64_t Gblk = ceil_div(std::min(n, pl.Kc), Wtb);
+    const int64_t blk_dirs = ceil_div(std::min(m, pl.K), kStripeRows) * Gblk * tstride * 64 * (int64_t)sizeof(uint16_t);
+    int64_t L = o && o->tb_lookahead != 0 ? std::max(0, o->tb_lookahead) : 3;
+    while (L > 0 && (L + 1) * (L + 1) * blk_dirs > kTracebackDirsCap) --L;
+    std::vector<WalkRegion> wregs((size_t)((L + 1) * (L + 1)));
+    HCK(c, ensure(c->wregs, sizeof(WalkRegion) * wregs.size()));
     int64_t guard = 0;
     while (!st.done) {
-        if (++guard > (m / std::max<int64_t>(pl.K, 1) + n / std::max<int64_t>(pl.Kc, 1) + 8) * 4)
-            return SWH_ERR_INTERNAL;
-        const int64_t r0 = (st.i / pl.K) * pl.K, c0 = (st.j / pl.Kc) * pl.Kc;
-        const int64_t rows = st.i - r0 + 1, cols = st.j - c0 + 1;
-        const int64_t W = tile_cols(c, cols, want_w);
-        const int64_t ns = ceil_div(rows, kStripeRows), ng = ceil_div(cols, W);
-        const int64_t tstride = ceil_div(W + 63, 64) * 64;
-        HCK(c, ensure(c->dirs, sizeof(uint16_t) * (size_t)(ns * ng * tstride * 64)));
-        Region r{};
-        r.r0 = r0;
-        r.r1 = st.i;
-        r.c0 = c0;
-        r.c1 = st.j;
-        r.top = r0 > 0 ? srow + (r0 / pl.K - 1) * n + c0 : nullptr;
-        r.left = c0 > 0 ? scol + (c0 / pl.Kc - 1) * m : nullptr;
-        r.dirs = (uint16_t*)c->dirs.p;
-        std::vector<Region> regions{r};
+        if (++guard > m / std::max<int64_t>(pl.K, 1) + n / std::max<int64_t>(pl.Kc, 1) + 8) return SWH_ERR_INTERNAL;
+        const int64_t brow = st.i / pl.K, bcol = st.j / pl.Kc;
+        std::vector<Region> regions;
+        std::vector<int64_t> slot;
+        int64_t dirs_elems = 0;
+        for (int64_t da = 0; da <= L; ++da) {
+            for (int64_t db = 0; db <= L; ++db) {
+                WalkRegion& w = wregs[(size_t)(da * (L + 1) + db)];
+                w = WalkRegion{};
+                if (brow - da < 0 || bcol - db < 0) continue;
+                Region r{};
+                r.r0 = (brow - da) * pl.K;
+                r.r1 = da == 0 ? st.i : std::min(m, r.r0 + pl.K) - 1;
+                r.c0 = (bcol - db) * pl.Kc;
+                r.c1 = db == 0 ? st.j : std::min(n, r.c0 + pl.Kc) - 1;
+                r.top = r.r0 > 0 ? srow + (r.r0 / pl.K - 1) * n + r.c0 : nullptr;
+                r.left = r.c0 > 0 ? scol + (r.c0 / pl.Kc - 1) * m : nullptr;
+                w.r0 = r.r0;
+                w.c0 = r.c0;
+                w.rows = r.r1 - r.r0 + 1;
+                w.cols = r.c1 - r.c0 + 1;
+                slot.push_back(dirs_elems);
+                dirs_elems += ceil_div(w.rows, kStripeRows) * Gblk * tstride * 64;
+                regions.push_back(r);
+            }
+        }
+        HCK(c, ensure(c->dirs, sizeof(uint16_t) * (size_t)dirs_elems));
+        for (size_t k = 0, q = 0; k < wregs.size(); ++k) {
+            if (wregs[k].rows <= 0) continue;
+            uint16_t* d = (uint16_t*)c->dirs.p + slot[q];
+            regions[q].dirs = d;
+            wregs[k].dirs = d;
+            ++q;
+        }
         SweepParams pd = p;
         pd.tstride = tstride;
-        rc = run_sweep(c, kDirs, pd, regions, W);
+        rc = run_sweep(c, kDirs, pd, regions, Wtb);
         if (rc) return rc;
+        HCK(c, hipMemcpyAsync(c->wregs.p, wregs.data(), sizeof(WalkRegion) * wregs.size(), hipMemcpyHostToDevice,
+                              c->stream));
         WalkArgs wa{};
         wa.A = dA;
         wa.B = dB;
-        wa.dirs = (const uint16_t*)c->dirs.p;
+        wa.regs = (const WalkRegion*)c->wregs.p;
+        wa.L = L;
+        wa.brow = brow;
+        wa.bcol = bcol;
         wa.tstride = tstride;
-        wa.G = ng;
-        wa.W = W;
-        wa.r0 = r0;
-        wa.c0 = c0;
-        wa.rows = rows;
-        wa.cols = cols;
+        wa.G = Gblk;
+        wa.W = Wtb;
         wa.K = pl.K;
+        wa.Kc = pl.Kc;
         wa.srow = pl.nsr ? srow : nullptr;
         wa.n = n;
         wa.st = (WalkState*)c->walk.p;
@@ -333,9 +370,10 @@ int align_core(swh_ctx* c, const uint8_t* dA, int64_t m, const uint8_t* dB, int6
         wa.xps = (XPoint*)c->xps.p;
         wa.xcap = xcap;
         HCK(c, launch_walk(wa, c->stream));
+        const int64_t pi = st.i, pj = st.j;
         HCK(c, hipMemcpyAsync(&st, c->walk.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
         HCK(c, hipStreamSynchronize(c->stream));
-        if (!st.done && (st.i >= r0 && st.j >= c0)) return SWH_ERR_INTERNAL;  // no progress
+        if (!st.done && st.i == pi && st.j == pj) return SWH_ERR_INTERNAL;  // no progress
     }
     HCK(c, hipEventRecord(c->ev[5], c->stream));
     std::string ops((size_t)st.nops, '\0');
@@ -419,21 +457,35 @@ int swh_ctx_create(int device, swh_ctx** out) {
         delete c;
         return SWH_ERR_HIP;
     }
+    // Reserve just over half a CU's LDS per sweep workgroup so that one fits a CU and its
+    // waves take one SIMD each (two tiles sharing a SIMD would double a launch's time).
+    const char* env = getenv("SWALIGN_SWEEP_SPREAD");
+    if (!(env && env[0] == '0')) {
+        const size_t per_cu = (size_t)c->props.maxSharedMemoryPerMultiProcessor;
+        const size_t stat = sweep_static_lds(), want = per_cu / 2 + 1024;
+        if (want > stat && want <= (size_t)c->props.sharedMemPerBlock) {
+            c->lds_pad = want - stat;
+            if (sweep_allow_lds(c->lds_pad) != hipSuccess) {
+                (void)hipGetLastError();
+                c->lds_pad = 0;
+            }
+        }
+    }
     *out = c;
     return SWH_OK;
 }
 
 int swh_ctx_destroy(swh_ctx* c) {
     if (!c) return SWH_ERR_INVALID_ARG;
-    hipSetDevice(c->device);
+    (void)hipSetDevice(c->device);
     DevBuf* bufs[] = {&c->seqA, &c->seqB, &c->rbuf, &c->cb, &c->cor, &c->rowmax, &c->srow, &c->scol,
-                      &c->dirs, &c->regions, &c->key, &c->found, &c->walk, &c->ops, &c->xps};
-    if (c->stream) hipStreamSynchronize(c->stream);
+                      &c->dirs, &c->regions, &c->key, &c->found, &c->walk, &c->ops, &c->xps, &c->wregs};
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (DevBuf* b : bufs)
-        if (b->p) hipFree(b->p);
+        if (b->p) (void)hipFree(b->p);
     for (auto& e : c->ev)
-        if (e) hipEventDestroy(e);
-    if (c->stream) hipStreamDestroy(c->stream);
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return SWH_OK;
 }

@@ -9,6 +9,9 @@ namespace swk {
 // Rows per lane of the forward sweep; a wave owns a stripe of 64*kRowsPerLane rows.
 constexpr int kRowsPerLane = 4;
 constexpr int kStripeRows = 64 * kRowsPerLane;
+// Waves per sweep workgroup, one tile each.  Sweeps reserve enough LDS that one
+// workgroup fits a CU, so the waves of a launch spread one per SIMD.
+constexpr int kSweepWaves = 4;
 
 // Sweep modes.
 enum SweepMode : int {
@@ -31,6 +34,7 @@ struct Region {
     int64_t r0, r1, c0, c1;
     const int2* top;   // {H,F} of row r0-1 at columns c0..c1 (index col-c0); nullptr = zero row
     const int2* left;  // {H,E} of column c0-1 indexed by absolute row; nullptr = zero column
+    int2* rb;          // 2 x (c1-c0+1) {H,F}: bottom row of stripe s, kept in slot (s+1)&1
     int2* cb;          // 2 x (r1-r0+1) {H,E}: right column of segment g, kept in slot (g+1)&1
     int32_t* cor;      // 2 x stripes: H(R0-1, C0-1) of the next segment, kept in slot (g+1)&1
     uint16_t* dirs;    // kDirs: [tile s*G+g][step][lane] packed direction codes
@@ -48,7 +52,6 @@ struct SweepParams {
     int32_t per;       // tiles per region in this launch (grid = nregions * per)
     int64_t S, G, W;   // stripes per region, segments per region (widest), columns per segment
     int64_t d, s_lo;   // anti-diagonal of this launch and its first stripe
-    int2* rbuf;        // 2 x n {H,F}: bottom row of stripe s at absolute columns, slot (s+1)&1
     int64_t tstride;   // kDirs: steps stored per tile
     // kStage1 outputs
     int32_t* rowmax;   // m
@@ -78,13 +81,21 @@ struct XPoint {
     int32_t layer;
 };
 
+// A recomputed region of direction codes: rows [r0, r0+rows), cols [c0, c0+cols).
+struct WalkRegion {
+    int64_t r0, c0, rows, cols;
+    const uint16_t* dirs;        // Region::dirs layout
+};
+
+// The walk visits the regions of one traceback round.  Region a*(L+1)+b covers (part of)
+// the checkpoint block (brow-a, bcol-b); rows == 0 marks a block not recomputed.
 struct WalkArgs {
     const uint8_t* A;
     const uint8_t* B;
-    const uint16_t* dirs;        // codes of the recomputed region (Region::dirs layout)
+    const WalkRegion* regs;
+    int64_t L, brow, bcol;       // lookahead and the block of the round's first region
     int64_t tstride, G, W;       // steps per tile, segments per stripe, columns per segment
-    int64_t r0, c0, rows, cols;  // recomputed region: rows [r0, r0+rows), cols [c0, c0+cols)
-    int64_t K;                   // checkpoint row spacing (crosspoint detection)
+    int64_t K, Kc;               // checkpoint row / column spacing
     const int2* srow;            // checkpoint rows (crosspoint values), may be null
     int64_t n;
     WalkState* st;
@@ -94,7 +105,9 @@ struct WalkArgs {
 };
 
 // Launchers (sw_kernels.hip).
-hipError_t launch_sweep(int mode, const SweepParams& p, hipStream_t s);
+hipError_t launch_sweep(int mode, const SweepParams& p, size_t lds_pad, hipStream_t s);
+size_t sweep_static_lds();                     // static LDS of a sweep workgroup
+hipError_t sweep_allow_lds(size_t dyn_bytes);  // permits dyn_bytes of dynamic LDS
 hipError_t launch_argmax_rows(const int32_t* rowmax, int64_t m, unsigned long long* key,
                               hipStream_t s);
 hipError_t launch_walk(const WalkArgs& a, hipStream_t s);

@@ -43,6 +43,9 @@ __device__ __forceinline__ int to_vgpr(int v) {
     return __builtin_amdgcn_mov_dpp(v, kDppRowMirror, 0xF, 0xF, false);
 }
 
+// Lane within the wavefront (a sweep workgroup holds several independent waves).
+__device__ __forceinline__ int wave_lane() { return (int)(threadIdx.x & 63u); }
+
 // One tile of a sweep: rows [R0, R0 + 64R) clipped to r1, columns [C0, C0 + w).
 struct TileCtx {
     int64_t R0, r1, C0, w;
@@ -57,7 +60,7 @@ struct TileCtx {
 // never equal a byte).
 __device__ __forceinline__ void load_chunk_inputs(const SweepParams& p, const TileCtx& T, int64_t cc,
                                                   int2& in, uint32_t& b0, uint32_t& b1) {
-    const int lane = threadIdx.x;
+    const int lane = wave_lane();
     const int64_t col = 64 * cc + lane;
     int H = 0, F = 0;
     if (col < T.w && T.top) {
@@ -78,7 +81,7 @@ __device__ __forceinline__ void chunk_steps(const SweepParams& p, const TileCtx&
                                             const uint32_t (&a)[R], int (&Ho)[R], int (&E)[R], int (&rmax)[R],
                                             int& Ho_up_prev, int& Ho_bot, int& Fe_bot, const int2* s_in,
                                             const uint32_t* s_b, int2* s_out, int& found_col) {
-    const int lane = threadIdx.x;
+    const int lane = wave_lane();
     const int O = p.sc.O, Ev = p.sc.E;
     const int MO = to_vgpr(p.sc.M - O), XO = to_vgpr(p.sc.X - O);
     const int64_t w = T.w;
@@ -166,7 +169,7 @@ __device__ void sweep_tile(const SweepParams& p, const TileCtx& T, const int2* l
     int2 (*s_in)[64] = lds.in;
     uint32_t (*s_b)[128] = lds.b;
     int2* s_out = lds.out;
-    const int lane = threadIdx.x;
+    const int lane = wave_lane();
     const int O = p.sc.O;
     const int64_t w = T.w;
     const int64_t row0 = T.R0 + (int64_t)lane * R;
@@ -265,12 +268,16 @@ __device__ void sweep_tile(const SweepParams& p, const TileCtx& T, const int2* l
     }
 }
 
-// One workgroup = one tile (s, g) with s + g = p.d of region blockIdx.x / p.per.
+// One wave = one tile (s, g) with s + g = p.d; tile index x = wave number in the grid,
+// region x / p.per, stripe p.s_lo + x % p.per.  Waves of a workgroup are independent.
 template <int R, int MODE>
 __device__ void sweep_main(const SweepParams& p, SweepLds& lds) {
     constexpr int64_t HR = 64 * R;
-    const int64_t q = blockIdx.x / p.per;
-    const int64_t s = p.s_lo + blockIdx.x % p.per;
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wave
+    const int64_t x = (int64_t)blockIdx.x * kSweepWaves + wid;
+    const int64_t q = x / p.per;
+    if (q >= p.nregions) return;
+    const int64_t s = p.s_lo + x % p.per;
     const int64_t g = p.d - s;
     const Region Rg = p.regions[q];
     const int64_t rows = Rg.r1 - Rg.r0 + 1, cols = Rg.c1 - Rg.c0 + 1;
@@ -282,8 +289,8 @@ __device__ void sweep_main(const SweepParams& p, SweepLds& lds) {
     T.C0 = Rg.c0 + g * p.W;
     T.w = min(p.W, Rg.c1 - T.C0 + 1);
     const int64_t coff = T.C0 - Rg.c0;
-    T.top = s == 0 ? (Rg.top ? Rg.top + coff : nullptr) : p.rbuf + (s & 1) * p.n + T.C0;
-    T.bot = s + 1 < nstr ? p.rbuf + ((s + 1) & 1) * p.n + T.C0 : nullptr;
+    T.top = s == 0 ? (Rg.top ? Rg.top + coff : nullptr) : Rg.rb + (s & 1) * cols + coff;
+    T.bot = s + 1 < nstr ? Rg.rb + ((s + 1) & 1) * cols + coff : nullptr;
     T.srow = nullptr;
     if (MODE == kStage1 && p.srow) {
         const int64_t brow = T.R0 + HR - 1;
@@ -305,15 +312,15 @@ __device__ void sweep_main(const SweepParams& p, SweepLds& lds) {
         else corner = Rg.cor[2 * s + (g & 1)];
     }
     // Hand the next segment its corner: the top boundary at this tile's last column.
-    if (s > 0 && g + 1 < nseg && threadIdx.x == 0) Rg.cor[2 * s + ((g + 1) & 1)] = T.top[T.w - 1].x;
+    if (s > 0 && g + 1 < nseg && wave_lane() == 0) Rg.cor[2 * s + ((g + 1) & 1)] = T.top[T.w - 1].x;
     int2* right = g + 1 < nseg ? Rg.cb + ((g + 1) & 1) * rows : nullptr;
     sweep_tile<R, MODE>(p, T, left, left_r0, corner, right, Rg.r0, g == 0, Rg.found, lds);
 }
 
 template <int R, int MODE>
-__global__ __launch_bounds__(64) void k_sweep(SweepParams p) {
-    __shared__ SweepLds lds;
-    sweep_main<R, MODE>(p, lds);
+__global__ __launch_bounds__(64 * kSweepWaves) void k_sweep(SweepParams p) {
+    __shared__ SweepLds lds[kSweepWaves];
+    sweep_main<R, MODE>(p, lds[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)]);
 }
 
 __device__ void argmax_rows_main(const int32_t* __restrict__ rowmax, int64_t m, unsigned long long* key) {
@@ -336,15 +343,28 @@ __global__ __launch_bounds__(256) void k_argmax_rows(const int32_t* __restrict__
     argmax_rows_main(rowmax, m, key);
 }
 
-// Greedy backward walk over one recomputed region (see WalkArgs).  The region's
-// direction codes are staged through LDS in 128x128-cell windows; lane 0 walks.
+// Greedy backward walk over the regions of one traceback round (see WalkArgs).  A
+// region's direction codes are staged through LDS in 128x128-cell windows together with
+// the sequence bytes of the window; lane 0 walks.  The walk stops at the start of the
+// alignment or when it reaches a cell outside every region of the round.
 constexpr int kWalkWinRows = 128, kWalkWinCols = 128;
 
 template <int R>
 struct WalkLds {
     uint16_t win[kWalkWinRows / R][kWalkWinCols];
+    uint8_t a[kWalkWinRows], b[kWalkWinCols];  // sequence bytes of the window's rows, columns
     WalkState sst;
 };
+
+// Region of the round holding cell (i, j), or nullptr.
+__device__ __forceinline__ const WalkRegion* walk_region(const WalkArgs& a, int64_t i, int64_t j) {
+    if (i < 0 || j < 0) return nullptr;
+    const int64_t da = a.brow - i / a.K, db = a.bcol - j / a.Kc;
+    if (da < 0 || db < 0 || da > a.L || db > a.L) return nullptr;
+    const WalkRegion* r = a.regs + da * (a.L + 1) + db;
+    if (r->rows <= 0 || i < r->r0 || j < r->c0 || i >= r->r0 + r->rows || j >= r->c0 + r->cols) return nullptr;
+    return r;
+}
 
 template <int R>
 __device__ void walk_main(const WalkArgs& a, WalkLds<R>& lds) {
@@ -358,8 +378,10 @@ __device__ void walk_main(const WalkArgs& a, WalkLds<R>& lds) {
     while (true) {
         WalkState st = sst;
         if (st.done) break;
-        const int64_t il = st.i - a.r0, jl = st.j - a.c0;
-        if (st.i < 0 || st.j < 0 || il < 0 || jl < 0) break;  // left the region
+        const WalkRegion* rg = walk_region(a, st.i, st.j);
+        if (!rg) break;  // outside the round's regions
+        const WalkRegion rgn = *rg;
+        const int64_t il = st.i - rgn.r0, jl = st.j - rgn.c0;
         // rows [wr0, wr0+WR) with wr0 a multiple of R and il inside the window
         int64_t wr0 = il - (WR - R);
         if (wr0 < 0) wr0 = 0;
@@ -370,11 +392,19 @@ __device__ void walk_main(const WalkArgs& a, WalkLds<R>& lds) {
             const int g = idx / WC, cc = idx % WC;
             const int64_t row = wr0 + (int64_t)g * R, col = wc0 + cc;
             uint16_t v = 0;
-            if (row < a.rows && col < a.cols) {
+            if (row < rgn.rows && col < rgn.cols) {
                 const int64_t s = row / HR, l = (row % HR) / R, sg = col / a.W, t = col % a.W + l;
-                v = a.dirs[((s * a.G + sg) * a.tstride + t) * 64 + l];
+                v = rgn.dirs[((s * a.G + sg) * a.tstride + t) * 64 + l];
             }
             win[g][cc] = v;
+        }
+        for (int k = lane; k < WR; k += 64) {
+            const int64_t row = wr0 + k;
+            lds.a[k] = row < rgn.rows ? a.A[rgn.r0 + row] : 0;
+        }
+        for (int k = lane; k < WC; k += 64) {
+            const int64_t col = wc0 + k;
+            lds.b[k] = col < rgn.cols ? a.B[rgn.c0 + col] : 0;
         }
         __syncthreads();
         if (lane == 0) {
@@ -383,9 +413,8 @@ __device__ void walk_main(const WalkArgs& a, WalkLds<R>& lds) {
                     st.done = 1;
                     break;
                 }
-                const int64_t ri = st.i - a.r0, rj = st.j - a.c0;
-                if (ri < 0 || rj < 0) break;          // left the region
-                if (ri < wr0 || rj < wc0) break;      // left the window
+                const int64_t ri = st.i - rgn.r0, rj = st.j - rgn.c0;
+                if (ri < wr0 || rj < wc0) break;  // left the window (or the region)
                 const int64_t gi = ri - wr0;
                 const uint32_t d = (win[gi / R][rj - wc0] >> (4 * (gi % R))) & 15u;
                 if (st.i != st.last_row) {
@@ -403,7 +432,7 @@ __device__ void walk_main(const WalkArgs& a, WalkLds<R>& lds) {
                 if (st.layer == 0) {
                     const uint32_t h = d & 3u;
                     if (h == 1u) {
-                        a.ops[st.nops++] = (a.A[st.i] == a.B[st.j]) ? (uint8_t)'=' : (uint8_t)'X';
+                        a.ops[st.nops++] = (lds.a[ri - wr0] == lds.b[rj - wc0]) ? (uint8_t)'=' : (uint8_t)'X';
                         st.start_i = st.i;
                         st.start_j = st.j;
                         --st.i;
@@ -442,16 +471,30 @@ __global__ __launch_bounds__(64) void k_walk(WalkArgs a) {
 
 }  // namespace
 
-hipError_t launch_sweep(int mode, const SweepParams& p, hipStream_t s) {
+hipError_t launch_sweep(int mode, const SweepParams& p, size_t lds_pad, hipStream_t s) {
     constexpr int R = kRowsPerLane;
-    const dim3 grid((unsigned)(p.nregions * p.per));
+    const int64_t tiles = (int64_t)p.nregions * p.per;
+    const dim3 grid((unsigned)((tiles + kSweepWaves - 1) / kSweepWaves)), block(64 * kSweepWaves);
     switch (mode) {
-        case kStage1: hipLaunchKernelGGL((k_sweep<R, kStage1>), grid, dim3(64), 0, s, p); break;
-        case kDirs: hipLaunchKernelGGL((k_sweep<R, kDirs>), grid, dim3(64), 0, s, p); break;
-        case kSearch: hipLaunchKernelGGL((k_sweep<R, kSearch>), grid, dim3(64), 0, s, p); break;
+        case kStage1: hipLaunchKernelGGL((k_sweep<R, kStage1>), grid, block, lds_pad, s, p); break;
+        case kDirs: hipLaunchKernelGGL((k_sweep<R, kDirs>), grid, block, lds_pad, s, p); break;
+        case kSearch: hipLaunchKernelGGL((k_sweep<R, kSearch>), grid, block, lds_pad, s, p); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+size_t sweep_static_lds() { return sizeof(SweepLds) * kSweepWaves; }
+
+hipError_t sweep_allow_lds(size_t dyn_bytes) {
+    constexpr int R = kRowsPerLane;
+    const int v = (int)dyn_bytes;
+    hipError_t e = hipFuncSetAttribute((const void*)k_sweep<R, kStage1>, hipFuncAttributeMaxDynamicSharedMemorySize, v);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)k_sweep<R, kDirs>, hipFuncAttributeMaxDynamicSharedMemorySize, v);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)k_sweep<R, kSearch>, hipFuncAttributeMaxDynamicSharedMemorySize, v);
+    return e;
 }
 
 hipError_t launch_argmax_rows(const int32_t* rowmax, int64_t m, unsigned long long* key, hipStream_t s) {

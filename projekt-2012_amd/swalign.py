@@ -31,7 +31,8 @@ class Opts(C.Structure):
     _fields_ = [("checkpoint_rows", C.c_int64), ("checkpoint_cols", C.c_int64),
                 ("hbm_budget_bytes", C.c_int64), ("score_only", C.c_int32),
                 ("precision", C.c_int32), ("want_xpoints", C.c_int32),
-                ("tile_cols", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("tile_cols", C.c_int32), ("tb_lookahead", C.c_int32),
+                ("reserved", C.c_int32 * 3)]
 
 
 class XPoint(C.Structure):
@@ -127,9 +128,10 @@ def _result_dict(r):
     return d
 
 
-def make_opts(K=0, Kc=0, score_only=False, xpoints=False, budget=0, precision=0, tile_cols=0):
+def make_opts(K=0, Kc=0, score_only=False, xpoints=False, budget=0, precision=0, tile_cols=0, lookahead=0):
     o = Opts()
     o.tile_cols = tile_cols
+    o.tb_lookahead = lookahead
     o.checkpoint_rows = K
     o.checkpoint_cols = Kc
     o.hbm_budget_bytes = budget

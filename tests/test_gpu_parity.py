@@ -120,13 +120,14 @@ def test_known_answers_large(sw, orc, ctx, name):
     assert s == got["score"] and end == got["end"]
 
 
-@pytest.mark.parametrize("W", [64, 128, 448])
-def test_tile_width_shapes(sw, orc, ctx, W):
+@pytest.mark.parametrize("W,L", [(64, 0), (128, -1), (448, 1), (37, 2), (1, 3)])
+def test_tile_width_shapes(sw, orc, ctx, W, L):
     """Tiles narrower than the matrix: hand-offs of bottom rows, right columns and corners
-    between tiles (ragged last tile, ragged last stripe)."""
+    between tiles (ragged last tile, ragged last stripe); traceback rounds with L blocks
+    of lookahead (-1: one block per round)."""
     (a, b), sc = sw.synth_pair(500 + W, 1700, 1333, 800, 0.1, 0.02), sw.scoring(1, -3, -5, -2)
     A, B = orient(a, b)
     ref = orc.align_linear(A, B, orc.scoring(1, -3, -5, -2), 512, 256)
-    got = ctx.align(a, b, sc, K=512, Kc=256, xpoints=True, tile_cols=W)
+    got = ctx.align(a, b, sc, K=512, Kc=256, xpoints=True, tile_cols=W, lookahead=L)
     _cmp(got, ref)
     assert got["xpoints"] == [tuple(x) for x in ref["xpoints"]]
