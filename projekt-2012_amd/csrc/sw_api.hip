@@ -100,9 +100,10 @@ int64_t tile_cols(swh_ctx* c, int64_t cols, int64_t want) {
 }
 
 // Runs a sweep over `regions` with tiles of kStripeRows x W: one launch per anti-diagonal
-// of the tile grid, all regions side by side.  Optional events are recorded immediately
-// before the first and after the last launch.
-int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions, int64_t W,
+// of the tile grid, all regions side by side.  The grid has G segments per stripe (the
+// widest region's, or G_fix when given: kDirs lays codes out by it).  Optional events are
+// recorded immediately before the first and after the last launch.
+int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions, int64_t W, int64_t G_fix = 0,
               hipEvent_t ev_before = nullptr, hipEvent_t ev_after = nullptr) {
     int64_t S = 0, G = 0, cb_elems = 0, rb_elems = 0, cor_elems = 0;
     for (const Region& r : regions) {
@@ -112,6 +113,10 @@ int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions,
         G = std::max(G, ceil_div(cols, W));
         cb_elems += 2 * rows;
         rb_elems += 2 * cols;
+    }
+    if (G_fix > 0) {
+        if (G > G_fix) return SWH_ERR_INTERNAL;
+        G = G_fix;
     }
     cor_elems = 2 * S * (int64_t)regions.size();
     HCK(c, ensure(c->cb, sizeof(int2) * cb_elems));
@@ -225,7 +230,7 @@ int align_core(swh_ctx* c, const uint8_t* dA, int64_t m, const uint8_t* dB, int6
         r.c0 = 0;
         r.c1 = n - 1;
         std::vector<Region> regions{r};
-        rc = run_sweep(c, kStage1, p, regions, tile_cols(c, n, want_w), c->ev[0], c->ev[1]);
+        rc = run_sweep(c, kStage1, p, regions, tile_cols(c, n, want_w), 0, c->ev[0], c->ev[1]);
         if (rc) return rc;
     }
     HCK(c, hipMemsetAsync(c->key.p, 0, 8, c->stream));
@@ -347,7 +352,7 @@ int align_core(swh_ctx* c, const uint8_t* dA, int64_t m, const uint8_t* dB, int6
         }
         SweepParams pd = p;
         pd.tstride = tstride;
-        rc = run_sweep(c, kDirs, pd, regions, Wtb);
+        rc = run_sweep(c, kDirs, pd, regions, Wtb, Gblk);
         if (rc) return rc;
         HCK(c, hipMemcpyAsync(c->wregs.p, wregs.data(), sizeof(WalkRegion) * wregs.size(), hipMemcpyHostToDevice,
                               c->stream));
