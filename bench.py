@@ -47,11 +47,15 @@ def cpu_baseline(a, b, budget_s=15.0):
     A, B = (a, b) if len(a) >= len(b) else (b, a)
     threads = max(1, min(16, os.cpu_count() or 1))
     sc = orc.scoring(1, -3, -5, -2)
-    # calibrate on a small band, then size the sample to about budget_s
-    rows = 512
-    t0 = time.perf_counter()
-    orc.forward(A[:rows], B, sc, threads=threads)
-    dt = time.perf_counter() - t0
+    # calibrate on growing bands until one takes a second, then size the sample to budget_s
+    rows = 256
+    while True:
+        t0 = time.perf_counter()
+        orc.forward(A[:rows], B, sc, threads=threads)
+        dt = time.perf_counter() - t0
+        if dt >= 1.0 or rows >= len(A):
+            break
+        rows = min(len(A), rows * 4)
     rate = rows * len(B) / max(dt, 1e-6)
     rows = int(min(len(A), max(1024, rate * budget_s / len(B))))
     t0 = time.perf_counter()

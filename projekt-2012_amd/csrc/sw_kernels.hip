@@ -29,11 +29,13 @@ namespace {
 #define SWK_DO_PRAGMA(x) _Pragma(#x)
 #define SWK_UNROLL(n) SWK_DO_PRAGMA(unroll n)
 
+constexpr int kDppWaveShl1 = 0x130;  // lane l <- lane l+1, lane 63 keeps `old`
 constexpr int kDppWaveShr1 = 0x138;  // lane l <- lane l-1, lane 0 keeps `old`
 
 __device__ __forceinline__ int shr1(int old, int src) {
     return __builtin_amdgcn_update_dpp(old, src, kDppWaveShr1, 0xF, 0xF, false);
 }
+__device__ __forceinline__ int shl1(int v) { return __builtin_amdgcn_update_dpp(v, v, kDppWaveShl1, 0xF, 0xF, false); }
 
 // Copies a wave-uniform value into a VGPR the compiler cannot see through (a DPP
 // row_mirror of a uniform value is the value itself), so that per-step selects keep
@@ -55,11 +57,11 @@ struct TileCtx {
     uint16_t* dirs;     // kDirs: [step][lane] codes of this tile
 };
 
-// Loads the inputs of chunk cc (steps [64cc, 64cc+64)): lane 0's top boundary for
-// columns 64cc + lane, and the column bytes 64cc-64+lane and 64cc+lane (0x100 pads
-// never equal a byte).
+// Loads the inputs of chunk cc (steps [64cc, 64cc+64)) one column per lane: the top
+// boundary {H+O, F+E} and the column byte of column 64cc + lane (0x100 pads never equal
+// a byte).
 __device__ __forceinline__ void load_chunk_inputs(const SweepParams& p, const TileCtx& T, int64_t cc,
-                                                  int2& in, uint32_t& b0, uint32_t& b1) {
+                                                  int2& in, int& b1) {
     const int lane = wave_lane();
     const int64_t col = 64 * cc + lane;
     int H = 0, F = 0;
@@ -69,18 +71,21 @@ __device__ __forceinline__ void load_chunk_inputs(const SweepParams& p, const Ti
         F = v.y;
     }
     in = make_int2(H + p.sc.O, F + p.sc.E);
-    const int64_t cb0 = 64 * cc - 64 + lane, cb1 = 64 * cc + lane;
-    b0 = (cb0 >= 0 && cb0 < T.w) ? (uint32_t)p.B[T.C0 + cb0] : 0x100u;
-    b1 = (cb1 < T.w) ? (uint32_t)p.B[T.C0 + cb1] : 0x100u;
+    b1 = (col < T.w) ? (int)p.B[T.C0 + col] : 0x100;
 }
 
-// 64 steps of one tile.  EDGE: some (lane, step) pairs fall outside [0, w) and must
-// leave the lane state untouched.  DUMP: stage-1 chunk containing a checkpoint column.
+// 64 steps of one tile.  The chunk's inputs wait in register queues, one column per
+// lane: Qh/Qf hold the top boundary, Qb the column bytes; each step lane 0 takes the head
+// of the queues (DPP wave_shr with the queue as `old`) and the queues shift down one lane
+// (wave_shl), so the inner loop touches no memory but lane 63's output.  bl is the
+// column byte of the lane's current column, passed down the lanes with the diagonal.
+// EDGE: some (lane, step) pairs fall outside [0, w) and must leave the lane state
+// untouched.  DUMP: stage-1 chunk containing a checkpoint column.
 template <int R, int MODE, bool EDGE, bool DUMP>
 __device__ __forceinline__ void chunk_steps(const SweepParams& p, const TileCtx& T, int64_t c, int64_t row0,
                                             const uint32_t (&a)[R], int (&Ho)[R], int (&E)[R], int (&rmax)[R],
-                                            int& Ho_up_prev, int& Ho_bot, int& Fe_bot, const int2* s_in,
-                                            const uint32_t* s_b, int2* s_out, int& found_col) {
+                                            int& Ho_up_prev, int& Ho_bot, int& Fe_bot, int& Qh, int& Qf, int& Qb,
+                                            int& bl, int2* s_out, int& found_col) {
     const int lane = wave_lane();
     const int O = p.sc.O, Ev = p.sc.E;
     const int MO = to_vgpr(p.sc.M - O), XO = to_vgpr(p.sc.X - O);
@@ -89,10 +94,13 @@ __device__ __forceinline__ void chunk_steps(const SweepParams& p, const TileCtx&
     if (MODE == kDirs) dcol = T.dirs + (64 * c) * 64 + lane;
     SWK_UNROLL(UNROLL_STEPS)
     for (int s = 0; s < 64; ++s) {
-        const int2 in = s_in[s];
-        const uint32_t b = s_b[64 + s - lane];
-        int Ho_up = shr1(in.x, Ho_bot);
-        int Fe_up = shr1(in.y, Fe_bot);
+        int Ho_up = shr1(Qh, Ho_bot);
+        int Fe_up = shr1(Qf, Fe_bot);
+        bl = shr1(Qb, bl);
+        Qh = shl1(Qh);
+        Qf = shl1(Qf);
+        Qb = shl1(Qb);
+        const uint32_t b = (uint32_t)bl;
         bool valid = true;
         int64_t col = 0;
         if (EDGE || MODE == kSearch) {
@@ -152,11 +160,8 @@ __device__ __forceinline__ void chunk_steps(const SweepParams& p, const TileCtx&
     }
 }
 
-// Per-wave LDS of the sweep: lane 0's top-boundary inputs and the column bytes of the
-// current and next chunk, and lane 63's bottom-row outputs of the current chunk.
+// Per-wave LDS of the sweep: lane 63's bottom-row outputs of the current chunk.
 struct SweepLds {
-    int2 in[2][64];
-    uint32_t b[2][128];
     int2 out[64];
 };
 
@@ -166,8 +171,6 @@ struct SweepLds {
 template <int R, int MODE>
 __device__ void sweep_tile(const SweepParams& p, const TileCtx& T, const int2* left, int64_t left_r0, int corner,
                            int2* right, int64_t right_r0, bool first_segment, int32_t* found, SweepLds& lds) {
-    int2 (*s_in)[64] = lds.in;
-    uint32_t (*s_b)[128] = lds.b;
     int2* s_out = lds.out;
     const int lane = wave_lane();
     const int O = p.sc.O;
@@ -202,19 +205,12 @@ __device__ void sweep_tile(const SweepParams& p, const TileCtx& T, const int2* l
     const int64_t last_interior = (w - 64) / 64;
 
     int2 in_reg;
-    uint32_t b0, b1;
-    load_chunk_inputs(p, T, 0, in_reg, b0, b1);
-    s_in[0][lane] = in_reg;
-    s_b[0][lane] = b0;
-    s_b[0][64 + lane] = b1;
-    // LDS handoffs between lanes of the wave (here, after lane 63's last output of a
-    // chunk and after the next chunk's inputs) need no hardware barrier, the wave runs
-    // in lockstep, but must not be reordered by the compiler.
-    __builtin_amdgcn_wave_barrier();
+    int b_reg;
+    load_chunk_inputs(p, T, 0, in_reg, b_reg);
+    int Qh = in_reg.x, Qf = in_reg.y, Qb = b_reg, bl = 0x100;
 
     for (int64_t c = 0; c < nchunks; ++c) {
-        const int cur = (int)(c & 1);
-        if (c + 1 < nchunks) load_chunk_inputs(p, T, c + 1, in_reg, b0, b1);
+        if (c + 1 < nchunks) load_chunk_inputs(p, T, c + 1, in_reg, b_reg);
         const bool interior = c >= 1 && c <= last_interior;
         bool dump = false;
         if (MODE == kStage1 && p.scol) {
@@ -225,15 +221,17 @@ __device__ void sweep_tile(const SweepParams& p, const TileCtx& T, const int2* l
         if (!dump) {
             if (interior)
                 chunk_steps<R, MODE, false, false>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
-                                                   s_in[cur], s_b[cur], s_out, found_col);
+                                                   Qh, Qf, Qb, bl, s_out, found_col);
             else
                 chunk_steps<R, MODE, true, false>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
-                                                  s_in[cur], s_b[cur], s_out, found_col);
+                                                  Qh, Qf, Qb, bl, s_out, found_col);
         } else {
-            chunk_steps<R, MODE, true, true>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot, s_in[cur],
-                                             s_b[cur], s_out, found_col);
+            chunk_steps<R, MODE, true, true>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot, Qh, Qf,
+                                             Qb, bl, s_out, found_col);
         }
-        // Bottom row of the tile for columns 64c-63+lane.
+        // Bottom row of the tile for columns 64c-63+lane.  The LDS hand-off from lane 63 to
+        // the other lanes needs no hardware barrier (one wave, LDS operations in order) but
+        // must not be reordered by the compiler.
         __builtin_amdgcn_wave_barrier();
         {
             const int2 o = s_out[lane];
@@ -244,11 +242,9 @@ __device__ void sweep_tile(const SweepParams& p, const TileCtx& T, const int2* l
                 if (MODE == kStage1 && T.srow) T.srow[col] = v;
             }
         }
-        if (c + 1 < nchunks) {
-            s_in[cur ^ 1][lane] = in_reg;
-            s_b[cur ^ 1][lane] = b0;
-            s_b[cur ^ 1][64 + lane] = b1;
-        }
+        Qh = in_reg.x;
+        Qf = in_reg.y;
+        Qb = b_reg;
         __builtin_amdgcn_wave_barrier();
     }
     if (right) {
