@@ -39,7 +39,8 @@ struct swh_ctx {
     hipDeviceProp_t props{};
     DevBuf seqA, seqB, rbuf, cb, cor, rowmax, srow, scol, dirs, regions, key, found, walk, ops, xps, wregs;
     hipEvent_t ev[8] = {};
-    size_t lds_pad = 0;  // dynamic LDS per sweep workgroup: one workgroup per CU
+    size_t lds_pad = 0;  // dynamic LDS per sweep workgroup: `wps` workgroups per CU
+    int wps = 1;         // sweep waves per SIMD
     hipError_t last = hipSuccess;
 };
 
@@ -91,11 +92,11 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
-// Columns per tile: `want` if given, else one tile per SIMD (four per CU) across `cols`
+// Columns per tile: `want` if given, else one tile per resident sweep wave across `cols`
 // columns, at least 256 (a tile spends 63 extra steps filling and draining its lanes).
 int64_t tile_cols(swh_ctx* c, int64_t cols, int64_t want) {
     if (want > 0) return want;
-    const int64_t target = 4 * std::max(1, c->props.multiProcessorCount);
+    const int64_t target = (int64_t)kSweepWaves * c->wps * std::max(1, c->props.multiProcessorCount);
     return std::max<int64_t>(256, ceil_div(cols, target));
 }
 
@@ -462,12 +463,15 @@ int swh_ctx_create(int device, swh_ctx** out) {
         delete c;
         return SWH_ERR_HIP;
     }
-    // Reserve just over half a CU's LDS per sweep workgroup so that one fits a CU and its
-    // waves take one SIMD each (two tiles sharing a SIMD would double a launch's time).
-    const char* env = getenv("SWALIGN_SWEEP_SPREAD");
-    if (!(env && env[0] == '0')) {
+    // Reserve LDS per sweep workgroup so that exactly `wps` of them fit a CU: the waves
+    // of a launch then spread evenly, `wps` per SIMD (an uneven spread leaves some SIMDs
+    // with more tiles than others, and a launch lasts as long as its busiest SIMD).
+    const char* env = getenv("SWALIGN_SWEEP_WAVES_PER_SIMD");
+    if (env && atoi(env) >= 1 && atoi(env) <= 4) c->wps = atoi(env);
+    const char* spread = getenv("SWALIGN_SWEEP_SPREAD");
+    if (!(spread && spread[0] == '0')) {
         const size_t per_cu = (size_t)c->props.maxSharedMemoryPerMultiProcessor;
-        const size_t stat = sweep_static_lds(), want = per_cu / 2 + 1024;
+        const size_t stat = sweep_static_lds(), want = per_cu / (size_t)(c->wps + 1) + 1024;
         if (want > stat && want <= (size_t)c->props.sharedMemPerBlock) {
             c->lds_pad = want - stat;
             if (sweep_allow_lds(c->lds_pad) != hipSuccess) {
