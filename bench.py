@@ -45,13 +45,13 @@ def cpu_baseline(a, b, budget_s=15.0):
     orc = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(orc)
     A, B = (a, b) if len(a) >= len(b) else (b, a)
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
     sc = orc.scoring(1, -3, -5, -2)
     # calibrate on growing bands until one takes a second, then size the sample to budget_s
     rows = 256
     while True:
         t0 = time.perf_counter()
-        orc.forward(A[:rows], B, sc, threads=threads)
+        orc.forward(A[:rows], B, sc, threads=threads, end_col=False)
         dt = time.perf_counter() - t0
         if dt >= 1.0 or rows >= len(A):
             break
@@ -59,10 +59,10 @@ def cpu_baseline(a, b, budget_s=15.0):
     rate = rows * len(B) / max(dt, 1e-6)
     rows = int(min(len(A), max(1024, rate * budget_s / len(B))))
     t0 = time.perf_counter()
-    orc.forward(A[:rows], B, sc, threads=threads)
+    orc.forward(A[:rows], B, sc, threads=threads, end_col=False)
     dt = time.perf_counter() - t0
     return dict(value=round(rows * len(B) / dt / 1e9, 3), unit="GCUPS", cores=threads, kind="port",
-                sample="oracle forward pass (score + end cell, no traceback) over the first %d rows x %d "
+                sample="oracle forward pass (score and per-row maxima, no traceback) over the first %d rows x %d "
                        "columns of the same pair, %d host threads, %.1f s" % (rows, len(B), threads, dt))
 
 
@@ -105,13 +105,16 @@ def main():
     def step():
         return ctx.align_device(da.data_ptr(), len(a), db.data_ptr(), len(b), sc, K=K)
 
-    for _ in range(args.warmup):
+    for k in range(args.warmup):
+        t1 = time.perf_counter()
         step()
+        print("bench: warmup step %d done in %.2f s" % (k, time.perf_counter() - t1), file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     results = []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for k in range(args.steps):
         results.append(step())
+        print("bench: step %d done (%.2f s elapsed)" % (k, time.perf_counter() - t0), file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
 

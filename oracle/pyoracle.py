@@ -50,7 +50,7 @@ def scoring(m, x, o, e):
     return Scoring(m, x, o, e)
 
 
-def forward(a, b, sc, rowmax=False, threads=1):
+def forward(a, b, sc, rowmax=False, threads=1, end_col=True):
     A, pa, m = _buf(a)
     B, pb, n = _buf(b)
     s = C.c_int32()
@@ -60,7 +60,7 @@ def forward(a, b, sc, rowmax=False, threads=1):
     prm = rm.ctypes.data_as(C.POINTER(C.c_int32)) if rowmax else None
     if threads > 1:
         rc = lib().orc_forward_mt(pa, C.c_int64(m), pb, C.c_int64(n), C.byref(sc), C.c_int(threads),
-                                  C.byref(s), C.byref(ei), C.byref(ej), prm)
+                                  C.byref(s), C.byref(ei), C.byref(ej) if end_col else None, prm)
     else:
         rc = lib().orc_forward(pa, C.c_int64(m), pb, C.c_int64(n), C.byref(sc), C.byref(s),
                                C.byref(ei), C.byref(ej), prm)

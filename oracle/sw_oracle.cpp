@@ -240,7 +240,7 @@ int orc_forward_mt(const uint8_t* A, int64_t m, const uint8_t* B, int64_t n, con
     for (int64_t i = 0; i < m; ++i)
         if (rmax_all[i] > best) { best = rmax_all[i]; bi = i; }
     int64_t bj = -1;
-    if (best > 0) {
+    if (best > 0 && end_j) {
         // smallest column of row bi holding the maximum: recompute rows 0..bi (linear space)
         std::vector<int32_t> H(n, 0), F(n, 0);
         for (int64_t i = 0; i <= bi && bj < 0; ++i) {
@@ -260,7 +260,7 @@ int orc_forward_mt(const uint8_t* A, int64_t m, const uint8_t* B, int64_t n, con
     if (rowmax) memcpy(rowmax, rmax_all.data(), sizeof(int32_t) * m);
     *score = best;
     *end_i = bi;
-    *end_j = bj;
+    if (end_j) *end_j = bj;
     return 0;
 }
 

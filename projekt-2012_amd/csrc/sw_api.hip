@@ -39,8 +39,8 @@ struct swh_ctx {
     hipDeviceProp_t props{};
     DevBuf seqA, seqB, rbuf, cb, cor, rowmax, srow, scol, dirs, regions, key, found, walk, ops, xps, wregs;
     hipEvent_t ev[8] = {};
-    size_t lds_pad = 0;  // dynamic LDS per sweep workgroup: `wps` workgroups per CU
-    int wps = 1;         // sweep waves per SIMD
+    size_t lds_pad[3] = {0, 0, 0};  // dynamic LDS per sweep workgroup for w workgroups per CU
+    int wps_fixed = 0;              // sweep waves per SIMD forced by the environment, 0 = model
     hipError_t last = hipSuccess;
 };
 
@@ -92,12 +92,39 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
-// Columns per tile: `want` if given, else one tile per resident sweep wave across `cols`
-// columns, at least 256 (a tile spends 63 extra steps filling and draining its lanes).
-int64_t tile_cols(swh_ctx* c, int64_t cols, int64_t want) {
+// Columns per tile: `want` if given, else one tile per resident sweep wave (wps per SIMD)
+// across `cols` columns, at least 256 (a tile spends 63 extra steps filling and draining
+// its lanes).
+int64_t tile_cols(swh_ctx* c, int64_t cols, int64_t want, int wps = 1) {
     if (want > 0) return want;
-    const int64_t target = (int64_t)kSweepWaves * c->wps * std::max(1, c->props.multiProcessorCount);
+    const int64_t target = (int64_t)kSweepWaves * wps * std::max(1, c->props.multiProcessorCount);
     return std::max<int64_t>(256, ceil_div(cols, target));
+}
+
+// Relative time of a wave that shares its SIMD with a second sweep wave: two waves per
+// SIMD retire 1.28x the steps of one (measured on MI355X: the second wave issues while the
+// first waits on scalar, LDS and branch instructions).
+constexpr double kTwoWaveCost = 2.0 / 1.28;
+
+// Modelled duration (in tile steps) of a sweep over rows x cols with wps waves per SIMD:
+// one launch per tile anti-diagonal, each lasting ceil(tiles / resident waves) tile times.
+double sweep_model(swh_ctx* c, int64_t rows, int64_t cols, int wps) {
+    const int64_t W = tile_cols(c, cols, 0, wps);
+    const int64_t S = ceil_div(rows, kStripeRows), G = ceil_div(cols, W);
+    const int64_t P = (int64_t)kSweepWaves * wps * std::max(1, c->props.multiProcessorCount);
+    double t = 0;
+    for (int64_t d = 0; d < S + G - 1; ++d) {
+        const int64_t tiles = std::min(d, S - 1) - std::max<int64_t>(0, d - (G - 1)) + 1;
+        t += (double)ceil_div(tiles, P) * (double)(W + 63);
+    }
+    return t * (wps == 2 ? kTwoWaveCost : 1.0);
+}
+
+// Sweep waves per SIMD for a rows x cols sweep: whichever of one or two the model prefers.
+int choose_wps(swh_ctx* c, int64_t rows, int64_t cols) {
+    if (c->wps_fixed) return c->wps_fixed;
+    if (!c->lds_pad[2]) return 1;
+    return sweep_model(c, rows, cols, 2) < sweep_model(c, rows, cols, 1) ? 2 : 1;
 }
 
 // Runs a sweep over `regions` with tiles of kStripeRows x W: one launch per anti-diagonal
@@ -105,7 +132,7 @@ int64_t tile_cols(swh_ctx* c, int64_t cols, int64_t want) {
 // widest region's, or G_fix when given: kDirs lays codes out by it).  Optional events are
 // recorded immediately before the first and after the last launch.
 int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions, int64_t W, int64_t G_fix = 0,
-              hipEvent_t ev_before = nullptr, hipEvent_t ev_after = nullptr) {
+              hipEvent_t ev_before = nullptr, hipEvent_t ev_after = nullptr, int wps = 1) {
     int64_t S = 0, G = 0, cb_elems = 0, rb_elems = 0, cor_elems = 0;
     for (const Region& r : regions) {
         const int64_t rows = r.r1 - r.r0 + 1, cols = r.c1 - r.c0 + 1;
@@ -145,7 +172,7 @@ int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions,
         p.d = d;
         p.s_lo = s_lo;
         p.per = (int32_t)(s_hi - s_lo + 1);
-        HCK(c, launch_sweep(mode, p, c->lds_pad, c->stream));
+        HCK(c, launch_sweep(mode, p, c->lds_pad[wps == 2 ? 2 : 1], c->stream));
     }
     if (ev_after) HCK(c, hipEventRecord(ev_after, c->stream));
     return SWH_OK;
@@ -231,7 +258,8 @@ int align_core(swh_ctx* c, const uint8_t* dA, int64_t m, const uint8_t* dB, int6
         r.c0 = 0;
         r.c1 = n - 1;
         std::vector<Region> regions{r};
-        rc = run_sweep(c, kStage1, p, regions, tile_cols(c, n, want_w), 0, c->ev[0], c->ev[1]);
+        const int wps = want_w > 0 ? 1 : choose_wps(c, m, n);
+        rc = run_sweep(c, kStage1, p, regions, tile_cols(c, n, want_w, wps), 0, c->ev[0], c->ev[1], wps);
         if (rc) return rc;
     }
     HCK(c, hipMemsetAsync(c->key.p, 0, 8, c->stream));
@@ -463,22 +491,19 @@ int swh_ctx_create(int device, swh_ctx** out) {
         delete c;
         return SWH_ERR_HIP;
     }
-    // Reserve LDS per sweep workgroup so that exactly `wps` of them fit a CU: the waves
-    // of a launch then spread evenly, `wps` per SIMD (an uneven spread leaves some SIMDs
+    // Reserve LDS per sweep workgroup so that exactly w of them fit a CU (w = 1, 2): the
+    // waves of a launch then spread evenly, w per SIMD (an uneven spread leaves some SIMDs
     // with more tiles than others, and a launch lasts as long as its busiest SIMD).
     const char* env = getenv("SWALIGN_SWEEP_WAVES_PER_SIMD");
-    if (env && atoi(env) >= 1 && atoi(env) <= 4) c->wps = atoi(env);
-    const char* spread = getenv("SWALIGN_SWEEP_SPREAD");
-    if (!(spread && spread[0] == '0')) {
-        const size_t per_cu = (size_t)c->props.maxSharedMemoryPerMultiProcessor;
-        const size_t stat = sweep_static_lds(), want = per_cu / (size_t)(c->wps + 1) + 1024;
-        if (want > stat && want <= (size_t)c->props.sharedMemPerBlock) {
-            c->lds_pad = want - stat;
-            if (sweep_allow_lds(c->lds_pad) != hipSuccess) {
-                (void)hipGetLastError();
-                c->lds_pad = 0;
-            }
-        }
+    if (env && (atoi(env) == 1 || atoi(env) == 2)) c->wps_fixed = atoi(env);
+    const size_t per_cu = (size_t)c->props.maxSharedMemoryPerMultiProcessor, stat = sweep_static_lds();
+    for (int w = 1; w <= 2; ++w) {
+        const size_t want = per_cu / (size_t)(w + 1) + 1024;
+        if (want > stat && want <= (size_t)c->props.sharedMemPerBlock) c->lds_pad[w] = want - stat;
+    }
+    if (c->lds_pad[1] && sweep_allow_lds(c->lds_pad[1]) != hipSuccess) {
+        (void)hipGetLastError();
+        c->lds_pad[1] = c->lds_pad[2] = 0;
     }
     *out = c;
     return SWH_OK;

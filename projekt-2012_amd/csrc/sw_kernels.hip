@@ -404,53 +404,66 @@ __device__ void walk_main(const WalkArgs& a, WalkLds<R>& lds) {
         }
         __syncthreads();
         if (lane == 0) {
+            // window-local coordinates (32-bit); the walk stays in the window while gi, gj >= 0
+            const int64_t i0 = rgn.r0 + wr0, j0 = rgn.c0 + wc0;
+            int gi = (int)(st.i - i0), gj = (int)(st.j - j0);
+            int layer = st.layer;
+            int64_t nops = st.nops, nxp = st.nxp, last_row = st.last_row;
+            int64_t xrow = ((st.i + 1) / a.K) * a.K - 1;  // checkpoint row at or above the row
+            uint8_t* ops = a.ops;
             while (true) {
-                if (st.i < 0 || st.j < 0) {
-                    st.done = 1;
+                if (gi < 0 || gj < 0) {
+                    if (i0 + gi < 0 || j0 + gj < 0) st.done = 1;  // walked past row or column 0
                     break;
                 }
-                const int64_t ri = st.i - rgn.r0, rj = st.j - rgn.c0;
-                if (ri < wr0 || rj < wc0) break;  // left the window (or the region)
-                const int64_t gi = ri - wr0;
-                const uint32_t d = (win[gi / R][rj - wc0] >> (4 * (gi % R))) & 15u;
-                if (st.i != st.last_row) {
-                    st.last_row = st.i;
-                    if (a.srow && (st.i + 1) % a.K == 0 && st.nxp < a.xcap) {
-                        const int2 v = a.srow[((st.i + 1) / a.K - 1) * a.n + st.j];
+                const uint32_t d = (win[gi / R][gj] >> (4 * (gi % R))) & 15u;
+                const int64_t i = i0 + gi;
+                if (i != last_row) {
+                    last_row = i;
+                    if (xrow > i) xrow -= a.K;
+                    if (a.srow && i == xrow && nxp < a.xcap) {
+                        const int64_t j = j0 + gj;
+                        const int2 v = a.srow[((i + 1) / a.K - 1) * a.n + j];
                         XPoint xp;
-                        xp.row = st.i;
-                        xp.col = st.j;
-                        xp.score = st.layer == 2 ? v.y : v.x;
-                        xp.layer = st.layer;
-                        a.xps[st.nxp++] = xp;
+                        xp.row = i;
+                        xp.col = j;
+                        xp.score = layer == 2 ? v.y : v.x;
+                        xp.layer = layer;
+                        a.xps[nxp++] = xp;
                     }
                 }
-                if (st.layer == 0) {
+                if (layer == 0) {
                     const uint32_t h = d & 3u;
                     if (h == 1u) {
-                        a.ops[st.nops++] = (lds.a[ri - wr0] == lds.b[rj - wc0]) ? (uint8_t)'=' : (uint8_t)'X';
-                        st.start_i = st.i;
-                        st.start_j = st.j;
-                        --st.i;
-                        --st.j;
+                        ops[nops++] = (lds.a[gi] == lds.b[gj]) ? (uint8_t)'=' : (uint8_t)'X';
+                        st.start_i = i;
+                        st.start_j = j0 + gj;
+                        --gi;
+                        --gj;
                     } else if (h == 2u) {
-                        st.layer = 2;
+                        layer = 2;
                     } else if (h == 3u) {
-                        st.layer = 1;
+                        layer = 1;
                     } else {
                         st.done = 1;
                         break;
                     }
-                } else if (st.layer == 2) {
-                    a.ops[st.nops++] = (uint8_t)'D';
-                    st.layer = (d >> 3) & 1u ? 2 : 0;
-                    --st.i;
+                } else if (layer == 2) {
+                    ops[nops++] = (uint8_t)'D';
+                    layer = (d >> 3) & 1u ? 2 : 0;
+                    --gi;
                 } else {
-                    a.ops[st.nops++] = (uint8_t)'I';
-                    st.layer = (d >> 2) & 1u ? 1 : 0;
-                    --st.j;
+                    ops[nops++] = (uint8_t)'I';
+                    layer = (d >> 2) & 1u ? 1 : 0;
+                    --gj;
                 }
             }
+            st.i = i0 + gi;
+            st.j = j0 + gj;
+            st.layer = layer;
+            st.nops = nops;
+            st.nxp = nxp;
+            st.last_row = last_row;
             sst = st;
         }
         __syncthreads();
