@@ -55,7 +55,7 @@ def forward(a, b, sc, rowmax=False, threads=1, end_col=True):
     B, pb, n = _buf(b)
     s = C.c_int32()
     ei = C.c_int64()
-    ej = C.c_int64()
+    ej = C.c_int64(-1)
     rm = np.zeros(max(m, 1), dtype=np.int32) if rowmax else None
     prm = rm.ctypes.data_as(C.POINTER(C.c_int32)) if rowmax else None
     if threads > 1:

@@ -197,7 +197,7 @@ int orc_forward(const uint8_t* A, int64_t m, const uint8_t* B, int64_t n, const 
     }
     *score = best;
     *end_i = bi;
-    *end_j = bj;
+    if (end_j) *end_j = bj;
     return 0;
 }
 

@@ -58,8 +58,8 @@ int orc_forward(const uint8_t* A, int64_t m, const uint8_t* B, int64_t n,
                 int32_t* rowmax);
 
 /* Same result, tiled anti-diagonal wavefront over `threads` host threads.  end_j may be
- * NULL: the score, end row and row maxima are then computed without the (single-thread)
- * end-column search. */
+ * NULL (here and in orc_forward): the score, end row and row maxima are then computed
+ * without the (single-thread) end-column search. */
 int orc_forward_mt(const uint8_t* A, int64_t m, const uint8_t* B, int64_t n,
                    const orc_scoring* sc, int threads, int32_t* score, int64_t* end_i,
                    int64_t* end_j, int32_t* rowmax);

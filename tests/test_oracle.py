@@ -146,6 +146,11 @@ def test_multithreaded_forward_equals_single(orc):
     (s1, rm1) = orc.forward(a, b, sc, rowmax=True)
     (s8, rm8) = orc.forward(a, b, sc, rowmax=True, threads=8)
     assert s1 == s8 and np.array_equal(rm1, rm8)
+    # without the end-column search (the CPU baseline's mode), small and large inputs
+    for x, y in [(a[:300], b[:200]), (a, b)]:
+        full = orc.forward(x, y, sc, threads=8)
+        part = orc.forward(x, y, sc, threads=8, end_col=False)
+        assert part[:2] == full[:2] and part[2] == -1
 
 
 @pytest.mark.parametrize("name", ["test1", "test2"])
