@@ -94,7 +94,7 @@ typedef struct {
     double ms_traceback;      /* block recomputation + walk                       */
     double ms_total;          /* device work of the call                          */
     int32_t precision_used;   /* 32 or 16                                        */
-    int32_t _pad;
+    int32_t forward_launches; /* kernel launches of the stage-1 sweep             */
 } swh_result;
 
 typedef struct swh_ctx swh_ctx;

@@ -132,7 +132,8 @@ int choose_wps(swh_ctx* c, int64_t rows, int64_t cols) {
 // widest region's, or G_fix when given: kDirs lays codes out by it).  Optional events are
 // recorded immediately before the first and after the last launch.
 int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions, int64_t W, int64_t G_fix = 0,
-              hipEvent_t ev_before = nullptr, hipEvent_t ev_after = nullptr, int wps = 1) {
+              hipEvent_t ev_before = nullptr, hipEvent_t ev_after = nullptr, int wps = 1,
+              int64_t* launches = nullptr) {
     int64_t S = 0, G = 0, cb_elems = 0, rb_elems = 0, cor_elems = 0;
     for (const Region& r : regions) {
         const int64_t rows = r.r1 - r.r0 + 1, cols = r.c1 - r.c0 + 1;
@@ -175,6 +176,7 @@ int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions,
         HCK(c, launch_sweep(mode, p, c->lds_pad[wps == 2 ? 2 : 1], c->stream));
     }
     if (ev_after) HCK(c, hipEventRecord(ev_after, c->stream));
+    if (launches) *launches = S + G - 1;
     return SWH_OK;
 }
 
@@ -259,7 +261,10 @@ int align_core(swh_ctx* c, const uint8_t* dA, int64_t m, const uint8_t* dB, int6
         r.c1 = n - 1;
         std::vector<Region> regions{r};
         const int wps = want_w > 0 ? 1 : choose_wps(c, m, n);
-        rc = run_sweep(c, kStage1, p, regions, tile_cols(c, n, want_w, wps), 0, c->ev[0], c->ev[1], wps);
+        int64_t launches = 0;
+        rc = run_sweep(c, kStage1, p, regions, tile_cols(c, n, want_w, wps), 0, c->ev[0], c->ev[1], wps,
+                       &launches);
+        out->forward_launches = (int32_t)launches;
         if (rc) return rc;
     }
     HCK(c, hipMemsetAsync(c->key.p, 0, 8, c->stream));

@@ -49,7 +49,7 @@ class Result(C.Structure):
                 ("checkpoint_rows", C.c_int64), ("checkpoint_cols", C.c_int64),
                 ("ms_forward", C.c_double), ("ms_end_search", C.c_double),
                 ("ms_traceback", C.c_double), ("ms_total", C.c_double),
-                ("precision_used", C.c_int32), ("_pad", C.c_int32)]
+                ("precision_used", C.c_int32), ("forward_launches", C.c_int32)]
 
 
 class Pair(C.Structure):
@@ -119,7 +119,8 @@ def _result_dict(r):
              cigar=r.cigar.decode() if r.cigar else None, n_ops=r.n_ops,
              checkpoint_rows=r.checkpoint_rows, checkpoint_cols=r.checkpoint_cols,
              ms_forward=r.ms_forward, ms_end_search=r.ms_end_search, ms_traceback=r.ms_traceback,
-             ms_total=r.ms_total, precision=r.precision_used)
+             ms_total=r.ms_total, precision=r.precision_used,
+             forward_launches=r.forward_launches)
     if r.xpoints and r.n_xpoints:
         d["xpoints"] = [(r.xpoints[k].row, r.xpoints[k].col, r.xpoints[k].score, r.xpoints[k].layer)
                         for k in range(r.n_xpoints)]
