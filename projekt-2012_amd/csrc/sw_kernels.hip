@@ -339,16 +339,20 @@ __global__ __launch_bounds__(256) void k_argmax_rows(const int32_t* __restrict__
     argmax_rows_main(rowmax, m, key);
 }
 
-// Greedy backward walk over the regions of one traceback round (see WalkArgs).  A
-// region's direction codes are staged through LDS in 128x128-cell windows together with
-// the sequence bytes of the window; lane 0 walks.  The walk stops at the start of the
-// alignment or when it reaches a cell outside every region of the round.
-constexpr int kWalkWinRows = 128, kWalkWinCols = 128;
+// Greedy backward walk over the regions of one traceback round (see WalkArgs).  The codes
+// of a tile are stored step by step ([step][lane], one 16-bit word per lane = 4 rows), so
+// a window of kWalkSteps consecutive steps of one tile is one contiguous, coalesced read:
+// it covers the tile's whole stripe of 64R rows along a band of anti-diagonals, which the
+// path (moving up and left, so to lower steps) leaves only by its top, its left edge or
+// the window's first step.  Windows are staged in LDS with the sequence bytes they need;
+// lane 0 walks.  The walk stops at the start of the alignment or at a cell outside every
+// region of the round.
+constexpr int kWalkSteps = 256;
 
 template <int R>
 struct WalkLds {
-    uint16_t win[kWalkWinRows / R][kWalkWinCols];
-    uint8_t a[kWalkWinRows], b[kWalkWinCols];  // sequence bytes of the window's rows, columns
+    uint16_t win[kWalkSteps * 64];                 // steps [t0, t0 + kWalkSteps) x 64 lanes
+    uint8_t a[64 * R], b[kWalkSteps + 64];         // rows of the stripe, columns of the band
     WalkState sst;
 };
 
@@ -364,9 +368,7 @@ __device__ __forceinline__ const WalkRegion* walk_region(const WalkArgs& a, int6
 
 template <int R>
 __device__ void walk_main(const WalkArgs& a, WalkLds<R>& lds) {
-    constexpr int WR = kWalkWinRows, WC = kWalkWinCols, NG = WR / R;
-    constexpr int64_t HR = 64 * R;
-    auto& win = lds.win;
+    constexpr int HR = 64 * R;
     WalkState& sst = lds.sst;
     const int lane = threadIdx.x;
     if (lane == 0) sst = *a.st;
@@ -377,52 +379,49 @@ __device__ void walk_main(const WalkArgs& a, WalkLds<R>& lds) {
         const WalkRegion* rg = walk_region(a, st.i, st.j);
         if (!rg) break;  // outside the round's regions
         const WalkRegion rgn = *rg;
+        // tile of the current cell and the window's steps [t0, t1]
         const int64_t il = st.i - rgn.r0, jl = st.j - rgn.c0;
-        // rows [wr0, wr0+WR) with wr0 a multiple of R and il inside the window
-        int64_t wr0 = il - (WR - R);
-        if (wr0 < 0) wr0 = 0;
-        wr0 -= wr0 % R;
-        int64_t wc0 = jl - WC + 1;
-        if (wc0 < 0) wc0 = 0;
-        for (int idx = lane; idx < NG * WC; idx += 64) {
-            const int g = idx / WC, cc = idx % WC;
-            const int64_t row = wr0 + (int64_t)g * R, col = wc0 + cc;
-            uint16_t v = 0;
-            if (row < rgn.rows && col < rgn.cols) {
-                const int64_t s = row / HR, l = (row % HR) / R, sg = col / a.W, t = col % a.W + l;
-                v = rgn.dirs[((s * a.G + sg) * a.tstride + t) * 64 + l];
-            }
-            win[g][cc] = v;
+        const int64_t s = il / HR, sg = jl / a.W;
+        const int64_t R0 = rgn.r0 + s * HR, C0 = rgn.c0 + sg * a.W;  // tile origin (absolute)
+        const int ri0 = (int)(st.i - R0), cj0 = (int)(st.j - C0);
+        const int t1 = cj0 + ri0 / R;
+        const int t0 = t1 >= kWalkSteps ? t1 - kWalkSteps + 1 : 0;
+        const uint16_t* tile = rgn.dirs + (s * a.G + sg) * a.tstride * 64;
+        {
+            // steps [t0, t1] are contiguous: (t1 - t0 + 1) * 64 codes
+            const uint4* src = (const uint4*)(tile + (int64_t)t0 * 64);
+            uint4* dst = (uint4*)lds.win;
+            const int n16 = (t1 - t0 + 1) * 64 * 2 / 16;
+            for (int k = lane; k < n16; k += 64) dst[k] = src[k];
         }
-        for (int k = lane; k < WR; k += 64) {
-            const int64_t row = wr0 + k;
-            lds.a[k] = row < rgn.rows ? a.A[rgn.r0 + row] : 0;
-        }
-        for (int k = lane; k < WC; k += 64) {
-            const int64_t col = wc0 + k;
-            lds.b[k] = col < rgn.cols ? a.B[rgn.c0 + col] : 0;
+        const int64_t rows_here = min((int64_t)HR, rgn.r0 + rgn.rows - R0);
+        for (int k = lane; k < HR; k += 64) lds.a[k] = k < rows_here ? a.A[R0 + k] : 0;
+        // columns of the band: col = t - l for t in [t0, t1], l in [0, 64): [t0 - 63, t1]
+        for (int k = lane; k < kWalkSteps + 64; k += 64) {
+            const int cj = t0 - 63 + k;
+            lds.b[k] = (cj >= 0 && cj <= t1) ? a.B[C0 + cj] : 0;
         }
         __syncthreads();
         if (lane == 0) {
-            // window-local coordinates (32-bit); the walk stays in the window while gi, gj >= 0
-            const int64_t i0 = rgn.r0 + wr0, j0 = rgn.c0 + wc0;
-            int gi = (int)(st.i - i0), gj = (int)(st.j - j0);
+            int ri = ri0, cj = cj0;  // tile-local row and column
             int layer = st.layer;
             int64_t nops = st.nops, nxp = st.nxp, last_row = st.last_row;
             int64_t xrow = ((st.i + 1) / a.K) * a.K - 1;  // checkpoint row at or above the row
             uint8_t* ops = a.ops;
             while (true) {
-                if (gi < 0 || gj < 0) {
-                    if (i0 + gi < 0 || j0 + gj < 0) st.done = 1;  // walked past row or column 0
+                if (ri < 0 || cj < 0) {
+                    if (R0 + ri < 0 || C0 + cj < 0) st.done = 1;  // walked past row or column 0
                     break;
                 }
-                const uint32_t d = (win[gi / R][gj] >> (4 * (gi % R))) & 15u;
-                const int64_t i = i0 + gi;
+                const int l = ri / R, t = cj + l;
+                if (t < t0) break;  // left the window
+                const uint32_t d = (lds.win[(t - t0) * 64 + l] >> (4 * (ri % R))) & 15u;
+                const int64_t i = R0 + ri;
                 if (i != last_row) {
                     last_row = i;
                     if (xrow > i) xrow -= a.K;
                     if (a.srow && i == xrow && nxp < a.xcap) {
-                        const int64_t j = j0 + gj;
+                        const int64_t j = C0 + cj;
                         const int2 v = a.srow[((i + 1) / a.K - 1) * a.n + j];
                         XPoint xp;
                         xp.row = i;
@@ -435,11 +434,11 @@ __device__ void walk_main(const WalkArgs& a, WalkLds<R>& lds) {
                 if (layer == 0) {
                     const uint32_t h = d & 3u;
                     if (h == 1u) {
-                        ops[nops++] = (lds.a[gi] == lds.b[gj]) ? (uint8_t)'=' : (uint8_t)'X';
+                        ops[nops++] = (lds.a[ri] == lds.b[cj - (t0 - 63)]) ? (uint8_t)'=' : (uint8_t)'X';
                         st.start_i = i;
-                        st.start_j = j0 + gj;
-                        --gi;
-                        --gj;
+                        st.start_j = C0 + cj;
+                        --ri;
+                        --cj;
                     } else if (h == 2u) {
                         layer = 2;
                     } else if (h == 3u) {
@@ -451,15 +450,15 @@ __device__ void walk_main(const WalkArgs& a, WalkLds<R>& lds) {
                 } else if (layer == 2) {
                     ops[nops++] = (uint8_t)'D';
                     layer = (d >> 3) & 1u ? 2 : 0;
-                    --gi;
+                    --ri;
                 } else {
                     ops[nops++] = (uint8_t)'I';
                     layer = (d >> 2) & 1u ? 1 : 0;
-                    --gj;
+                    --cj;
                 }
             }
-            st.i = i0 + gi;
-            st.j = j0 + gj;
+            st.i = R0 + ri;
+            st.j = C0 + cj;
             st.layer = layer;
             st.nops = nops;
             st.nxp = nxp;
