@@ -348,11 +348,17 @@ __global__ __launch_bounds__(256) void k_argmax_rows(const int32_t* __restrict__
 // lane 0 walks.  The walk stops at the start of the alignment or at a cell outside every
 // region of the round.
 constexpr int kWalkSteps = 256;
+// Ops of one window: the path stays in the window's 64R rows and kWalkSteps + 63 columns.
+constexpr int kWalkOps = 64 * kRowsPerLane + kWalkSteps + 64;
 
 template <int R>
 struct WalkLds {
     uint16_t win[kWalkSteps * 64];                 // steps [t0, t0 + kWalkSteps) x 64 lanes
     uint8_t a[64 * R], b[kWalkSteps + 64];         // rows of the stripe, columns of the band
+    uint8_t ops[kWalkOps];                         // ops of the window, flushed when it ends
+    XPoint xp;                                     // crosspoint of the window (at most one)
+    int32_t nw, nxw;                               // ops and crosspoints of the window
+    int64_t ops_at;                                // where the window's ops go in a.ops
     WalkState sst;
 };
 
@@ -403,11 +409,15 @@ __device__ void walk_main(const WalkArgs& a, WalkLds<R>& lds) {
         }
         __syncthreads();
         if (lane == 0) {
+            // Lane 0 walks.  Ops and the crosspoint go to LDS and are flushed by the whole
+            // wave after the window, so the loop issues no global memory operation (a
+            // pending one would make every LDS read wait for it).
+            static_assert(64 * R + kWalkSteps + 64 <= kWalkOps, "window ops buffer");
             int ri = ri0, cj = cj0;  // tile-local row and column
             int layer = st.layer;
-            int64_t nops = st.nops, nxp = st.nxp, last_row = st.last_row;
+            int nw = 0, nxw = 0;
+            int64_t last_row = st.last_row;
             int64_t xrow = ((st.i + 1) / a.K) * a.K - 1;  // checkpoint row at or above the row
-            uint8_t* ops = a.ops;
             while (true) {
                 if (ri < 0 || cj < 0) {
                     if (R0 + ri < 0 || C0 + cj < 0) st.done = 1;  // walked past row or column 0
@@ -420,21 +430,17 @@ __device__ void walk_main(const WalkArgs& a, WalkLds<R>& lds) {
                 if (i != last_row) {
                     last_row = i;
                     if (xrow > i) xrow -= a.K;
-                    if (a.srow && i == xrow && nxp < a.xcap) {
-                        const int64_t j = C0 + cj;
-                        const int2 v = a.srow[((i + 1) / a.K - 1) * a.n + j];
-                        XPoint xp;
-                        xp.row = i;
-                        xp.col = j;
-                        xp.score = layer == 2 ? v.y : v.x;
-                        xp.layer = layer;
-                        a.xps[nxp++] = xp;
+                    if (i == xrow && nxw == 0) {
+                        lds.xp.row = i;
+                        lds.xp.col = C0 + cj;
+                        lds.xp.layer = layer;
+                        nxw = 1;
                     }
                 }
                 if (layer == 0) {
                     const uint32_t h = d & 3u;
                     if (h == 1u) {
-                        ops[nops++] = (lds.a[ri] == lds.b[cj - (t0 - 63)]) ? (uint8_t)'=' : (uint8_t)'X';
+                        lds.ops[nw++] = (lds.a[ri] == lds.b[cj - (t0 - 63)]) ? (uint8_t)'=' : (uint8_t)'X';
                         st.start_i = i;
                         st.start_j = C0 + cj;
                         --ri;
@@ -448,22 +454,37 @@ __device__ void walk_main(const WalkArgs& a, WalkLds<R>& lds) {
                         break;
                     }
                 } else if (layer == 2) {
-                    ops[nops++] = (uint8_t)'D';
+                    lds.ops[nw++] = (uint8_t)'D';
                     layer = (d >> 3) & 1u ? 2 : 0;
                     --ri;
                 } else {
-                    ops[nops++] = (uint8_t)'I';
+                    lds.ops[nw++] = (uint8_t)'I';
                     layer = (d >> 2) & 1u ? 1 : 0;
                     --cj;
                 }
             }
+            lds.nw = nw;
+            lds.nxw = nxw;
+            lds.ops_at = st.nops;
             st.i = R0 + ri;
             st.j = C0 + cj;
             st.layer = layer;
-            st.nops = nops;
-            st.nxp = nxp;
+            st.nops += nw;
             st.last_row = last_row;
             sst = st;
+        }
+        __syncthreads();
+        {
+            const int nw = lds.nw;
+            uint8_t* dst = a.ops + lds.ops_at;
+            for (int k = lane; k < nw; k += 64) dst[k] = lds.ops[k];
+            if (lane == 0 && lds.nxw && a.srow && sst.nxp < a.xcap) {
+                XPoint xp = lds.xp;
+                const int2 v = a.srow[((xp.row + 1) / a.K - 1) * a.n + xp.col];
+                xp.score = xp.layer == 2 ? v.y : v.x;
+                a.xps[sst.nxp] = xp;
+                sst.nxp += 1;
+            }
         }
         __syncthreads();
     }
