@@ -35,7 +35,112 @@ WORKLOADS = {
        "checkpoint rows every 8192, full traceback",
     2: "10 Mbp x 10 Mbp pair, GC 40%, tandem repeats over 1%, 2 Mbp homology (5% sub, 1% indel), "
        "+2/-3/-5/-1, full traceback",
+    3: "65,536 pairs, target lengths log-uniform in 100 bp - 20 kbp, query = target at 10% sub / 3% indel "
+       "with flanks trimmed or extended by up to 20%, +1/-3/-5/-2, full traceback, all pairs in one batch",
 }
+METRIC = "GCUPS on 1 MI355X (single long pair, batched mix) + served p50/p99 vs plain"
+
+
+def roofline(cells, fwd_ms, launches):
+    achieved = cells * OPS_PER_CELL / (fwd_ms / 1000.0) / 1e12
+    return {
+        "bound": "valu",
+        "kernel": "k_sweep<4, kStage1> (forward sweep with checkpoints)",
+        "achieved": round(achieved, 3),
+        "peak": round(PEAK_INT32_TOPS, 2),
+        "unit": "TOP/s",
+        "frac": round(achieved / PEAK_INT32_TOPS, 4),
+        "traffic": None,
+        "launches": launches,
+        "avg_launch_us": round(fwd_ms * 1000.0 / max(launches, 1), 2),
+        "ops_per_launch": int(cells * OPS_PER_CELL / max(launches, 1)),
+    }
+
+
+def run_batch(args, torch, sw):
+    """Workload 3: the whole batch is one step (swh_batch_align_device), inputs resident in
+    device memory.  The plain path (swh_align_pair_device pair after pair) is timed on the
+    first pairs of the same batch for comparison."""
+    pairs, sc = sw.workload_batch()
+    ctx = sw.Context(0)
+    blob = b"".join(a + b for a, b in pairs)
+    dev = torch.frombuffer(bytearray(blob + b"\0"), dtype=torch.uint8).to("cuda:0")
+    base, off, dp = dev.data_ptr(), 0, []
+    for a, b in pairs:
+        dp.append((base + off, len(a), base + off + len(a), len(b)))
+        off += len(a) + len(b)
+    cells = sum(len(a) * len(b) for a, b in pairs)
+    torch.cuda.synchronize()
+    for k in range(args.warmup):
+        ctx.batch_device(dp, sc)
+        print("bench: warmup step %d done in %.2f s" % (k, ctx.last_call_s), file=sys.stderr, flush=True)
+    times, res = [], None
+    for k in range(args.steps):
+        res, st = ctx.batch_device(dp, sc)
+        assert all(x == 0 for x in st)
+        times.append(ctx.last_call_s)
+        print("bench: step %d done in %.2f s" % (k, ctx.last_call_s), file=sys.stderr, flush=True)
+    sec = statistics.median(times)
+    r0 = res[0]
+    # plain path: one swh_align_pair_device per pair, on a prefix of the batch
+    nplain, t_plain, cells_plain = 0, 0.0, 0
+    t0 = time.perf_counter()
+    for (p1, l1, p2, l2), (a, b) in zip(dp, pairs):
+        ctx.align_device(p1, l1, p2, l2, sc)
+        nplain += 1
+        cells_plain += len(a) * len(b)
+        if time.perf_counter() - t0 > 10.0:
+            break
+    t_plain = time.perf_counter() - t0
+    plain_gcups = cells_plain / t_plain / 1e9
+    gcups = cells / sec / 1e9
+    line = {
+        "metric": METRIC, "value": round(gcups, 2), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(sec * 1000.0, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic (numpy PCG64 lengths and targets, seed 2015; queries by swh_synth_mutate)",
+        "config": {
+            "workload": WORKLOADS[3], "pairs": len(pairs), "cells": cells,
+            "stage_ms": {"forward": round(r0["ms_forward"], 3), "end_search": round(r0["ms_end_search"], 3),
+                         "traceback": round(r0["ms_traceback"], 3)},
+            "forward_gcups": round(cells / (r0["ms_forward"] / 1000.0) / 1e9, 2),
+            "plain_per_pair_gcups": round(plain_gcups, 2), "plain_sample_pairs": nplain,
+            "batch_vs_plain": round(gcups / plain_gcups, 2),
+        },
+        "roofline": roofline(cells, r0["ms_forward"], r0["forward_launches"]),
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline_pairs(pairs, sc)
+    print(json.dumps(line))
+    ctx.close()
+
+
+def cpu_baseline_pairs(pairs, sc, budget_s=15.0):
+    """The oracle's forward pass (single thread per pair, pairs over the host's cores) on a
+    prefix of the batch sized to about budget_s."""
+    import importlib.util
+    from concurrent.futures import ThreadPoolExecutor
+    spec = importlib.util.spec_from_file_location("pyoracle", os.path.join(ROOT, "oracle", "pyoracle.py"))
+    orc = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(orc)
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    osc = orc.scoring(sc.match, sc.mismatch, sc.gap_open, sc.gap_extend)
+
+    def one(p):
+        a, b = p if len(p[0]) >= len(p[1]) else (p[1], p[0])
+        orc.forward(a, b, osc, end_col=True)
+        return len(a) * len(b)
+
+    cells, t0, k = 0, time.perf_counter(), 0
+    with ThreadPoolExecutor(threads) as ex:
+        while time.perf_counter() - t0 < budget_s and k < len(pairs):
+            chunk = pairs[k:k + 4 * threads]
+            cells += sum(ex.map(one, chunk))
+            k += len(chunk)
+    dt = time.perf_counter() - t0
+    return dict(value=round(cells / dt / 1e9, 3), unit="GCUPS", cores=threads, kind="port",
+                sample="oracle forward pass (score + end cell) over the first %d pairs of the batch, one pair "
+                       "per host thread, %d threads, %.1f s" % (k, threads, dt))
 
 
 def cpu_baseline(a, b, budget_s=15.0):
@@ -83,6 +188,9 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.workload == 3:
+        import swalign as sw
+        return run_batch(args, torch, sw)
     if args.copy_probe:
         src = torch.ones(args.copy_probe, dtype=torch.uint8, device="cuda:0")
         dst = torch.empty_like(src)
@@ -126,10 +234,9 @@ def main():
     ms_step = elapsed * 1000.0 / args.steps
     gcups = cells / (elapsed / args.steps) / 1e9
     fwd_ms = statistics.mean(r["ms_forward"] for r in results)
-    achieved_tops = cells * OPS_PER_CELL / (fwd_ms / 1000.0) / 1e12
     clock_ghz = info["clock_khz"] / 1e6 if info["clock_khz"] else PEAK_CLOCK_GHZ
     line = {
-        "metric": "GCUPS on 1 MI355X (single long pair, batched mix) + served p50/p99 vs plain",
+        "metric": METRIC,
         "value": round(gcups, 2),
         "unit": "GCUPS",
         "n_gpus": 1,
@@ -155,15 +262,7 @@ def main():
             "forward_gcups": round(cells / (fwd_ms / 1000.0) / 1e9, 2),
             "device": info["name"], "cus": info["cus"], "clock_mhz": round(clock_ghz * 1000),
         },
-        "roofline": {
-            "bound": "valu",
-            "kernel": "k_sweep<4, kStage1> (forward sweep with checkpoints)",
-            "achieved": round(achieved_tops, 3),
-            "peak": round(PEAK_INT32_TOPS, 2),
-            "unit": "TOP/s",
-            "frac": round(achieved_tops / PEAK_INT32_TOPS, 4),
-            "traffic": None,
-        },
+        "roofline": roofline(cells, fwd_ms, r0["forward_launches"]),
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(a, b)

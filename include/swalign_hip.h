@@ -136,7 +136,7 @@ int swh_forward(swh_ctx* ctx, const uint8_t* A, int64_t m, const uint8_t* B, int
                 const swh_scoring* sc, int64_t K, int64_t Kc, int32_t* score, int64_t* end_row,
                 int64_t* end_col, int32_t* rowmax, int32_t* srow, int32_t* scol);
 
-/* ---- batched pairs (many pairs of uneven length in one persistent launch) -------- */
+/* ---- batched pairs ------------------------------------------------------------------ */
 typedef struct {
     const uint8_t* seq1;
     int64_t len1;
@@ -144,11 +144,21 @@ typedef struct {
     int64_t len2;
 } swh_pair;
 
-/* Aligns npairs pairs; out[npairs] receives one result each (free each with
- * swh_result_free).  Per-pair errors are reported in statuses[npairs] (nullable)
- * without failing the batch. */
+/* Aligns npairs pairs of uneven length together: every sweep (stage 1, end search, each
+ * traceback round) covers all pairs side by side, one kernel launch per tile
+ * anti-diagonal, so short pairs fill the device while long ones proceed.  out[npairs]
+ * receives one result each (free each with swh_result_free); per-pair errors are
+ * reported in statuses[npairs] (nullable) without failing the batch.  Checkpoints are
+ * spaced alike for the whole batch (from 1024 rows, doubled to fit the budget); stage
+ * times in the results are those of the whole batch.  No reference counterpart (the
+ * reference aligns one pair per process, src/main.cu:102-250). */
 int swh_batch_align(swh_ctx* ctx, const swh_pair* pairs, int64_t npairs, const swh_scoring* sc,
                     const swh_opts* opts, swh_result* out, int32_t* statuses);
+
+/* Same, for pairs whose sequences are already in device memory (seq1/seq2 are device
+ * pointers on the context's device). */
+int swh_batch_align_device(swh_ctx* ctx, const swh_pair* pairs, int64_t npairs, const swh_scoring* sc,
+                           const swh_opts* opts, swh_result* out, int32_t* statuses);
 
 /* ---- synthetic workloads ------------------------------------------------------------ */
 /* Generates the synthetic pair used by the benchmarks (seeded std::mt19937_64):

@@ -37,7 +37,10 @@ struct swh_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipDeviceProp_t props{};
-    DevBuf seqA, seqB, rbuf, cb, cor, rowmax, srow, scol, dirs, regions, key, found, walk, ops, xps, wregs;
+    DevBuf seqA, seqB, rbuf, cb, cor, rowmax, srow, scol, dirs, regions, key, found, walk, ops, xps, wregs,
+        argjobs, wargs, seqs, spans;
+    uint8_t* staging = nullptr;  // pinned host staging of batch inputs
+    size_t staging_cap = 0;
     hipEvent_t ev[8] = {};
     size_t lds_pad[3] = {0, 0, 0};  // dynamic LDS per sweep workgroup for w workgroups per CU
     int wps_fixed = 0;              // sweep waves per SIMD forced by the environment, 0 = model
@@ -134,49 +137,81 @@ int choose_wps(swh_ctx* c, int64_t rows, int64_t cols) {
 int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions, int64_t W, int64_t G_fix = 0,
               hipEvent_t ev_before = nullptr, hipEvent_t ev_after = nullptr, int wps = 1,
               int64_t* launches = nullptr) {
-    int64_t S = 0, G = 0, cb_elems = 0, rb_elems = 0, cor_elems = 0;
-    for (const Region& r : regions) {
+    const int64_t nr = (int64_t)regions.size();
+    std::vector<int64_t> Sr((size_t)nr), Gr((size_t)nr);
+    int64_t G = 0, D = 0, cb_elems = 0, rb_elems = 0, cor_elems = 0;
+    for (int64_t q = 0; q < nr; ++q) {
+        const Region& r = regions[(size_t)q];
         const int64_t rows = r.r1 - r.r0 + 1, cols = r.c1 - r.c0 + 1;
         if (rows <= 0 || cols <= 0) return SWH_ERR_INTERNAL;
-        S = std::max(S, ceil_div(rows, kStripeRows));
-        G = std::max(G, ceil_div(cols, W));
+        Sr[(size_t)q] = ceil_div(rows, kStripeRows);
+        Gr[(size_t)q] = ceil_div(cols, W);
+        G = std::max(G, Gr[(size_t)q]);
+        D = std::max(D, Sr[(size_t)q] + Gr[(size_t)q] - 1);
         cb_elems += 2 * rows;
         rb_elems += 2 * cols;
+        cor_elems += 2 * Sr[(size_t)q];
     }
     if (G_fix > 0) {
         if (G > G_fix) return SWH_ERR_INTERNAL;
         G = G_fix;
     }
-    cor_elems = 2 * S * (int64_t)regions.size();
-    HCK(c, ensure(c->cb, sizeof(int2) * cb_elems));
-    HCK(c, ensure(c->cor, sizeof(int32_t) * cor_elems));
-    HCK(c, ensure(c->rbuf, sizeof(int2) * rb_elems));
-    int64_t cbo = 0, rbo = 0;
-    for (size_t q = 0; q < regions.size(); ++q) {
-        regions[q].cb = (int2*)c->cb.p + cbo;
-        regions[q].rb = (int2*)c->rbuf.p + rbo;
-        regions[q].cor = (int32_t*)c->cor.p + 2 * S * (int64_t)q;
-        cbo += 2 * (regions[q].r1 - regions[q].r0 + 1);
-        rbo += 2 * (regions[q].c1 - regions[q].c0 + 1);
+    HCK(c, ensure(c->cb, sizeof(int2) * std::max<int64_t>(cb_elems, 1)));
+    HCK(c, ensure(c->cor, sizeof(int32_t) * std::max<int64_t>(cor_elems, 1)));
+    HCK(c, ensure(c->rbuf, sizeof(int2) * std::max<int64_t>(rb_elems, 1)));
+    int64_t cbo = 0, rbo = 0, coro = 0;
+    for (int64_t q = 0; q < nr; ++q) {
+        Region& r = regions[(size_t)q];
+        r.cb = (int2*)c->cb.p + cbo;
+        r.rb = (int2*)c->rbuf.p + rbo;
+        r.cor = (int32_t*)c->cor.p + coro;
+        cbo += 2 * (r.r1 - r.r0 + 1);
+        rbo += 2 * (r.c1 - r.c0 + 1);
+        coro += 2 * Sr[(size_t)q];
     }
-    HCK(c, ensure(c->regions, sizeof(Region) * regions.size()));
-    HCK(c, hipMemcpyAsync(c->regions.p, regions.data(), sizeof(Region) * regions.size(), hipMemcpyHostToDevice,
+    // Spans of every launch: for diagonal d, each region with tiles on it contributes its
+    // stripes [max(0, d - G_r + 1), min(S_r - 1, d)].  Regions are visited longest-lived
+    // first so that each diagonal stops at the first region that has ended.
+    std::vector<int64_t> order((size_t)nr);
+    for (int64_t q = 0; q < nr; ++q) order[(size_t)q] = q;
+    std::sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
+        return Sr[(size_t)x] + Gr[(size_t)x] > Sr[(size_t)y] + Gr[(size_t)y];
+    });
+    std::vector<TileSpan> spans;
+    std::vector<int64_t> span_at((size_t)D + 1, 0), tiles_at((size_t)D, 0);
+    for (int64_t d = 0; d < D; ++d) {
+        span_at[(size_t)d] = (int64_t)spans.size();
+        int64_t t = 0;
+        for (int64_t q : order) {
+            const int64_t Sq = Sr[(size_t)q], Gq = Gr[(size_t)q];
+            if (d > Sq + Gq - 2) break;
+            const int64_t s_lo = std::max<int64_t>(0, d - (Gq - 1)), s_hi = std::min(Sq - 1, d);
+            spans.push_back(TileSpan{t, (int32_t)q, (int32_t)s_lo});
+            t += s_hi - s_lo + 1;
+        }
+        tiles_at[(size_t)d] = t;
+    }
+    span_at[(size_t)D] = (int64_t)spans.size();
+    HCK(c, ensure(c->regions, sizeof(Region) * std::max<int64_t>(nr, 1)));
+    HCK(c, hipMemcpyAsync(c->regions.p, regions.data(), sizeof(Region) * nr, hipMemcpyHostToDevice, c->stream));
+    HCK(c, ensure(c->spans, sizeof(TileSpan) * std::max<size_t>(spans.size(), 1)));
+    HCK(c, hipMemcpyAsync(c->spans.p, spans.data(), sizeof(TileSpan) * spans.size(), hipMemcpyHostToDevice,
                           c->stream));
     p.regions = (const Region*)c->regions.p;
-    p.nregions = (int32_t)regions.size();
-    p.S = S;
     p.G = G;
     p.W = W;
     if (ev_before) HCK(c, hipEventRecord(ev_before, c->stream));
-    for (int64_t d = 0; d < S + G - 1; ++d) {
-        const int64_t s_lo = std::max<int64_t>(0, d - (G - 1)), s_hi = std::min(S - 1, d);
+    for (int64_t d = 0; d < D; ++d) {
         p.d = d;
-        p.s_lo = s_lo;
-        p.per = (int32_t)(s_hi - s_lo + 1);
+        p.spans = (const TileSpan*)c->spans.p + span_at[(size_t)d];
+        p.nspans = (int32_t)(span_at[(size_t)d + 1] - span_at[(size_t)d]);
+        p.ntiles = tiles_at[(size_t)d];
         HCK(c, launch_sweep(mode, p, c->lds_pad[wps == 2 ? 2 : 1], c->stream));
     }
     if (ev_after) HCK(c, hipEventRecord(ev_after, c->stream));
-    if (launches) *launches = S + G - 1;
+    if (launches) *launches = D;
+    // the host vectors above fed asynchronous copies: keep them alive until those finish
+    HCK(c, hipStreamSynchronize(c->stream));
     return SWH_OK;
 }
 
@@ -184,28 +219,30 @@ struct Plan {
     int64_t K, Kc, nsr, nsc;
 };
 
-int choose_plan(swh_ctx* c, int64_t m, int64_t n, const swh_opts* o, Plan* pl) {
+int64_t plan_mem(int64_t m, int64_t n, int64_t K, int64_t Kc) {
+    return ((m / K) * n + (n / Kc) * m) * (int64_t)sizeof(int2);
+}
+
+// Checkpoint spacing of one pair: K rows (multiple of 256) and Kc columns (power of two),
+// from the options or doubled from 8192 until the checkpoints fit `budget` bytes.
+int choose_plan(int64_t m, int64_t n, const swh_opts* o, int64_t budget, Plan* pl) {
     int64_t K = o && o->checkpoint_rows > 0 ? ceil_div(o->checkpoint_rows, kStripeRows) * kStripeRows : 0;
     int64_t Kc = o && o->checkpoint_cols > 0 ? next_pow2(o->checkpoint_cols) : 0;
-    size_t freeb = 0, totalb = 0;
-    (void)hipMemGetInfo(&freeb, &totalb);
-    int64_t budget = o && o->hbm_budget_bytes > 0 ? o->hbm_budget_bytes : (int64_t)(freeb / 2);
-    auto mem = [&](int64_t k, int64_t kc) { return ((m / k) * n + (n / kc) * m) * (int64_t)sizeof(int2); };
     if (o && o->score_only) {
         K = INT64_C(1) << 62;
         Kc = INT64_C(1) << 62;
     } else if (K == 0) {
         K = 8192;
         if (Kc == 0) Kc = next_pow2(K);
-        while (mem(K, Kc) > budget && K < (INT64_C(1) << 40)) {
+        while (plan_mem(m, n, K, Kc) > budget && K < (INT64_C(1) << 40)) {
             K *= 2;
             Kc *= 2;
         }
     } else if (Kc == 0) {
         Kc = next_pow2(K);
-        while (mem(K, Kc) > budget && Kc < (INT64_C(1) << 40)) Kc *= 2;
+        while (plan_mem(m, n, K, Kc) > budget && Kc < (INT64_C(1) << 40)) Kc *= 2;
     }
-    if (!(o && o->score_only) && mem(K, Kc) > budget) return SWH_ERR_BUDGET;
+    if (!(o && o->score_only) && plan_mem(m, n, K, Kc) > budget) return SWH_ERR_BUDGET;
     pl->K = K;
     pl->Kc = Kc;
     pl->nsr = m / K;
@@ -213,250 +250,424 @@ int choose_plan(swh_ctx* c, int64_t m, int64_t n, const swh_opts* o, Plan* pl) {
     return SWH_OK;
 }
 
-// Core: A (m rows) and B (n columns) resident on the device.
-int align_core(swh_ctx* c, const uint8_t* dA, int64_t m, const uint8_t* dB, int64_t n, const swh_scoring* sc,
-               const swh_opts* o, swh_result* out, bool traceback = true) {
-    const bool want_tb = traceback && !(o && o->score_only);
-    const bool want_xp = o && o->want_xpoints;
-    out->m = m;
-    out->n = n;
-    out->score = 0;
-    out->end_row = out->end_col = out->start_row = out->start_col = -1;
-    out->precision_used = 32;
-    if (m == 0 || n == 0) return SWH_OK;
-    Plan pl;
-    int rc = choose_plan(c, m, n, o, &pl);
-    if (rc) return rc;
-    out->checkpoint_rows = pl.nsr ? pl.K : 0;
-    out->checkpoint_cols = pl.nsc ? pl.Kc : 0;
+int64_t default_budget(const swh_opts* o) {
+    if (o && o->hbm_budget_bytes > 0) return o->hbm_budget_bytes;
+    size_t freeb = 0, totalb = 0;
+    (void)hipMemGetInfo(&freeb, &totalb);
+    return (int64_t)(freeb / 2);
+}
 
-    const int64_t want_w = o ? o->tile_cols : 0;
-    HCK(c, ensure(c->rowmax, sizeof(int32_t) * m));
-    HCK(c, ensure(c->srow, sizeof(int2) * std::max<int64_t>(pl.nsr * n, 1)));
-    HCK(c, ensure(c->scol, sizeof(int2) * std::max<int64_t>(pl.nsc * m, 1)));
-    HCK(c, ensure(c->key, 64));
-    HCK(c, ensure(c->found, 64));
+// One pair of an alignment batch (a single alignment is a batch of one).
+struct Job {
+    const uint8_t* A = nullptr;  // device, oriented: A = rows (the longer), B = columns
+    const uint8_t* B = nullptr;
+    int64_t m = 0, n = 0;
+    bool swapped = false;
+    bool traceback = true;
+    Plan pl{};
+    int64_t rowmax_off = 0, srow_off = 0, scol_off = 0, ops_off = 0, xps_off = 0, xcap = 0;
+    int32_t best = 0;
+    int64_t bi = -1, bj = -1;
+    WalkState st{};
+    int status = SWH_OK;
+};
+
+struct EngineOpts {
+    int64_t tile_cols = 0;   // 0 = auto
+    int lookahead = 3;       // traceback blocks ahead per direction (0 = none)
+    bool want_xp = false;
+    bool batch = false;      // many pairs: fixed tile width, two waves per SIMD
+};
+
+// Columns per tile of a batch sweep: wide enough that the 63 fill steps of a tile cost
+// about a tenth, narrow enough that short pairs still cut into several tiles.
+constexpr int64_t kBatchTileCols = 512;
+
+// Runs stage 1, the end search and (for jobs that want it) the traceback of every job,
+// all jobs side by side in each sweep.  Fills out[k] (result_init'ed by the caller) and
+// jobs[k].status.  Stage times are those of the whole batch.
+int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const EngineOpts& eo, swh_result* out) {
+    const int64_t nj = (int64_t)jobs.size();
+    int64_t rm_elems = 0, sr_elems = 0, scl_elems = 0, ops_elems = 0, xps_elems = 0;
+    int64_t max_m = 0, max_n = 0;
+    for (Job& j : jobs) {
+        j.rowmax_off = rm_elems;
+        j.srow_off = sr_elems;
+        j.scol_off = scl_elems;
+        j.ops_off = ops_elems;
+        j.xps_off = xps_elems;
+        j.xcap = j.pl.nsr + 2;
+        rm_elems += j.m;
+        sr_elems += j.pl.nsr * j.n;
+        scl_elems += j.pl.nsc * j.m;
+        ops_elems += j.m + j.n + 16;
+        xps_elems += j.xcap;
+        max_m = std::max(max_m, j.m);
+        max_n = std::max(max_n, j.n);
+    }
+    HCK(c, ensure(c->rowmax, sizeof(int32_t) * std::max<int64_t>(rm_elems, 1)));
+    HCK(c, ensure(c->srow, sizeof(int2) * std::max<int64_t>(sr_elems, 1)));
+    HCK(c, ensure(c->scol, sizeof(int2) * std::max<int64_t>(scl_elems, 1)));
+    HCK(c, ensure(c->key, sizeof(unsigned long long) * std::max<int64_t>(nj, 1)));
+    HCK(c, ensure(c->found, sizeof(int32_t) * std::max<int64_t>(nj, 1)));
+    int32_t* rowmax = (int32_t*)c->rowmax.p;
     int2* srow = (int2*)c->srow.p;
     int2* scol = (int2*)c->scol.p;
-
-    Score S{sc->match, sc->mismatch, sc->gap_open, sc->gap_extend};
+    const Score S{sc->match, sc->mismatch, sc->gap_open, sc->gap_extend};
     SweepParams p{};
-    p.A = dA;
-    p.m = m;
-    p.B = dB;
-    p.n = n;
     p.sc = S;
-    p.rowmax = (int32_t*)c->rowmax.p;
-    p.srow = pl.nsr ? srow : nullptr;
-    p.K = pl.K;
-    p.scol = pl.nsc ? scol : nullptr;
-    p.Kc = pl.Kc;
+    for (int64_t k = 0; k < nj; ++k) {
+        out[k].m = jobs[k].m;
+        out[k].n = jobs[k].n;
+        out[k].swapped = jobs[k].swapped ? 1 : 0;
+        out[k].score = 0;
+        out[k].end_row = out[k].end_col = out[k].start_row = out[k].start_col = -1;
+        out[k].precision_used = 32;
+        out[k].checkpoint_rows = jobs[k].pl.nsr ? jobs[k].pl.K : 0;
+        out[k].checkpoint_cols = jobs[k].pl.nsc ? jobs[k].pl.Kc : 0;
+    }
 
-    // ---- stage 1 ------------------------------------------------------------------
-    {
+    // ---- stage 1: all pairs, one region each --------------------------------------
+    std::vector<Region> regions;
+    std::vector<int64_t> live;  // jobs with cells
+    for (int64_t k = 0; k < nj; ++k) {
+        const Job& j = jobs[k];
+        if (j.m == 0 || j.n == 0) continue;
         Region r{};
+        r.A = j.A;
+        r.B = j.B;
+        r.m = j.m;
+        r.n = j.n;
         r.r0 = 0;
-        r.r1 = m - 1;
+        r.r1 = j.m - 1;
         r.c0 = 0;
-        r.c1 = n - 1;
-        std::vector<Region> regions{r};
-        const int wps = want_w > 0 ? 1 : choose_wps(c, m, n);
-        int64_t launches = 0;
-        rc = run_sweep(c, kStage1, p, regions, tile_cols(c, n, want_w, wps), 0, c->ev[0], c->ev[1], wps,
-                       &launches);
-        out->forward_launches = (int32_t)launches;
-        if (rc) return rc;
+        r.c1 = j.n - 1;
+        r.rowmax = rowmax + j.rowmax_off;
+        r.srow = j.pl.nsr ? srow + j.srow_off : nullptr;
+        r.K = j.pl.K;
+        r.scol = j.pl.nsc ? scol + j.scol_off : nullptr;
+        r.Kc = j.pl.Kc;
+        regions.push_back(r);
+        live.push_back(k);
     }
-    HCK(c, hipMemsetAsync(c->key.p, 0, 8, c->stream));
-    HCK(c, launch_argmax_rows((const int32_t*)c->rowmax.p, m, (unsigned long long*)c->key.p, c->stream));
-    unsigned long long key = 0;
-    HCK(c, hipMemcpyAsync(&key, c->key.p, 8, hipMemcpyDeviceToHost, c->stream));
-    HCK(c, hipStreamSynchronize(c->stream));
-    out->ms_forward = elapsed(c->ev[0], c->ev[1]);
-    const int32_t best = (int32_t)(key >> 32);
-    const int64_t bi = (int64_t)(0xFFFFFFFFu - (uint32_t)(key & 0xFFFFFFFFu));
-    out->score = best;
-    if (best <= 0) {
-        out->ms_total = out->ms_forward;
-        return SWH_OK;
+    if (live.empty()) return SWH_OK;
+    int wps = 1;
+    int64_t W = 0;
+    if (eo.batch) {
+        wps = c->wps_fixed ? c->wps_fixed : (c->lds_pad[2] ? 2 : 1);
+        W = eo.tile_cols > 0 ? eo.tile_cols : kBatchTileCols;
+    } else {
+        wps = eo.tile_cols > 0 ? 1 : choose_wps(c, max_m, max_n);
+        W = tile_cols(c, max_n, eo.tile_cols, wps);
     }
+    int64_t launches = 0;
+    int rc = run_sweep(c, kStage1, p, regions, W, 0, c->ev[0], c->ev[1], wps, &launches);
+    if (rc) return rc;
 
-    // ---- end-cell column search (the argmax above is counted here) ---------------------
-    const hipEvent_t ev_search0 = c->ev[1];
-    const int64_t br0 = (bi / pl.K) * pl.K;
-    const int64_t nq = ceil_div(n, pl.Kc);
+    // ---- argmax of the row maxima, per pair ---------------------------------------
+    std::vector<ArgmaxJob> aj;  // source of an asynchronous copy: lives until the sync below
     {
-        // one region per checkpoint-column block: blocks start from their own checkpoints
-        std::vector<Region> regions;
-        regions.reserve(nq);
-        for (int64_t q = 0; q < nq; ++q) {
-            Region r{};
-            r.r0 = br0;
-            r.r1 = bi;
-            r.c0 = q * pl.Kc;
-            r.c1 = std::min(n, r.c0 + pl.Kc) - 1;
-            r.top = br0 > 0 ? srow + (br0 / pl.K - 1) * n + r.c0 : nullptr;
-            r.left = r.c0 > 0 ? scol + (r.c0 / pl.Kc - 1) * m : nullptr;
-            r.found = (int32_t*)c->found.p;
-            regions.push_back(r);
-        }
-        const int32_t big = INT32_MAX;
-        HCK(c, hipMemcpyAsync(c->found.p, &big, 4, hipMemcpyHostToDevice, c->stream));
-        SweepParams ps = p;
-        ps.target_row = bi;
-        ps.target = best;
-        rc = run_sweep(c, kSearch, ps, regions, tile_cols(c, std::min(n, pl.Kc), want_w));
-        if (rc) return rc;
-        int32_t bj = -1;
-        HCK(c, hipMemcpyAsync(&bj, c->found.p, 4, hipMemcpyDeviceToHost, c->stream));
-        HCK(c, hipEventRecord(c->ev[3], c->stream));
-        HCK(c, hipStreamSynchronize(c->stream));
-        if (bj == INT32_MAX || bj < 0) return SWH_ERR_INTERNAL;
-        out->end_row = bi;
-        out->end_col = bj;
+        for (int64_t k : live) aj.push_back(ArgmaxJob{rowmax + jobs[k].rowmax_off, jobs[k].m, (unsigned long long*)c->key.p + k});
+        HCK(c, ensure(c->argjobs, sizeof(ArgmaxJob) * aj.size()));
+        HCK(c, hipMemcpyAsync(c->argjobs.p, aj.data(), sizeof(ArgmaxJob) * aj.size(), hipMemcpyHostToDevice, c->stream));
+        HCK(c, hipMemsetAsync(c->key.p, 0, sizeof(unsigned long long) * nj, c->stream));
+        const int bpj = (int)std::min<int64_t>(1024, std::max<int64_t>(1, ceil_div(max_m, 16384)));
+        HCK(c, launch_argmax_rows((const ArgmaxJob*)c->argjobs.p, (int)aj.size(), bpj, c->stream));
     }
-    out->ms_end_search = elapsed(ev_search0, c->ev[3]);
-    if (!want_tb) {
-        out->ms_total = out->ms_forward + out->ms_end_search;
-        return SWH_OK;
+    std::vector<unsigned long long> keys((size_t)nj, 0);
+    HCK(c, hipMemcpyAsync(keys.data(), c->key.p, sizeof(unsigned long long) * nj, hipMemcpyDeviceToHost, c->stream));
+    HCK(c, hipStreamSynchronize(c->stream));
+    const float ms_fwd = elapsed(c->ev[0], c->ev[1]);
+    std::vector<int64_t> scored;
+    for (int64_t k : live) {
+        Job& j = jobs[k];
+        j.best = (int32_t)(keys[k] >> 32);
+        j.bi = keys[k] ? (int64_t)(0xFFFFFFFFu - (uint32_t)(keys[k] & 0xFFFFFFFFu)) : -1;
+        out[k].score = j.best > 0 ? j.best : 0;
+        out[k].forward_launches = (int32_t)launches;
+        out[k].ms_forward = ms_fwd;
+        out[k].ms_total = ms_fwd;
+        if (j.best > 0) scored.push_back(k);
     }
+    if (scored.empty()) return SWH_OK;
 
-    // ---- traceback ----------------------------------------------------------------
-    HCK(c, hipEventRecord(c->ev[4], c->stream));
-    const int64_t xcap = pl.nsr + 2;
-    HCK(c, ensure(c->ops, (size_t)(m + n + 16)));
-    HCK(c, ensure(c->xps, sizeof(XPoint) * xcap));
-    HCK(c, ensure(c->walk, sizeof(WalkState)));
-    WalkState st{};
-    st.i = out->end_row;
-    st.j = out->end_col;
-    st.start_i = st.i;
-    st.start_j = st.j;
-    st.last_row = st.i;
-    st.layer = 0;
-    st.done = 0;
-    HCK(c, hipMemcpyAsync(c->walk.p, &st, sizeof st, hipMemcpyHostToDevice, c->stream));
-    // Rounds: the checkpoint block holding the current cell (cut at that cell) and the
-    // full blocks up to L block-rows above and L block-columns to the left are
-    // recomputed in one sweep (independent regions, each from its own checkpoints), then
-    // the walk follows the path through them until it leaves them.
-    const int64_t Wtb = tile_cols(c, std::min(n, pl.Kc), want_w);
-    const int64_t tstride = ceil_div(Wtb + 63, 64) * 64;
-    const int64_t Gblk = ceil_div(std::min(n, pl.Kc), Wtb);
-    const int64_t blk_dirs = ceil_div(std::min(m, pl.K), kStripeRows) * Gblk * tstride * 64 * (int64_t)sizeof(uint16_t);
-    int64_t L = o && o->tb_lookahead != 0 ? std::max(0, o->tb_lookahead) : 3;
-    while (L > 0 && (L + 1) * (L + 1) * blk_dirs > kTracebackDirsCap) --L;
-    std::vector<WalkRegion> wregs((size_t)((L + 1) * (L + 1)));
-    HCK(c, ensure(c->wregs, sizeof(WalkRegion) * wregs.size()));
-    int64_t guard = 0;
-    while (!st.done) {
-        if (++guard > m / std::max<int64_t>(pl.K, 1) + n / std::max<int64_t>(pl.Kc, 1) + 8) return SWH_ERR_INTERNAL;
-        const int64_t brow = st.i / pl.K, bcol = st.j / pl.Kc;
-        std::vector<Region> regions;
-        std::vector<int64_t> slot;
-        int64_t dirs_elems = 0;
-        for (int64_t da = 0; da <= L; ++da) {
-            for (int64_t db = 0; db <= L; ++db) {
-                WalkRegion& w = wregs[(size_t)(da * (L + 1) + db)];
-                w = WalkRegion{};
-                if (brow - da < 0 || bcol - db < 0) continue;
+    // ---- end-cell search: per pair, the band above the end row, one region per
+    // checkpoint-column block -------------------------------------------------------
+    {
+        regions.clear();
+        int64_t wmax = 0;
+        for (int64_t k : scored) {
+            const Job& j = jobs[k];
+            const int64_t br0 = (j.bi / j.pl.K) * j.pl.K;
+            const int64_t nq = ceil_div(j.n, std::min(j.pl.Kc, j.n));
+            const int64_t bw = std::min(j.pl.Kc, j.n);
+            wmax = std::max(wmax, bw);
+            for (int64_t q = 0; q < nq; ++q) {
                 Region r{};
-                r.r0 = (brow - da) * pl.K;
-                r.r1 = da == 0 ? st.i : std::min(m, r.r0 + pl.K) - 1;
-                r.c0 = (bcol - db) * pl.Kc;
-                r.c1 = db == 0 ? st.j : std::min(n, r.c0 + pl.Kc) - 1;
-                r.top = r.r0 > 0 ? srow + (r.r0 / pl.K - 1) * n + r.c0 : nullptr;
-                r.left = r.c0 > 0 ? scol + (r.c0 / pl.Kc - 1) * m : nullptr;
-                w.r0 = r.r0;
-                w.c0 = r.c0;
-                w.rows = r.r1 - r.r0 + 1;
-                w.cols = r.c1 - r.c0 + 1;
-                slot.push_back(dirs_elems);
-                dirs_elems += ceil_div(w.rows, kStripeRows) * Gblk * tstride * 64;
+                r.A = j.A;
+                r.B = j.B;
+                r.m = j.m;
+                r.n = j.n;
+                r.r0 = br0;
+                r.r1 = j.bi;
+                r.c0 = q * bw;
+                r.c1 = std::min(j.n, r.c0 + bw) - 1;
+                r.top = br0 > 0 ? srow + j.srow_off + (br0 / j.pl.K - 1) * j.n + r.c0 : nullptr;
+                r.left = r.c0 > 0 ? scol + j.scol_off + (r.c0 / j.pl.Kc - 1) * j.m : nullptr;
+                r.K = j.pl.K;
+                r.Kc = j.pl.Kc;
+                r.target_row = j.bi;
+                r.target = j.best;
+                r.found = (int32_t*)c->found.p + k;
                 regions.push_back(r);
             }
         }
-        HCK(c, ensure(c->dirs, sizeof(uint16_t) * (size_t)dirs_elems));
-        for (size_t k = 0, q = 0; k < wregs.size(); ++k) {
-            if (wregs[k].rows <= 0) continue;
-            uint16_t* d = (uint16_t*)c->dirs.p + slot[q];
+        std::vector<int32_t> big((size_t)nj, INT32_MAX);  // copied before run_sweep's sync
+        HCK(c, hipMemcpyAsync(c->found.p, big.data(), sizeof(int32_t) * nj, hipMemcpyHostToDevice, c->stream));
+        const int64_t Ws = eo.batch ? W : tile_cols(c, wmax, eo.tile_cols);
+        rc = run_sweep(c, kSearch, p, regions, Ws);
+        if (rc) return rc;
+        std::vector<int32_t> found((size_t)nj, -1);
+        HCK(c, hipMemcpyAsync(found.data(), c->found.p, sizeof(int32_t) * nj, hipMemcpyDeviceToHost, c->stream));
+        HCK(c, hipEventRecord(c->ev[3], c->stream));
+        HCK(c, hipStreamSynchronize(c->stream));
+        const float ms_search = elapsed(c->ev[1], c->ev[3]);
+        for (int64_t k : scored) {
+            Job& j = jobs[k];
+            if (found[k] == INT32_MAX || found[k] < 0) {
+                j.status = SWH_ERR_INTERNAL;
+                continue;
+            }
+            j.bj = found[k];
+            out[k].end_row = j.bi;
+            out[k].end_col = j.bj;
+            out[k].ms_end_search = ms_search;
+            out[k].ms_total = ms_fwd + ms_search;
+        }
+    }
+
+    // ---- traceback rounds ----------------------------------------------------------
+    // Each round recomputes, for every walking pair, the checkpoint block of its current
+    // cell (cut at the cell) and up to L blocks above and to the left, all in one sweep
+    // (independent regions, each from its own checkpoints); then one walk per pair
+    // follows the path through its blocks until it leaves them.
+    std::vector<int64_t> active;
+    for (int64_t k : scored)
+        if (jobs[k].traceback && jobs[k].status == SWH_OK) active.push_back(k);
+    if (active.empty()) return SWH_OK;
+    HCK(c, hipEventRecord(c->ev[4], c->stream));
+    HCK(c, ensure(c->ops, (size_t)ops_elems));
+    HCK(c, ensure(c->xps, sizeof(XPoint) * (size_t)xps_elems));
+    HCK(c, ensure(c->walk, sizeof(WalkState) * (size_t)nj));
+    std::vector<WalkState> states((size_t)nj);
+    for (int64_t k : active) {
+        WalkState& st = states[(size_t)k];
+        st = WalkState{};
+        st.i = jobs[k].bi;
+        st.j = jobs[k].bj;
+        st.start_i = st.i;
+        st.start_j = st.j;
+        st.last_row = st.i;
+    }
+    HCK(c, hipMemcpyAsync(c->walk.p, states.data(), sizeof(WalkState) * nj, hipMemcpyHostToDevice, c->stream));
+    int64_t wmax = 0;
+    for (int64_t k : active) wmax = std::max(wmax, std::min(jobs[k].n, jobs[k].pl.Kc));
+    const int64_t Wtb = eo.batch ? W : tile_cols(c, wmax, eo.tile_cols);
+    const int64_t tstride = ceil_div(Wtb + 63, 64) * 64;
+    const int64_t Gblk = ceil_div(wmax, Wtb);
+    auto blk_dirs = [&](const Job& j) {
+        return ceil_div(std::min(j.m, j.pl.K), kStripeRows) * Gblk * tstride * 64 * (int64_t)sizeof(uint16_t);
+    };
+    int64_t guard = 0;
+    int64_t max_blocks = 0;
+    for (int64_t k : active)
+        max_blocks = std::max(max_blocks, jobs[k].m / jobs[k].pl.K + jobs[k].n / jobs[k].pl.Kc + 8);
+    while (!active.empty()) {
+        if (++guard > 4 * (max_blocks + (int64_t)active.size())) return SWH_ERR_INTERNAL;
+        // lookahead and the pairs walked this round, within the direction-code budget
+        int64_t L = std::max(0, eo.lookahead);
+        int64_t need = 0;
+        for (int64_t k : active) need += blk_dirs(jobs[k]);
+        while (L > 0 && (L + 1) * (L + 1) * need > kTracebackDirsCap) --L;
+        std::vector<int64_t> round;
+        int64_t used = 0;
+        for (int64_t k : active) {
+            const int64_t b = (L + 1) * (L + 1) * blk_dirs(jobs[k]);
+            if (!round.empty() && used + b > kTracebackDirsCap) break;
+            round.push_back(k);
+            used += b;
+        }
+        const int64_t nr = (int64_t)round.size();
+        std::vector<WalkRegion> wregs((size_t)(nr * (L + 1) * (L + 1)));
+        std::vector<WalkArgs> wargs((size_t)nr);
+        regions.clear();
+        std::vector<std::pair<int64_t, int64_t>> slots;  // (wregs index, dirs offset)
+        int64_t dirs_elems = 0;
+        for (int64_t x = 0; x < nr; ++x) {
+            const Job& j = jobs[round[x]];
+            const WalkState& st = states[(size_t)round[x]];
+            const int64_t brow = st.i / j.pl.K, bcol = st.j / j.pl.Kc;
+            for (int64_t da = 0; da <= L; ++da) {
+                for (int64_t db = 0; db <= L; ++db) {
+                    const int64_t wi = x * (L + 1) * (L + 1) + da * (L + 1) + db;
+                    WalkRegion& w = wregs[(size_t)wi];
+                    w = WalkRegion{};
+                    if (brow - da < 0 || bcol - db < 0) continue;
+                    Region r{};
+                    r.A = j.A;
+                    r.B = j.B;
+                    r.m = j.m;
+                    r.n = j.n;
+                    r.r0 = (brow - da) * j.pl.K;
+                    r.r1 = da == 0 ? st.i : std::min(j.m, r.r0 + j.pl.K) - 1;
+                    r.c0 = (bcol - db) * j.pl.Kc;
+                    r.c1 = db == 0 ? st.j : std::min(j.n, r.c0 + j.pl.Kc) - 1;
+                    r.top = r.r0 > 0 ? srow + j.srow_off + (r.r0 / j.pl.K - 1) * j.n + r.c0 : nullptr;
+                    r.left = r.c0 > 0 ? scol + j.scol_off + (r.c0 / j.pl.Kc - 1) * j.m : nullptr;
+                    r.K = j.pl.K;
+                    r.Kc = j.pl.Kc;
+                    w.r0 = r.r0;
+                    w.c0 = r.c0;
+                    w.rows = r.r1 - r.r0 + 1;
+                    w.cols = r.c1 - r.c0 + 1;
+                    slots.emplace_back(wi, dirs_elems);
+                    dirs_elems += ceil_div(w.rows, kStripeRows) * Gblk * tstride * 64;
+                    regions.push_back(r);
+                }
+            }
+        }
+        HCK(c, ensure(c->dirs, sizeof(uint16_t) * (size_t)std::max<int64_t>(dirs_elems, 1)));
+        for (size_t q = 0; q < slots.size(); ++q) {
+            uint16_t* d = (uint16_t*)c->dirs.p + slots[q].second;
             regions[q].dirs = d;
-            wregs[k].dirs = d;
-            ++q;
+            wregs[(size_t)slots[q].first].dirs = d;
         }
         SweepParams pd = p;
         pd.tstride = tstride;
         rc = run_sweep(c, kDirs, pd, regions, Wtb, Gblk);
         if (rc) return rc;
+        HCK(c, ensure(c->wregs, sizeof(WalkRegion) * wregs.size()));
+        HCK(c, ensure(c->wargs, sizeof(WalkArgs) * wargs.size()));
+        for (int64_t x = 0; x < nr; ++x) {
+            const Job& j = jobs[round[x]];
+            const WalkState& st = states[(size_t)round[x]];
+            WalkArgs& wa = wargs[(size_t)x];
+            wa = WalkArgs{};
+            wa.A = j.A;
+            wa.B = j.B;
+            wa.regs = (const WalkRegion*)c->wregs.p + x * (L + 1) * (L + 1);
+            wa.L = L;
+            wa.brow = st.i / j.pl.K;
+            wa.bcol = st.j / j.pl.Kc;
+            wa.tstride = tstride;
+            wa.G = Gblk;
+            wa.W = Wtb;
+            wa.K = j.pl.K;
+            wa.Kc = j.pl.Kc;
+            wa.srow = j.pl.nsr ? srow + j.srow_off : nullptr;
+            wa.n = j.n;
+            wa.st = (WalkState*)c->walk.p + round[x];
+            wa.ops = (uint8_t*)c->ops.p + j.ops_off;
+            wa.xps = (XPoint*)c->xps.p + j.xps_off;
+            wa.xcap = j.xcap;
+        }
         HCK(c, hipMemcpyAsync(c->wregs.p, wregs.data(), sizeof(WalkRegion) * wregs.size(), hipMemcpyHostToDevice,
                               c->stream));
-        WalkArgs wa{};
-        wa.A = dA;
-        wa.B = dB;
-        wa.regs = (const WalkRegion*)c->wregs.p;
-        wa.L = L;
-        wa.brow = brow;
-        wa.bcol = bcol;
-        wa.tstride = tstride;
-        wa.G = Gblk;
-        wa.W = Wtb;
-        wa.K = pl.K;
-        wa.Kc = pl.Kc;
-        wa.srow = pl.nsr ? srow : nullptr;
-        wa.n = n;
-        wa.st = (WalkState*)c->walk.p;
-        wa.ops = (uint8_t*)c->ops.p;
-        wa.xps = (XPoint*)c->xps.p;
-        wa.xcap = xcap;
-        HCK(c, launch_walk(wa, c->stream));
-        const int64_t pi = st.i, pj = st.j;
-        HCK(c, hipMemcpyAsync(&st, c->walk.p, sizeof st, hipMemcpyDeviceToHost, c->stream));
+        HCK(c, hipMemcpyAsync(c->wargs.p, wargs.data(), sizeof(WalkArgs) * wargs.size(), hipMemcpyHostToDevice,
+                              c->stream));
+        HCK(c, launch_walk((const WalkArgs*)c->wargs.p, (int)nr, c->stream));
+        std::vector<WalkState> prev(states);
+        HCK(c, hipMemcpyAsync(states.data(), c->walk.p, sizeof(WalkState) * nj, hipMemcpyDeviceToHost, c->stream));
         HCK(c, hipStreamSynchronize(c->stream));
-        if (!st.done && st.i == pi && st.j == pj) return SWH_ERR_INTERNAL;  // no progress
+        std::vector<int64_t> next;
+        for (int64_t k : active) {
+            const WalkState& st = states[(size_t)k];
+            const WalkState& pv = prev[(size_t)k];
+            if (st.done) continue;
+            const bool walked = std::find(round.begin(), round.end(), k) != round.end();
+            if (walked && st.i == pv.i && st.j == pv.j && st.layer == pv.layer) {
+                jobs[k].status = SWH_ERR_INTERNAL;  // no progress
+                continue;
+            }
+            next.push_back(k);
+        }
+        active.swap(next);
     }
     HCK(c, hipEventRecord(c->ev[5], c->stream));
-    std::string ops((size_t)st.nops, '\0');
-    if (st.nops) HCK(c, hipMemcpyAsync(&ops[0], c->ops.p, (size_t)st.nops, hipMemcpyDeviceToHost, c->stream));
-    std::vector<XPoint> xps((size_t)st.nxp);
-    if (want_xp && st.nxp)
-        HCK(c, hipMemcpyAsync(xps.data(), c->xps.p, sizeof(XPoint) * st.nxp, hipMemcpyDeviceToHost, c->stream));
+
+    // ---- results: ops (reversed) -> run-length CIGAR, crosspoints start -> end ---------
+    std::vector<uint8_t> ops_h((size_t)ops_elems);
+    std::vector<XPoint> xps_h((size_t)xps_elems);
+    HCK(c, hipMemcpyAsync(ops_h.data(), c->ops.p, (size_t)ops_elems, hipMemcpyDeviceToHost, c->stream));
+    if (eo.want_xp)
+        HCK(c, hipMemcpyAsync(xps_h.data(), c->xps.p, sizeof(XPoint) * xps_elems, hipMemcpyDeviceToHost, c->stream));
     HCK(c, hipStreamSynchronize(c->stream));
-    out->ms_traceback = elapsed(c->ev[4], c->ev[5]);
-    out->ms_total = out->ms_forward + out->ms_end_search + out->ms_traceback;
-    out->start_row = st.start_i;
-    out->start_col = st.start_j;
-    std::reverse(ops.begin(), ops.end());
-    std::string rle;
-    rle.reserve(ops.size() / 4 + 16);
-    char num[32];
-    for (size_t a = 0; a < ops.size();) {
-        size_t b = a;
-        while (b < ops.size() && ops[b] == ops[a]) ++b;
-        int k = snprintf(num, sizeof num, "%zu%c", b - a, ops[a]);
-        rle.append(num, (size_t)k);
-        a = b;
-    }
-    out->n_ops = (int64_t)ops.size();
-    out->cigar = (char*)malloc(rle.size() + 1);
-    if (!out->cigar) return SWH_ERR_OUT_OF_MEMORY;
-    memcpy(out->cigar, rle.data(), rle.size());
-    out->cigar[rle.size()] = 0;
-    out->cigar_len = (int64_t)rle.size();
-    if (want_xp) {
-        const int64_t nx = std::min<int64_t>(st.nxp, xcap);
-        out->xpoints = (swh_xpoint*)malloc(sizeof(swh_xpoint) * (nx + 1));
-        if (!out->xpoints) return SWH_ERR_OUT_OF_MEMORY;
-        for (int64_t k = 0; k < nx; ++k) {
-            const XPoint& x = xps[(size_t)(nx - 1 - k)];
-            out->xpoints[k] = swh_xpoint{x.row, x.col, x.score, x.layer};
+    const float ms_tb = elapsed(c->ev[4], c->ev[5]);
+    for (int64_t k : scored) {
+        Job& j = jobs[k];
+        swh_result& r = out[k];
+        if (!j.traceback || j.status != SWH_OK) continue;
+        const WalkState& st = states[(size_t)k];
+        r.ms_traceback = ms_tb;
+        r.ms_total = r.ms_forward + r.ms_end_search + ms_tb;
+        r.start_row = st.start_i;
+        r.start_col = st.start_j;
+        const uint8_t* ops = ops_h.data() + j.ops_off;
+        std::string rle;
+        rle.reserve((size_t)st.nops / 4 + 16);
+        char num[32];
+        for (int64_t b = st.nops - 1; b >= 0;) {  // ops were written end -> start
+            int64_t e = b;
+            while (e >= 0 && ops[e] == ops[b]) --e;
+            const int len = snprintf(num, sizeof num, "%lld%c", (long long)(b - e), ops[b]);
+            rle.append(num, (size_t)len);
+            b = e;
         }
-        out->xpoints[nx] = swh_xpoint{out->end_row, out->end_col, best, 0};
-        out->n_xpoints = nx + 1;
+        r.n_ops = st.nops;
+        r.cigar = (char*)malloc(rle.size() + 1);
+        if (!r.cigar) return SWH_ERR_OUT_OF_MEMORY;
+        memcpy(r.cigar, rle.data(), rle.size());
+        r.cigar[rle.size()] = 0;
+        r.cigar_len = (int64_t)rle.size();
+        if (eo.want_xp) {
+            const int64_t nx = std::min<int64_t>(st.nxp, j.xcap);
+            r.xpoints = (swh_xpoint*)malloc(sizeof(swh_xpoint) * (size_t)(nx + 1));
+            if (!r.xpoints) return SWH_ERR_OUT_OF_MEMORY;
+            const XPoint* xs = xps_h.data() + j.xps_off;
+            for (int64_t q = 0; q < nx; ++q) {
+                const XPoint& x = xs[nx - 1 - q];
+                r.xpoints[q] = swh_xpoint{x.row, x.col, x.score, x.layer};
+            }
+            r.xpoints[nx] = swh_xpoint{r.end_row, r.end_col, j.best, 0};
+            r.n_xpoints = nx + 1;
+        }
     }
     return SWH_OK;
 }
 
 void result_init(swh_result* r) { memset(r, 0, sizeof *r); }
+
+EngineOpts engine_opts(const swh_opts* o, bool batch) {
+    EngineOpts eo;
+    eo.tile_cols = o ? o->tile_cols : 0;
+    eo.want_xp = o && o->want_xpoints;
+    eo.batch = batch;
+    eo.lookahead = batch ? 1 : 3;
+    if (o && o->tb_lookahead != 0) eo.lookahead = std::max(0, o->tb_lookahead);
+    return eo;
+}
+
+// A job for device-resident sequences (orientation of src/SWutils.cpp:84-88).
+Job make_job(const uint8_t* d1, int64_t len1, const uint8_t* d2, int64_t len2, bool traceback) {
+    Job j;
+    j.swapped = len1 < len2;
+    j.A = j.swapped ? d2 : d1;
+    j.B = j.swapped ? d1 : d2;
+    j.m = j.swapped ? len2 : len1;
+    j.n = j.swapped ? len1 : len2;
+    j.traceback = traceback;
+    return j;
+}
 
 }  // namespace
 
@@ -518,13 +729,15 @@ int swh_ctx_destroy(swh_ctx* c) {
     if (!c) return SWH_ERR_INVALID_ARG;
     (void)hipSetDevice(c->device);
     DevBuf* bufs[] = {&c->seqA, &c->seqB, &c->rbuf, &c->cb, &c->cor, &c->rowmax, &c->srow, &c->scol,
-                      &c->dirs, &c->regions, &c->key, &c->found, &c->walk, &c->ops, &c->xps, &c->wregs};
+                      &c->dirs, &c->regions, &c->key, &c->found, &c->walk, &c->ops, &c->xps, &c->wregs,
+                      &c->argjobs, &c->wargs, &c->seqs, &c->spans};
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->staging) (void)hipHostFree(c->staging);
     delete c;
     return SWH_OK;
 }
@@ -546,12 +759,11 @@ int swh_align_pair_device(swh_ctx* c, const uint8_t* d1, int64_t len1, const uin
     result_init(out);
     if (!scoring_ok(sc)) return SWH_ERR_SCORING;
     HCK(c, hipSetDevice(c->device));
-    const bool swap = len1 < len2;  // src/SWutils.cpp:84-88
-    out->swapped = swap ? 1 : 0;
-    const uint8_t* A = swap ? d2 : d1;
-    const uint8_t* B = swap ? d1 : d2;
-    const int64_t m = swap ? len2 : len1, n = swap ? len1 : len2;
-    return align_core(c, A, m, B, n, sc, o, out);
+    std::vector<Job> jobs{make_job(d1, len1, d2, len2, !(o && o->score_only))};
+    int rc = choose_plan(jobs[0].m, jobs[0].n, o, default_budget(o), &jobs[0].pl);
+    if (rc) return rc;
+    rc = run_jobs(c, jobs, sc, engine_opts(o, false), out);
+    return rc ? rc : jobs[0].status;
 }
 
 int swh_align_pair(swh_ctx* c, const uint8_t* s1, int64_t len1, const uint8_t* s2, int64_t len2,
@@ -587,15 +799,21 @@ int swh_forward(swh_ctx* c, const uint8_t* A, int64_t m, const uint8_t* B, int64
     HCK(c, ensure(c->seqB, (size_t)n + 64));
     if (m) HCK(c, hipMemcpyAsync(c->seqA.p, A, (size_t)m, hipMemcpyHostToDevice, c->stream));
     if (n) HCK(c, hipMemcpyAsync(c->seqB.p, B, (size_t)n, hipMemcpyHostToDevice, c->stream));
-    swh_opts o{};
-    o.checkpoint_rows = K;
-    o.checkpoint_cols = Kc;
-    o.hbm_budget_bytes = INT64_MAX / 4;
+    // A and B are already oriented: one job without the swap rule
+    Job j;
+    j.A = (const uint8_t*)c->seqA.p;
+    j.B = (const uint8_t*)c->seqB.p;
+    j.m = m;
+    j.n = n;
+    j.traceback = false;
+    j.pl = Plan{K, Kc, m / K, n / Kc};
+    std::vector<Job> jobs{j};
     swh_result r;
     result_init(&r);
-    const int rc = align_core(c, (const uint8_t*)c->seqA.p, m, (const uint8_t*)c->seqB.p, n, sc, &o, &r,
-                              /*traceback=*/false);
+    const int rc = run_jobs(c, jobs, sc, engine_opts(nullptr, false), &r);
+    swh_result_free(&r);
     if (rc) return rc;
+    if (jobs[0].status) return jobs[0].status;
     *score = r.score;
     *end_row = r.end_row;
     *end_col = r.end_col;
@@ -604,19 +822,101 @@ int swh_forward(swh_ctx* c, const uint8_t* A, int64_t m, const uint8_t* B, int64
         HCK(c, hipMemcpy(srow_out, c->srow.p, sizeof(int2) * (m / K) * n, hipMemcpyDeviceToHost));
     if (scol_out && n / Kc > 0)
         HCK(c, hipMemcpy(scol_out, c->scol.p, sizeof(int2) * (n / Kc) * m, hipMemcpyDeviceToHost));
-    swh_result_free(&r);
     return SWH_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Shared by the batch entry points: jobs over device-resident pairs, one checkpoint
+// spacing for the batch (from 1024 rows, or the options, doubled to fit the budget).
+int batch_run(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const swh_opts* o, swh_result* out,
+              int32_t* statuses) {
+    const int64_t budget = default_budget(o);
+    swh_opts ob = o ? *o : swh_opts{};
+    if (!ob.score_only) {
+        int64_t K = ob.checkpoint_rows > 0 ? ceil_div(ob.checkpoint_rows, kStripeRows) * kStripeRows : 1024;
+        int64_t Kc = ob.checkpoint_cols > 0 ? next_pow2(ob.checkpoint_cols) : next_pow2(K);
+        int64_t ckpt = 0;
+        while (true) {
+            ckpt = 0;
+            for (const Job& j : jobs) ckpt += plan_mem(j.m, j.n, K, Kc);
+            if (ckpt <= budget || K >= (INT64_C(1) << 40)) break;
+            K *= 2;
+            Kc *= 2;
+        }
+        if (ckpt > budget) return SWH_ERR_BUDGET;
+        ob.checkpoint_rows = K;
+        ob.checkpoint_cols = Kc;
+    }
+    for (Job& j : jobs) {
+        const int rc = choose_plan(j.m, j.n, &ob, INT64_MAX / 4, &j.pl);
+        if (rc) return rc;
+    }
+    const int rc = run_jobs(c, jobs, sc, engine_opts(o, true), out);
+    if (statuses)
+        for (size_t k = 0; k < jobs.size(); ++k) statuses[k] = rc ? rc : jobs[k].status;
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int swh_batch_align_device(swh_ctx* c, const swh_pair* pairs, int64_t npairs, const swh_scoring* sc,
+                           const swh_opts* o, swh_result* out, int32_t* statuses) {
+    if (!c || (npairs > 0 && (!pairs || !out)) || npairs < 0) return SWH_ERR_INVALID_ARG;
+    if (!scoring_ok(sc)) return SWH_ERR_SCORING;
+    HCK(c, hipSetDevice(c->device));
+    for (int64_t k = 0; k < npairs; ++k) result_init(&out[k]);
+    if (npairs == 0) return SWH_OK;
+    std::vector<Job> jobs((size_t)npairs);
+    const bool tb = !(o && o->score_only);
+    for (int64_t k = 0; k < npairs; ++k) {
+        const swh_pair& q = pairs[k];
+        if (q.len1 < 0 || q.len2 < 0 || (q.len1 && !q.seq1) || (q.len2 && !q.seq2)) return SWH_ERR_INVALID_ARG;
+        jobs[(size_t)k] = make_job(q.seq1, q.len1, q.seq2, q.len2, tb);
+    }
+    return batch_run(c, jobs, sc, o, out, statuses);
 }
 
 int swh_batch_align(swh_ctx* c, const swh_pair* pairs, int64_t npairs, const swh_scoring* sc,
                     const swh_opts* o, swh_result* out, int32_t* statuses) {
     if (!c || (npairs > 0 && (!pairs || !out)) || npairs < 0) return SWH_ERR_INVALID_ARG;
     if (!scoring_ok(sc)) return SWH_ERR_SCORING;
+    HCK(c, hipSetDevice(c->device));
+    for (int64_t k = 0; k < npairs; ++k) result_init(&out[k]);
+    if (npairs == 0) return SWH_OK;
+    // inputs packed into one pinned staging buffer and copied with one transfer
+    int64_t total = 0;
     for (int64_t k = 0; k < npairs; ++k) {
-        const int rc = swh_align_pair(c, pairs[k].seq1, pairs[k].len1, pairs[k].seq2, pairs[k].len2, sc, o, &out[k]);
-        if (statuses) statuses[k] = rc;
+        const swh_pair& q = pairs[k];
+        if (q.len1 < 0 || q.len2 < 0 || (q.len1 && !q.seq1) || (q.len2 && !q.seq2)) return SWH_ERR_INVALID_ARG;
+        total += q.len1 + q.len2;
     }
-    return SWH_OK;
+    HCK(c, hipStreamSynchronize(c->stream));  // the staging buffer may still feed a copy
+    if ((size_t)total + 64 > c->staging_cap) {
+        if (c->staging) (void)hipHostFree(c->staging);
+        c->staging = nullptr;
+        c->staging_cap = 0;
+        HCK(c, hipHostMalloc((void**)&c->staging, (size_t)total + 64, 0));
+        c->staging_cap = (size_t)total + 64;
+    }
+    HCK(c, ensure(c->seqs, (size_t)total + 64));
+    const uint8_t* dev = (const uint8_t*)c->seqs.p;
+    std::vector<Job> jobs((size_t)npairs);
+    const bool tb = !(o && o->score_only);
+    int64_t off = 0;
+    for (int64_t k = 0; k < npairs; ++k) {
+        const swh_pair& q = pairs[k];
+        if (q.len1) memcpy(c->staging + off, q.seq1, (size_t)q.len1);
+        if (q.len2) memcpy(c->staging + off + q.len1, q.seq2, (size_t)q.len2);
+        jobs[(size_t)k] = make_job(dev + off, q.len1, dev + off + q.len1, q.len2, tb);
+        off += q.len1 + q.len2;
+    }
+    if (total) HCK(c, hipMemcpyAsync(c->seqs.p, c->staging, (size_t)total, hipMemcpyHostToDevice, c->stream));
+    return batch_run(c, jobs, sc, o, out, statuses);
 }
 
 void swh_free(void* p) { free(p); }

@@ -24,45 +24,56 @@ struct Score {
     int32_t M, X, O, E;  // match, mismatch, gap open ("first"), gap extension
 };
 
-// One independent rectangle of the DP matrix, rows [r0, r1] x columns [c0, c1]
-// (absolute, inclusive).  It is cut into tiles of kStripeRows rows (stripe s) by W
-// columns (segment g); tile (s, g) needs the bottom row of (s-1, g), the right column of
-// (s, g-1) and the bottom-right cell of (s-1, g-1).  All tiles with s + g = d are
-// independent and form one launch; launches run in order of d, so every hand-off between
-// tiles crosses a kernel boundary and no wave ever waits on another.
+// One independent rectangle of one pair's DP matrix, rows [r0, r1] x columns [c0, c1]
+// (absolute within the pair, inclusive).  It is cut into tiles of kStripeRows rows
+// (stripe s) by W columns (segment g); tile (s, g) needs the bottom row of (s-1, g), the
+// right column of (s, g-1) and the bottom-right cell of (s-1, g-1).  All tiles with
+// s + g = d are independent and form one launch, together with the same diagonal of every
+// other region of the sweep (other blocks, other pairs); launches run in order of d, so
+// every hand-off between tiles crosses a kernel boundary and no wave waits on another.
 struct Region {
+    const uint8_t* A;  // the pair's rows (m bytes)
+    const uint8_t* B;  // the pair's columns (n bytes)
+    int64_t m, n;
     int64_t r0, r1, c0, c1;
     const int2* top;   // {H,F} of row r0-1 at columns c0..c1 (index col-c0); nullptr = zero row
-    const int2* left;  // {H,E} of column c0-1 indexed by absolute row; nullptr = zero column
+    const int2* left;  // {H,E} of column c0-1 indexed by row; nullptr = zero column
     int2* rb;          // 2 x (c1-c0+1) {H,F}: bottom row of stripe s, kept in slot (s+1)&1
     int2* cb;          // 2 x (r1-r0+1) {H,E}: right column of segment g, kept in slot (g+1)&1
     int32_t* cor;      // 2 x stripes: H(R0-1, C0-1) of the next segment, kept in slot (g+1)&1
-    uint16_t* dirs;    // kDirs: [tile s*G+g][step][lane] packed direction codes
-    int32_t* found;    // kSearch: smallest absolute column holding the target value
-};
-
-struct SweepParams {
-    const uint8_t* A;  // rows
-    int64_t m;
-    const uint8_t* B;  // columns
-    int64_t n;
-    Score sc;
-    const Region* regions;
-    int32_t nregions;
-    int32_t per;       // tiles per region in this launch (grid = nregions * per)
-    int64_t S, G, W;   // stripes per region, segments per region (widest), columns per segment
-    int64_t d, s_lo;   // anti-diagonal of this launch and its first stripe
-    int64_t tstride;   // kDirs: steps stored per tile
-    // kStage1 outputs
-    int32_t* rowmax;   // m
-    int2* srow;        // (m/K) x n {H,F}
+    // kStage1 outputs (nullable checkpoints)
+    int32_t* rowmax;   // m row maxima
+    int2* srow;        // (m/K) x n {H,F}: rows K-1, 2K-1, ...
     int64_t K;         // multiple of kStripeRows
-    int2* scol;        // (n/Kc) x m {H,E}
+    int2* scol;        // (n/Kc) x m {H,E}: columns Kc-1, 2Kc-1, ...
     int64_t Kc;        // power of two >= 64
+    // kDirs
+    uint16_t* dirs;    // [tile s*G+g][step][lane] packed direction codes
     // kSearch
     int64_t target_row;
     int32_t target;
     int32_t _pad;
+    int32_t* found;    // smallest column of target_row holding the target value
+};
+
+// The tiles of one region on the launch's anti-diagonal: stripes s_lo, s_lo+1, ... are
+// tiles first_tile, first_tile+1, ... of the launch.
+struct TileSpan {
+    int64_t first_tile;
+    int32_t region;
+    int32_t s_lo;
+};
+
+struct SweepParams {
+    Score sc;
+    const Region* regions;
+    const TileSpan* spans;  // this launch's spans, ordered by first_tile
+    int32_t nspans;
+    int32_t _pad;
+    int64_t ntiles;         // tiles of this launch (grid = ceil(ntiles / kSweepWaves) workgroups)
+    int64_t G, W;           // segments per stripe (kDirs code layout), columns per segment
+    int64_t d;              // anti-diagonal of this launch
+    int64_t tstride;        // kDirs: steps stored per tile
 };
 
 struct WalkState {
@@ -87,8 +98,9 @@ struct WalkRegion {
     const uint16_t* dirs;        // Region::dirs layout
 };
 
-// The walk visits the regions of one traceback round.  Region a*(L+1)+b covers (part of)
-// the checkpoint block (brow-a, bcol-b); rows == 0 marks a block not recomputed.
+// One pair's walk in a traceback round: it visits the pair's regions of the round.  Region
+// a*(L+1)+b covers (part of) the checkpoint block (brow-a, bcol-b); rows == 0 marks a
+// block not recomputed.
 struct WalkArgs {
     const uint8_t* A;
     const uint8_t* B;
@@ -108,8 +120,14 @@ struct WalkArgs {
 hipError_t launch_sweep(int mode, const SweepParams& p, size_t lds_pad, hipStream_t s);
 size_t sweep_static_lds();                     // static LDS of a sweep workgroup
 hipError_t sweep_allow_lds(size_t dyn_bytes);  // permits dyn_bytes of dynamic LDS
-hipError_t launch_argmax_rows(const int32_t* rowmax, int64_t m, unsigned long long* key,
-                              hipStream_t s);
-hipError_t launch_walk(const WalkArgs& a, hipStream_t s);
+// Row argmax of many pairs: job k reduces rowmax[off[k] .. off[k] + len[k]) into key[k]
+// (score << 32 | ~row; key must be zeroed).
+struct ArgmaxJob {
+    const int32_t* rowmax;
+    int64_t m;
+    unsigned long long* key;
+};
+hipError_t launch_argmax_rows(const ArgmaxJob* jobs, int njobs, int blocks_per_job, hipStream_t s);
+hipError_t launch_walk(const WalkArgs* jobs, int njobs, hipStream_t s);  // one workgroup per job
 
 }  // namespace swk

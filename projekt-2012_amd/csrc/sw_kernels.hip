@@ -50,7 +50,15 @@ __device__ __forceinline__ int wave_lane() { return (int)(threadIdx.x & 63u); }
 
 // One tile of a sweep: rows [R0, R0 + 64R) clipped to r1, columns [C0, C0 + w).
 struct TileCtx {
+    const uint8_t* A;   // the pair's rows and columns
+    const uint8_t* B;
+    int64_t m;          // the pair's rows (stride of the checkpoint columns)
     int64_t R0, r1, C0, w;
+    int2* scol;         // stage 1: checkpoint columns (nullable), every Kc columns
+    int64_t Kc;
+    int32_t* rowmax;    // stage 1: row maxima
+    int64_t target_row; // kSearch
+    int32_t target;
     const int2* top;    // {H,F} of row R0-1 at columns C0.. (index col-C0); nullptr = zero row
     int2* bot;          // receives {H,F} of the tile's last row (index col-C0); nullptr = none
     int2* srow;         // stage 1: checkpoint row receiving the last row (index col-C0) or nullptr
@@ -71,7 +79,7 @@ __device__ __forceinline__ void load_chunk_inputs(const SweepParams& p, const Ti
         F = v.y;
     }
     in = make_int2(H + p.sc.O, F + p.sc.E);
-    b1 = (col < T.w) ? (int)p.B[T.C0 + col] : 0x100;
+    b1 = (col < T.w) ? (int)T.B[T.C0 + col] : 0x100;
 }
 
 // 64 steps of one tile.  The chunk's inputs wait in register queues, one column per
@@ -138,7 +146,7 @@ __device__ __forceinline__ void chunk_steps(const SweepParams& p, const TileCtx&
                 if (!EDGE || valid) rmax[k] = max(rmax[k], H);
             }
             if (MODE == kSearch) {
-                if (valid && row0 + k == p.target_row && H == p.target && found_col < 0) found_col = (int)col;
+                if (valid && row0 + k == T.target_row && H == T.target && found_col < 0) found_col = (int)col;
             }
             Ho_up = Hon;
             Fe_up = F + Ev;
@@ -150,8 +158,8 @@ __device__ __forceinline__ void chunk_steps(const SweepParams& p, const TileCtx&
         if (DUMP) {
             const int64_t x = 64 * c + s - lane;
             const int64_t X = T.C0 + x;
-            if (x >= 0 && x < w && ((X + 1) & (p.Kc - 1)) == 0) {
-                int2* dst = p.scol + ((X + 1) / p.Kc - 1) * p.m;
+            if (x >= 0 && x < w && ((X + 1) & (T.Kc - 1)) == 0) {
+                int2* dst = T.scol + ((X + 1) / T.Kc - 1) * T.m;
 #pragma unroll
                 for (int k = 0; k < R; ++k)
                     if (row0 + k <= T.r1) dst[row0 + k] = make_int2(Ho[k] - O, E[k]);
@@ -183,7 +191,7 @@ __device__ void sweep_tile(const SweepParams& p, const TileCtx& T, const int2* l
     for (int k = 0; k < R; ++k) {
         const int64_t row = row0 + k;
         const bool in = row <= T.r1;
-        a[k] = in ? (uint32_t)p.A[row] : 0x200u;
+        a[k] = in ? (uint32_t)T.A[row] : 0x200u;
         if (left && in) {
             const int2 v = left[row - left_r0];
             Ho[k] = v.x + O;
@@ -213,9 +221,9 @@ __device__ void sweep_tile(const SweepParams& p, const TileCtx& T, const int2* l
         if (c + 1 < nchunks) load_chunk_inputs(p, T, c + 1, in_reg, b_reg);
         const bool interior = c >= 1 && c <= last_interior;
         bool dump = false;
-        if (MODE == kStage1 && p.scol) {
+        if (MODE == kStage1 && T.scol) {
             const int64_t lo = T.C0 + max(64 * c - 63, (int64_t)0), hi = T.C0 + min(64 * c + 63, w - 1);
-            const int64_t x = ((lo + p.Kc) / p.Kc) * p.Kc - 1;  // smallest x >= lo, x = -1 mod Kc
+            const int64_t x = ((lo + T.Kc) / T.Kc) * T.Kc - 1;  // smallest x >= lo, x = -1 mod Kc
             dump = x <= hi;
         }
         if (!dump) {
@@ -256,7 +264,7 @@ __device__ void sweep_tile(const SweepParams& p, const TileCtx& T, const int2* l
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             const int64_t row = row0 + k;
-            if (row <= T.r1) p.rowmax[row] = first_segment ? rmax[k] : max(p.rowmax[row], rmax[k]);
+            if (row <= T.r1) T.rowmax[row] = first_segment ? rmax[k] : max(T.rowmax[row], rmax[k]);
         }
     }
     if (MODE == kSearch) {
@@ -265,21 +273,35 @@ __device__ void sweep_tile(const SweepParams& p, const TileCtx& T, const int2* l
 }
 
 // One wave = one tile (s, g) with s + g = p.d; tile index x = wave number in the grid,
-// region x / p.per, stripe p.s_lo + x % p.per.  Waves of a workgroup are independent.
+// found in the launch's spans by binary search.  Waves of a workgroup are independent.
 template <int R, int MODE>
 __device__ void sweep_main(const SweepParams& p, SweepLds& lds) {
     constexpr int64_t HR = 64 * R;
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wave
     const int64_t x = (int64_t)blockIdx.x * kSweepWaves + wid;
-    const int64_t q = x / p.per;
-    if (q >= p.nregions) return;
-    const int64_t s = p.s_lo + x % p.per;
+    if (x >= p.ntiles) return;
+    int lo = 0, hi = p.nspans - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (p.spans[mid].first_tile <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    const TileSpan sp = p.spans[lo];
+    const int64_t s = sp.s_lo + (x - sp.first_tile);
     const int64_t g = p.d - s;
-    const Region Rg = p.regions[q];
+    const Region Rg = p.regions[sp.region];
     const int64_t rows = Rg.r1 - Rg.r0 + 1, cols = Rg.c1 - Rg.c0 + 1;
     const int64_t nstr = (rows + HR - 1) / HR, nseg = (cols + p.W - 1) / p.W;
     if (s >= nstr || g >= nseg) return;  // a narrower region has fewer segments
     TileCtx T;
+    T.A = Rg.A;
+    T.B = Rg.B;
+    T.m = Rg.m;
+    T.scol = Rg.scol;
+    T.Kc = Rg.Kc;
+    T.rowmax = Rg.rowmax;
+    T.target_row = Rg.target_row;
+    T.target = Rg.target;
     T.R0 = Rg.r0 + s * HR;
     T.r1 = Rg.r1;
     T.C0 = Rg.c0 + g * p.W;
@@ -288,9 +310,9 @@ __device__ void sweep_main(const SweepParams& p, SweepLds& lds) {
     T.top = s == 0 ? (Rg.top ? Rg.top + coff : nullptr) : Rg.rb + (s & 1) * cols + coff;
     T.bot = s + 1 < nstr ? Rg.rb + ((s + 1) & 1) * cols + coff : nullptr;
     T.srow = nullptr;
-    if (MODE == kStage1 && p.srow) {
+    if (MODE == kStage1 && Rg.srow) {
         const int64_t brow = T.R0 + HR - 1;
-        if (brow <= Rg.r1 && (brow + 1) % p.K == 0) T.srow = p.srow + ((brow + 1) / p.K - 1) * p.n + T.C0;
+        if (brow <= Rg.r1 && (brow + 1) % Rg.K == 0) T.srow = Rg.srow + ((brow + 1) / Rg.K - 1) * Rg.n + T.C0;
     }
     T.dirs = MODE == kDirs ? Rg.dirs + (s * p.G + g) * p.tstride * 64 : nullptr;
     // Left column and corner H(R0-1, C0-1).
@@ -319,24 +341,25 @@ __global__ __launch_bounds__(64 * kSweepWaves) void k_sweep(SweepParams p) {
     sweep_main<R, MODE>(p, lds[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)]);
 }
 
-__device__ void argmax_rows_main(const int32_t* __restrict__ rowmax, int64_t m, unsigned long long* key) {
+// Row argmax of job blockIdx.y over blockIdx.x's share of its rows: the largest score,
+// then the smallest row, packed as score << 32 | ~row.
+__device__ void argmax_rows_main(const ArgmaxJob& j) {
     unsigned long long best = 0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
-        // larger score first, then smaller row
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < j.m; i += (int64_t)gridDim.x * blockDim.x) {
         const unsigned long long k =
-            ((unsigned long long)(uint32_t)rowmax[i] << 32) | (uint32_t)(0xFFFFFFFFu - (uint32_t)i);
+            ((unsigned long long)(uint32_t)j.rowmax[i] << 32) | (uint32_t)(0xFFFFFFFFu - (uint32_t)i);
         best = k > best ? k : best;
     }
     for (int off = 32; off > 0; off >>= 1) {
         const unsigned long long o = __shfl_xor(best, off);
         best = o > best ? o : best;
     }
-    if ((threadIdx.x & 63) == 0) atomicMax(key, best);
+    if ((threadIdx.x & 63) == 0 && best) atomicMax(j.key, best);
 }
 
-__global__ __launch_bounds__(256) void k_argmax_rows(const int32_t* __restrict__ rowmax, int64_t m,
-                                                     unsigned long long* key) {
-    argmax_rows_main(rowmax, m, key);
+__global__ __launch_bounds__(256) void k_argmax_rows(const ArgmaxJob* jobs) {
+    const ArgmaxJob j = jobs[blockIdx.y];
+    argmax_rows_main(j);
 }
 
 // Greedy backward walk over the regions of one traceback round (see WalkArgs).  The codes
@@ -493,8 +516,9 @@ __device__ void walk_main(const WalkArgs& a, WalkLds<R>& lds) {
 }
 
 template <int R>
-__global__ __launch_bounds__(64) void k_walk(WalkArgs a) {
+__global__ __launch_bounds__(64) void k_walk(const WalkArgs* jobs) {
     __shared__ WalkLds<R> lds;
+    const WalkArgs a = jobs[blockIdx.x];
     walk_main<R>(a, lds);
 }
 
@@ -502,8 +526,7 @@ __global__ __launch_bounds__(64) void k_walk(WalkArgs a) {
 
 hipError_t launch_sweep(int mode, const SweepParams& p, size_t lds_pad, hipStream_t s) {
     constexpr int R = kRowsPerLane;
-    const int64_t tiles = (int64_t)p.nregions * p.per;
-    const dim3 grid((unsigned)((tiles + kSweepWaves - 1) / kSweepWaves)), block(64 * kSweepWaves);
+    const dim3 grid((unsigned)((p.ntiles + kSweepWaves - 1) / kSweepWaves)), block(64 * kSweepWaves);
     switch (mode) {
         case kStage1: hipLaunchKernelGGL((k_sweep<R, kStage1>), grid, block, lds_pad, s, p); break;
         case kDirs: hipLaunchKernelGGL((k_sweep<R, kDirs>), grid, block, lds_pad, s, p); break;
@@ -526,16 +549,16 @@ hipError_t sweep_allow_lds(size_t dyn_bytes) {
     return e;
 }
 
-hipError_t launch_argmax_rows(const int32_t* rowmax, int64_t m, unsigned long long* key, hipStream_t s) {
-    int64_t blocks = (m + 255) / 256;
-    if (blocks > 4096) blocks = 4096;
-    if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_argmax_rows, dim3((unsigned)blocks), dim3(256), 0, s, rowmax, m, key);
+hipError_t launch_argmax_rows(const ArgmaxJob* jobs, int njobs, int blocks_per_job, hipStream_t s) {
+    if (njobs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_argmax_rows, dim3((unsigned)(blocks_per_job > 1 ? blocks_per_job : 1), (unsigned)njobs), dim3(256), 0, s,
+                       jobs);
     return hipGetLastError();
 }
 
-hipError_t launch_walk(const WalkArgs& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_walk<kRowsPerLane>, dim3(1), dim3(64), 0, s, a);
+hipError_t launch_walk(const WalkArgs* jobs, int njobs, hipStream_t s) {
+    if (njobs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_walk<kRowsPerLane>, dim3((unsigned)njobs), dim3(64), 0, s, jobs);
     return hipGetLastError();
 }
 

@@ -98,7 +98,7 @@ std::vector<char> mutate(Rng& r, const char* src, int64_t len, double p_sub, dou
 uint8_t* to_heap(const std::vector<char>& v) {
     uint8_t* p = (uint8_t*)malloc(v.size() + 1);
     if (!p) return nullptr;
-    memcpy(p, v.data(), v.size());
+    if (!v.empty()) memcpy(p, v.data(), v.size());
     p[v.size()] = 0;
     return p;
 }

@@ -74,6 +74,8 @@ EXPORTS = [
                               C.c_void_p]),
     ("swh_batch_align", C.c_int, [C.c_void_p, C.POINTER(Pair), C.c_int64, C.POINTER(Scoring),
                                   C.POINTER(Opts), C.POINTER(Result), C.POINTER(C.c_int32)]),
+    ("swh_batch_align_device", C.c_int, [C.c_void_p, C.POINTER(Pair), C.c_int64, C.POINTER(Scoring),
+                                         C.POINTER(Opts), C.POINTER(Result), C.POINTER(C.c_int32)]),
     ("swh_synth_pair", C.c_int, [C.c_uint64, C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_double,
                                  C.c_double, C.c_double, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     ("swh_synth_mutate", C.c_int, [C.c_uint64, C.c_void_p, C.c_int64, C.c_double, C.c_double,
@@ -235,16 +237,30 @@ class Context:
             bb = C.create_string_buffer(b, len(b))
             keep += [ba, bb]
             arr[k] = Pair(C.cast(ba, C.c_void_p), len(a), C.cast(bb, C.c_void_p), len(b))
-        res = (Result * len(pairs))()
-        st = (C.c_int32 * len(pairs))()
+        return self._batch_call("swh_batch_align", arr, len(pairs), sc, kw)
+
+    def batch_device(self, dev_pairs, sc, **kw):
+        """dev_pairs: list of (ptr1, len1, ptr2, len2) with device pointers (e.g. slices of
+        one torch uint8 tensor).  Returns (results, statuses)."""
+        arr = (Pair * len(dev_pairs))()
+        for k, (p1, l1, p2, l2) in enumerate(dev_pairs):
+            arr[k] = Pair(C.c_void_p(p1), l1, C.c_void_p(p2), l2)
+        return self._batch_call("swh_batch_align_device", arr, len(dev_pairs), sc, kw)
+
+    def _batch_call(self, fn, arr, npairs, sc, kw):
+        import time
+        res = (Result * npairs)()
+        st = (C.c_int32 * npairs)()
         o = make_opts(**kw)
-        rc = lib().swh_batch_align(self._h, arr, len(pairs), C.byref(sc), C.byref(o), res, st)
-        if rc != SWH_OK:
-            raise SWError(rc, "swh_batch_align")
+        t0 = time.perf_counter()
+        rc = getattr(lib(), fn)(self._h, arr, npairs, C.byref(sc), C.byref(o), res, st)
+        self.last_call_s = time.perf_counter() - t0  # the library call alone (results on the host)
         out = []
-        for k in range(len(pairs)):
-            out.append(_result_dict(res[k]) if st[k] == SWH_OK else None)
+        for k in range(npairs):
+            out.append(_result_dict(res[k]) if rc == SWH_OK and st[k] == SWH_OK else None)
             lib().swh_result_free(C.byref(res[k]))
+        if rc != SWH_OK:
+            raise SWError(rc, fn)
         return out, list(st)
 
 
@@ -276,6 +292,22 @@ def synth_mutate(seed, src, p_sub, p_indel, flank_frac):
 
 
 # Synthetic workloads (seed = 2012 + workload index).
+def workload_batch(npairs=65536, seed=2015, lmin=100, lmax=20000):
+    """The batched workload: target lengths log-uniform in [lmin, lmax] (numpy PCG64 from
+    `seed`), uniform ACGT targets; each query is its target mutated at 10% substitutions
+    and 3% indels with flanks trimmed or extended by up to 20% (swh_synth_mutate, seeded
+    per pair).  Returns ([(target, query)], scoring)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    lens = np.exp(rng.uniform(np.log(lmin), np.log(lmax), npairs)).astype(np.int64)
+    alpha = np.frombuffer(b"ACGT", np.uint8)
+    pairs = []
+    for k, L in enumerate(lens):
+        t = alpha[rng.integers(0, 4, int(L))].tobytes()
+        pairs.append((t, synth_mutate(seed * 1_000_003 + k, t, 0.10, 0.03, 0.20)))
+    return pairs, scoring(1, -3, -5, -2)
+
+
 def workload_pair(index):
     """The single-pair workloads: 0 = 10 kbp x 10 kbp (3 kbp homology, 10%/2%),
     1 = 1 Mbp x 1 Mbp (200 kbp homology, 8%/1.5%), 2 = 10 Mbp x 10 Mbp, GC 40%, tandem

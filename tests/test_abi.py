@@ -21,6 +21,7 @@ def declared_functions():
 def test_header_declares_the_boundary():
     names = declared_functions()
     for must in ("swh_ctx_create", "swh_align_pair", "swh_align_pair_device", "swh_forward", "swh_batch_align",
+                 "swh_batch_align_device",
                  "swh_result_free", "swh_strerror"):
         assert must in names
 

@@ -1,0 +1,95 @@
+"""Batched alignment (swh_batch_align / swh_batch_align_device) on an MI355X: every pair of
+a mixed-length batch gets exactly the result it gets alone and the oracle's canonical
+alignment (integer arithmetic: exact comparisons)."""
+import random
+
+import pytest
+
+from conftest import orient
+
+pytestmark = pytest.mark.gpu
+
+SC = (1, -3, -5, -2)
+
+
+def _mixed_pairs(sw, seed, n, lens):
+    rng = random.Random(seed)
+    pairs = []
+    for k in range(n):
+        L = rng.choice(lens)
+        L2 = max(0, L + rng.randint(-L // 3 - 1, L // 3 + 1))
+        a, b = sw.synth_pair(seed * 1000 + k, L, L2, max(1, min(L, L2) // 2), 0.1, 0.02)
+        if rng.random() < 0.3:
+            a, b = b, a
+        pairs.append((a, b))
+    return pairs
+
+
+def _check(orc, pairs, res, st, K=1024, xpoints=True):
+    for (a, b), r, s in zip(pairs, res, st):
+        assert s == 0
+        A, B = orient(a, b)
+        if not A or not B:
+            assert r["score"] == 0
+            continue
+        ref = orc.align_linear(A, B, orc.scoring(*SC), K, K)
+        if ref["score"] == 0:
+            assert r["score"] == 0
+            continue
+        for k in ("score", "end", "start", "cigar"):
+            assert r[k] == ref[k], (k, len(a), len(b))
+        if xpoints:
+            assert r["xpoints"] == [tuple(x) for x in ref["xpoints"]]
+
+
+def test_batch_matches_oracle(sw, orc, ctx):
+    pairs = _mixed_pairs(sw, 7, 150, [0, 1, 2, 17, 64, 255, 256, 257, 600, 1500, 3000])
+    res, st = ctx.batch(pairs, sw.scoring(*SC), xpoints=True)
+    _check(orc, pairs, res, st)
+
+
+@pytest.mark.parametrize("kw", [dict(tile_cols=100), dict(lookahead=-1), dict(lookahead=2, tile_cols=37),
+                                dict(K=512, Kc=256)])
+def test_batch_options_same_result(sw, orc, ctx, kw):
+    pairs = _mixed_pairs(sw, 11, 40, [5, 300, 1100, 2500])
+    res, st = ctx.batch(pairs, sw.scoring(*SC), **kw)
+    _check(orc, pairs, res, st, K=1024, xpoints=False)
+
+
+def test_batch_equals_single(sw, ctx):
+    pairs = _mixed_pairs(sw, 13, 30, [50, 900, 4000])
+    res, st = ctx.batch(pairs, sw.scoring(*SC))
+    for (a, b), r, s in zip(pairs, res, st):
+        one = ctx.align(a, b, sw.scoring(*SC))
+        assert s == 0
+        assert (r["score"], r["end"], r["start"], r["cigar"], r["swapped"]) == \
+            (one["score"], one["end"], one["start"], one["cigar"], one["swapped"])
+
+
+def test_batch_device_resident(sw, orc, ctx):
+    import torch
+    pairs = _mixed_pairs(sw, 17, 60, [1, 120, 800, 2000])
+    blob = b"".join(a + b for a, b in pairs)
+    dev = torch.frombuffer(bytearray(blob + b"\0"), dtype=torch.uint8).to("cuda:0")
+    base, off, dp = dev.data_ptr(), 0, []
+    for a, b in pairs:
+        dp.append((base + off, len(a), base + off + len(a), len(b)))
+        off += len(a) + len(b)
+    torch.cuda.synchronize()
+    res, st = ctx.batch_device(dp, sw.scoring(*SC), xpoints=True)
+    _check(orc, pairs, res, st)
+
+
+def test_batch_score_only(sw, orc, ctx):
+    pairs = _mixed_pairs(sw, 19, 50, [3, 700, 2600])
+    res, st = ctx.batch(pairs, sw.scoring(*SC), score_only=True)
+    for (a, b), r, s in zip(pairs, res, st):
+        assert s == 0
+        A, B = orient(a, b)
+        if not A or not B:
+            assert r["score"] == 0
+            continue
+        sc, i, j = orc.forward(A, B, orc.scoring(*SC))
+        assert r["score"] == sc
+        if sc > 0:
+            assert r["end"] == (i, j) and r["cigar"] is None
