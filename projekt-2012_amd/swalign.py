@@ -291,6 +291,40 @@ def synth_mutate(seed, src, p_sub, p_indel, flank_frac):
         lib().swh_free(out)
 
 
+class DeviceBuffer:
+    """Device memory holding a copy of `data`, allocated with the HIP runtime the library
+    itself uses (for callers and tests that have no other device allocator at hand)."""
+
+    def __init__(self, data):
+        lib()  # the library's HIP runtime is loaded first, so the name below resolves to it
+        self._hip = C.CDLL("libamdhip64.so.7")
+        self._hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self._hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self._hip.hipFree.argtypes = [C.c_void_p]
+        data = _bytes(data)
+        self.size = len(data)
+        self.ptr = C.c_void_p()
+        if self._hip.hipMalloc(C.byref(self.ptr), max(1, self.size)) != 0:
+            raise MemoryError("hipMalloc of %d bytes failed" % self.size)
+        if self.size and self._hip.hipMemcpy(self.ptr, data, self.size, 1) != 0:  # host to device
+            raise RuntimeError("hipMemcpy to the device failed")
+
+    @property
+    def address(self):
+        return self.ptr.value
+
+    def free(self):
+        if self.ptr:
+            self._hip.hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 # Synthetic workloads (seed = 2012 + workload index).
 def workload_batch(npairs=65536, seed=2015, lmin=100, lmax=20000):
     """The batched workload: target lengths log-uniform in [lmin, lmax] (numpy PCG64 from

@@ -67,17 +67,15 @@ def test_batch_equals_single(sw, ctx):
 
 
 def test_batch_device_resident(sw, orc, ctx):
-    import torch
     pairs = _mixed_pairs(sw, 17, 60, [1, 120, 800, 2000])
-    blob = b"".join(a + b for a, b in pairs)
-    dev = torch.frombuffer(bytearray(blob + b"\0"), dtype=torch.uint8).to("cuda:0")
-    base, off, dp = dev.data_ptr(), 0, []
+    dev = sw.DeviceBuffer(b"".join(a + b for a, b in pairs))
+    base, off, dp = dev.address, 0, []
     for a, b in pairs:
         dp.append((base + off, len(a), base + off + len(a), len(b)))
         off += len(a) + len(b)
-    torch.cuda.synchronize()
     res, st = ctx.batch_device(dp, sw.scoring(*SC), xpoints=True)
     _check(orc, pairs, res, st)
+    dev.free()
 
 
 def test_batch_score_only(sw, orc, ctx):
