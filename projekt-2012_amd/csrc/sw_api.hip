@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/swalign_hip.h"
@@ -458,7 +459,6 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
     HCK(c, hipEventRecord(c->ev[4], c->stream));
     HCK(c, ensure(c->ops, (size_t)ops_elems));
     HCK(c, ensure(c->xps, sizeof(XPoint) * (size_t)xps_elems));
-    HCK(c, ensure(c->walk, sizeof(WalkState) * (size_t)nj));
     std::vector<WalkState> states((size_t)nj);
     for (int64_t k : active) {
         WalkState& st = states[(size_t)k];
@@ -469,7 +469,6 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
         st.start_j = st.j;
         st.last_row = st.i;
     }
-    HCK(c, hipMemcpyAsync(c->walk.p, states.data(), sizeof(WalkState) * nj, hipMemcpyHostToDevice, c->stream));
     int64_t wmax = 0;
     for (int64_t k : active) wmax = std::max(wmax, std::min(jobs[k].n, jobs[k].pl.Kc));
     const int64_t Wtb = eo.batch ? W : tile_cols(c, wmax, eo.tile_cols);
@@ -478,6 +477,13 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
     auto blk_dirs = [&](const Job& j) {
         return ceil_div(std::min(j.m, j.pl.K), kStripeRows) * Gblk * tstride * 64 * (int64_t)sizeof(uint16_t);
     };
+    // direction codes per round: a quarter of the free memory, between 2 and 32 GiB
+    int64_t dirs_cap = kTracebackDirsCap;
+    {
+        size_t freeb = 0, totalb = 0;
+        (void)hipMemGetInfo(&freeb, &totalb);
+        dirs_cap = std::max<int64_t>(kTracebackDirsCap, std::min<int64_t>((int64_t)(freeb / 4), INT64_C(32) << 30));
+    }
     int64_t guard = 0;
     int64_t max_blocks = 0;
     for (int64_t k : active)
@@ -488,12 +494,12 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
         int64_t L = std::max(0, eo.lookahead);
         int64_t need = 0;
         for (int64_t k : active) need += blk_dirs(jobs[k]);
-        while (L > 0 && (L + 1) * (L + 1) * need > kTracebackDirsCap) --L;
+        while (L > 0 && (L + 1) * (L + 1) * need > dirs_cap) --L;
         std::vector<int64_t> round;
         int64_t used = 0;
         for (int64_t k : active) {
             const int64_t b = (L + 1) * (L + 1) * blk_dirs(jobs[k]);
-            if (!round.empty() && used + b > kTracebackDirsCap) break;
+            if (!round.empty() && used + b > dirs_cap) break;
             round.push_back(k);
             used += b;
         }
@@ -546,8 +552,10 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
         pd.tstride = tstride;
         rc = run_sweep(c, kDirs, pd, regions, Wtb, Gblk);
         if (rc) return rc;
+        // device arrays of the round; allocated before their addresses go into the walk args
         HCK(c, ensure(c->wregs, sizeof(WalkRegion) * wregs.size()));
         HCK(c, ensure(c->wargs, sizeof(WalkArgs) * wargs.size()));
+        HCK(c, ensure(c->walk, sizeof(WalkState) * (size_t)nr));
         for (int64_t x = 0; x < nr; ++x) {
             const Job& j = jobs[round[x]];
             const WalkState& st = states[(size_t)round[x]];
@@ -566,31 +574,40 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
             wa.Kc = j.pl.Kc;
             wa.srow = j.pl.nsr ? srow + j.srow_off : nullptr;
             wa.n = j.n;
-            wa.st = (WalkState*)c->walk.p + round[x];
+            wa.st = (WalkState*)c->walk.p + x;
             wa.ops = (uint8_t*)c->ops.p + j.ops_off;
             wa.xps = (XPoint*)c->xps.p + j.xps_off;
             wa.xcap = j.xcap;
         }
+        // the round's walk states, compacted
+        std::vector<WalkState> rst((size_t)nr);
+        for (int64_t x = 0; x < nr; ++x) rst[(size_t)x] = states[(size_t)round[x]];
+        HCK(c, hipMemcpyAsync(c->walk.p, rst.data(), sizeof(WalkState) * nr, hipMemcpyHostToDevice, c->stream));
         HCK(c, hipMemcpyAsync(c->wregs.p, wregs.data(), sizeof(WalkRegion) * wregs.size(), hipMemcpyHostToDevice,
                               c->stream));
         HCK(c, hipMemcpyAsync(c->wargs.p, wargs.data(), sizeof(WalkArgs) * wargs.size(), hipMemcpyHostToDevice,
                               c->stream));
         HCK(c, launch_walk((const WalkArgs*)c->wargs.p, (int)nr, c->stream));
-        std::vector<WalkState> prev(states);
-        HCK(c, hipMemcpyAsync(states.data(), c->walk.p, sizeof(WalkState) * nj, hipMemcpyDeviceToHost, c->stream));
+        std::vector<WalkState> after((size_t)nr);
+        HCK(c, hipMemcpyAsync(after.data(), c->walk.p, sizeof(WalkState) * nr, hipMemcpyDeviceToHost, c->stream));
         HCK(c, hipStreamSynchronize(c->stream));
-        std::vector<int64_t> next;
-        for (int64_t k : active) {
-            const WalkState& st = states[(size_t)k];
-            const WalkState& pv = prev[(size_t)k];
-            if (st.done) continue;
-            const bool walked = std::find(round.begin(), round.end(), k) != round.end();
-            if (walked && st.i == pv.i && st.j == pv.j && st.layer == pv.layer) {
+        std::vector<char> finished((size_t)nj, 0);
+        for (int64_t x = 0; x < nr; ++x) {
+            const int64_t k = round[x];
+            const WalkState& pv = rst[(size_t)x];
+            const WalkState& st = after[(size_t)x];
+            states[(size_t)k] = st;
+            if (st.done) {
+                finished[(size_t)k] = 1;
+            } else if (st.i == pv.i && st.j == pv.j && st.layer == pv.layer) {
                 jobs[k].status = SWH_ERR_INTERNAL;  // no progress
-                continue;
+                finished[(size_t)k] = 1;
             }
-            next.push_back(k);
         }
+        std::vector<int64_t> next;
+        next.reserve(active.size());
+        for (int64_t k : active)
+            if (!finished[(size_t)k]) next.push_back(k);
         active.swap(next);
     }
     HCK(c, hipEventRecord(c->ev[5], c->stream));
@@ -603,45 +620,67 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
         HCK(c, hipMemcpyAsync(xps_h.data(), c->xps.p, sizeof(XPoint) * xps_elems, hipMemcpyDeviceToHost, c->stream));
     HCK(c, hipStreamSynchronize(c->stream));
     const float ms_tb = elapsed(c->ev[4], c->ev[5]);
-    for (int64_t k : scored) {
-        Job& j = jobs[k];
-        swh_result& r = out[k];
-        if (!j.traceback || j.status != SWH_OK) continue;
-        const WalkState& st = states[(size_t)k];
-        r.ms_traceback = ms_tb;
-        r.ms_total = r.ms_forward + r.ms_end_search + ms_tb;
-        r.start_row = st.start_i;
-        r.start_col = st.start_j;
-        const uint8_t* ops = ops_h.data() + j.ops_off;
-        std::string rle;
-        rle.reserve((size_t)st.nops / 4 + 16);
-        char num[32];
-        for (int64_t b = st.nops - 1; b >= 0;) {  // ops were written end -> start
-            int64_t e = b;
-            while (e >= 0 && ops[e] == ops[b]) --e;
-            const int len = snprintf(num, sizeof num, "%lld%c", (long long)(b - e), ops[b]);
-            rle.append(num, (size_t)len);
-            b = e;
-        }
-        r.n_ops = st.nops;
-        r.cigar = (char*)malloc(rle.size() + 1);
-        if (!r.cigar) return SWH_ERR_OUT_OF_MEMORY;
-        memcpy(r.cigar, rle.data(), rle.size());
-        r.cigar[rle.size()] = 0;
-        r.cigar_len = (int64_t)rle.size();
-        if (eo.want_xp) {
-            const int64_t nx = std::min<int64_t>(st.nxp, j.xcap);
-            r.xpoints = (swh_xpoint*)malloc(sizeof(swh_xpoint) * (size_t)(nx + 1));
-            if (!r.xpoints) return SWH_ERR_OUT_OF_MEMORY;
-            const XPoint* xs = xps_h.data() + j.xps_off;
-            for (int64_t q = 0; q < nx; ++q) {
-                const XPoint& x = xs[nx - 1 - q];
-                r.xpoints[q] = swh_xpoint{x.row, x.col, x.score, x.layer};
+    // CIGARs are built on the host threads, pairs split between them
+    std::vector<int> fail(16, 0);
+    auto build = [&](int tid, int nth) {
+        for (size_t x = (size_t)tid; x < scored.size(); x += (size_t)nth) {
+            const int64_t k = scored[x];
+            Job& j = jobs[k];
+            swh_result& r = out[k];
+            if (!j.traceback || j.status != SWH_OK) continue;
+            const WalkState& st = states[(size_t)k];
+            r.ms_traceback = ms_tb;
+            r.ms_total = r.ms_forward + r.ms_end_search + ms_tb;
+            r.start_row = st.start_i;
+            r.start_col = st.start_j;
+            const uint8_t* ops = ops_h.data() + j.ops_off;
+            std::string rle;
+            rle.reserve((size_t)st.nops / 4 + 16);
+            char num[32];
+            for (int64_t b = st.nops - 1; b >= 0;) {  // ops were written end -> start
+                int64_t e = b;
+                while (e >= 0 && ops[e] == ops[b]) --e;
+                const int len = snprintf(num, sizeof num, "%lld%c", (long long)(b - e), ops[b]);
+                rle.append(num, (size_t)len);
+                b = e;
             }
-            r.xpoints[nx] = swh_xpoint{r.end_row, r.end_col, j.best, 0};
-            r.n_xpoints = nx + 1;
+            r.n_ops = st.nops;
+            r.cigar = (char*)malloc(rle.size() + 1);
+            if (!r.cigar) {
+                fail[(size_t)tid] = 1;
+                return;
+            }
+            memcpy(r.cigar, rle.data(), rle.size());
+            r.cigar[rle.size()] = 0;
+            r.cigar_len = (int64_t)rle.size();
+            if (eo.want_xp) {
+                const int64_t nx = std::min<int64_t>(st.nxp, j.xcap);
+                r.xpoints = (swh_xpoint*)malloc(sizeof(swh_xpoint) * (size_t)(nx + 1));
+                if (!r.xpoints) {
+                    fail[(size_t)tid] = 1;
+                    return;
+                }
+                const XPoint* xs = xps_h.data() + j.xps_off;
+                for (int64_t q = 0; q < nx; ++q) {
+                    const XPoint& xp = xs[nx - 1 - q];
+                    r.xpoints[q] = swh_xpoint{xp.row, xp.col, xp.score, xp.layer};
+                }
+                r.xpoints[nx] = swh_xpoint{r.end_row, r.end_col, j.best, 0};
+                r.n_xpoints = nx + 1;
+            }
         }
+    };
+    const int nth = (int)std::max<int64_t>(1, std::min<int64_t>({16, (int64_t)std::thread::hardware_concurrency(),
+                                                                (int64_t)scored.size() / 64 + 1}));
+    if (nth == 1) {
+        build(0, 1);
+    } else {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nth; ++t) pool.emplace_back(build, t, nth);
+        for (auto& t : pool) t.join();
     }
+    for (int f : fail)
+        if (f) return SWH_ERR_OUT_OF_MEMORY;
     return SWH_OK;
 }
 
@@ -652,7 +691,7 @@ EngineOpts engine_opts(const swh_opts* o, bool batch) {
     eo.tile_cols = o ? o->tile_cols : 0;
     eo.want_xp = o && o->want_xpoints;
     eo.batch = batch;
-    eo.lookahead = batch ? 1 : 3;
+    eo.lookahead = batch ? 0 : 3;
     if (o && o->tb_lookahead != 0) eo.lookahead = std::max(0, o->tb_lookahead);
     return eo;
 }
