@@ -41,8 +41,55 @@ WORKLOADS = {
 METRIC = "GCUPS on 1 MI355X (single long pair, batched mix) + served p50/p99 vs plain"
 
 
-def roofline(cells, fwd_ms, launches):
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def _counter_avg(fname, counter, kernel_part):
+    """Average per dispatch of `counter` for the kernel whose name contains kernel_part, read
+    from a committed rocprofv3 summary (profiles/*.txt), or None."""
+    path = os.path.join(PROFILES, fname)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = [x.strip() for x in line.split("|")]
+        if len(f) == 5 and f[0] == counter and kernel_part in f[1]:
+            return float(f[4])
+    return None
+
+
+def measured_traffic(workload):
+    """HBM bytes per stage-1 launch from the committed FETCH_SIZE / WRITE_SIZE profiles of
+    this workload, each scaled by what the counter reads for a 256 MiB copy."""
+    if workload != 1:
+        return None, None
+    k = "k_sweep<4, 0>"
+    fetch = _counter_avg("k_sweep_stage1_1Mbp_pair_fetch_size.txt", "FETCH_SIZE", k)
+    write = _counter_avg("k_sweep_stage1_1Mbp_pair_write_size.txt", "WRITE_SIZE", k)
+    cf = _counter_avg("copy_256MiB_fetch_size.txt", "FETCH_SIZE", "copyBuffer")
+    cw = _counter_avg("copy_256MiB_write_size.txt", "WRITE_SIZE", "copyBuffer")
+    if None in (fetch, write, cf, cw):
+        return None, None
+    kib = 268435456 / 1024.0
+    return int((fetch * kib / cf + write * kib / cw) * 1024), "profiles/k_sweep_stage1_1Mbp_pair_{fetch,write}_size.txt"
+
+
+def algorithmic_bytes(cells, n_cols_per_tile, K, Kc):
+    """Stage-1 HBM bytes a tile of 256 x W must move: top row {H,F} + column bytes (9W),
+    left column {H,E} + row bytes (2304), row maxima read back (1024); bottom row (8W),
+    right column (2048), row maxima (1024); plus the checkpoint rows and columns."""
+    W = max(1, n_cols_per_tile)
+    per_cell = (17.0 * W + 6400.0) / (256.0 * W) + 8.0 / K + 8.0 / Kc
+    return cells * per_cell
+
+
+def roofline(cells, fwd_ms, launches, workload=None, W=None, K=None, Kc=None):
     achieved = cells * OPS_PER_CELL / (fwd_ms / 1000.0) / 1e12
+    traffic, source = measured_traffic(workload)
+    extra = {}
+    if W and K and Kc:
+        extra["algorithmic_bytes_per_launch"] = int(algorithmic_bytes(cells, W, K, Kc) / max(launches, 1))
+    if source:
+        extra["traffic_source"] = source
     return {
         "bound": "valu",
         "kernel": "k_sweep<4, kStage1> (forward sweep with checkpoints)",
@@ -50,7 +97,8 @@ def roofline(cells, fwd_ms, launches):
         "peak": round(PEAK_INT32_TOPS, 2),
         "unit": "TOP/s",
         "frac": round(achieved / PEAK_INT32_TOPS, 4),
-        "traffic": None,
+        "traffic": traffic,
+        **extra,
         "launches": launches,
         "avg_launch_us": round(fwd_ms * 1000.0 / max(launches, 1), 2),
         "ops_per_launch": int(cells * OPS_PER_CELL / max(launches, 1)),
@@ -262,7 +310,9 @@ def main():
             "forward_gcups": round(cells / (fwd_ms / 1000.0) / 1e9, 2),
             "device": info["name"], "cus": info["cus"], "clock_mhz": round(clock_ghz * 1000),
         },
-        "roofline": roofline(cells, fwd_ms, r0["forward_launches"]),
+        "roofline": roofline(cells, fwd_ms, r0["forward_launches"], args.workload,
+                             W=-(-n // (4 * int(info["cus"] or 256))), K=r0["checkpoint_rows"] or m,
+                             Kc=r0["checkpoint_cols"] or n),
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(a, b)
