@@ -20,35 +20,11 @@
 #include <thread>
 #include <vector>
 
-#include "../../include/swalign_hip.h"
-#include "sw_internal.h"
+#include "sw_host.h"
 
 using namespace swk;
 
-namespace {
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t cap = 0;
-};
-
-}  // namespace
-
-struct swh_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipDeviceProp_t props{};
-    DevBuf seqA, seqB, rbuf, cb, cor, rowmax, srow, scol, dirs, regions, key, found, walk, ops, xps, wregs,
-        argjobs, wargs, seqs, spans;
-    uint8_t* staging = nullptr;  // pinned host staging of batch inputs
-    size_t staging_cap = 0;
-    hipEvent_t ev[8] = {};
-    size_t lds_pad[3] = {0, 0, 0};  // dynamic LDS per sweep workgroup for w workgroups per CU
-    int wps_fixed = 0;              // sweep waves per SIMD forced by the environment, 0 = model
-    hipError_t last = hipSuccess;
-};
-
-namespace {
+namespace swh_host {
 
 int map_hip(swh_ctx* c, hipError_t e) {
     if (c) c->last = e;
@@ -56,11 +32,6 @@ int map_hip(swh_ctx* c, hipError_t e) {
     return SWH_ERR_HIP;
 }
 
-#define HCK(ctx, x)                                   \
-    do {                                              \
-        hipError_t e__ = (x);                         \
-        if (e__ != hipSuccess) return map_hip(ctx, e__); \
-    } while (0)
 
 hipError_t ensure(DevBuf& b, size_t bytes) {
     if (bytes == 0) bytes = 64;
@@ -99,7 +70,7 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 // Columns per tile: `want` if given, else one tile per resident sweep wave (wps per SIMD)
 // across `cols` columns, at least 256 (a tile spends 63 extra steps filling and draining
 // its lanes).
-int64_t tile_cols(swh_ctx* c, int64_t cols, int64_t want, int wps = 1) {
+int64_t tile_cols(swh_ctx* c, int64_t cols, int64_t want, int wps) {
     if (want > 0) return want;
     const int64_t target = (int64_t)kSweepWaves * wps * std::max(1, c->props.multiProcessorCount);
     return std::max<int64_t>(256, ceil_div(cols, target));
@@ -135,9 +106,8 @@ int choose_wps(swh_ctx* c, int64_t rows, int64_t cols) {
 // of the tile grid, all regions side by side.  The grid has G segments per stripe (the
 // widest region's, or G_fix when given: kDirs lays codes out by it).  Optional events are
 // recorded immediately before the first and after the last launch.
-int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions, int64_t W, int64_t G_fix = 0,
-              hipEvent_t ev_before = nullptr, hipEvent_t ev_after = nullptr, int wps = 1,
-              int64_t* launches = nullptr) {
+int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions, int64_t W, int64_t G_fix,
+              hipEvent_t ev_before, hipEvent_t ev_after, int wps, int64_t* launches, int64_t tstride) {
     const int64_t nr = (int64_t)regions.size();
     std::vector<int64_t> Sr((size_t)nr), Gr((size_t)nr);
     int64_t G = 0, D = 0, cb_elems = 0, rb_elems = 0, cor_elems = 0;
@@ -163,6 +133,9 @@ int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions,
     int64_t cbo = 0, rbo = 0, coro = 0;
     for (int64_t q = 0; q < nr; ++q) {
         Region& r = regions[(size_t)q];
+        r.W = W;
+        r.G = G;
+        r.tstride = tstride;
         r.cb = (int2*)c->cb.p + cbo;
         r.rb = (int2*)c->rbuf.p + rbo;
         r.cor = (int32_t*)c->cor.p + coro;
@@ -187,7 +160,7 @@ int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions,
             const int64_t Sq = Sr[(size_t)q], Gq = Gr[(size_t)q];
             if (d > Sq + Gq - 2) break;
             const int64_t s_lo = std::max<int64_t>(0, d - (Gq - 1)), s_hi = std::min(Sq - 1, d);
-            spans.push_back(TileSpan{t, (int32_t)q, (int32_t)s_lo});
+            spans.push_back(TileSpan{t, (int32_t)q, (int32_t)s_lo, d});
             t += s_hi - s_lo + 1;
         }
         tiles_at[(size_t)d] = t;
@@ -199,11 +172,8 @@ int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions,
     HCK(c, hipMemcpyAsync(c->spans.p, spans.data(), sizeof(TileSpan) * spans.size(), hipMemcpyHostToDevice,
                           c->stream));
     p.regions = (const Region*)c->regions.p;
-    p.G = G;
-    p.W = W;
     if (ev_before) HCK(c, hipEventRecord(ev_before, c->stream));
     for (int64_t d = 0; d < D; ++d) {
-        p.d = d;
         p.spans = (const TileSpan*)c->spans.p + span_at[(size_t)d];
         p.nspans = (int32_t)(span_at[(size_t)d + 1] - span_at[(size_t)d]);
         p.ntiles = tiles_at[(size_t)d];
@@ -215,10 +185,6 @@ int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions,
     HCK(c, hipStreamSynchronize(c->stream));
     return SWH_OK;
 }
-
-struct Plan {
-    int64_t K, Kc, nsr, nsc;
-};
 
 int64_t plan_mem(int64_t m, int64_t n, int64_t K, int64_t Kc) {
     return ((m / K) * n + (n / Kc) * m) * (int64_t)sizeof(int2);
@@ -548,9 +514,7 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
             regions[q].dirs = d;
             wregs[(size_t)slots[q].first].dirs = d;
         }
-        SweepParams pd = p;
-        pd.tstride = tstride;
-        rc = run_sweep(c, kDirs, pd, regions, Wtb, Gblk);
+        rc = run_sweep(c, kDirs, p, regions, Wtb, Gblk, nullptr, nullptr, 1, nullptr, tstride);
         if (rc) return rc;
         // device arrays of the round; allocated before their addresses go into the walk args
         HCK(c, ensure(c->wregs, sizeof(WalkRegion) * wregs.size()));
@@ -628,45 +592,12 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
             Job& j = jobs[k];
             swh_result& r = out[k];
             if (!j.traceback || j.status != SWH_OK) continue;
-            const WalkState& st = states[(size_t)k];
             r.ms_traceback = ms_tb;
             r.ms_total = r.ms_forward + r.ms_end_search + ms_tb;
-            r.start_row = st.start_i;
-            r.start_col = st.start_j;
-            const uint8_t* ops = ops_h.data() + j.ops_off;
-            std::string rle;
-            rle.reserve((size_t)st.nops / 4 + 16);
-            char num[32];
-            for (int64_t b = st.nops - 1; b >= 0;) {  // ops were written end -> start
-                int64_t e = b;
-                while (e >= 0 && ops[e] == ops[b]) --e;
-                const int len = snprintf(num, sizeof num, "%lld%c", (long long)(b - e), ops[b]);
-                rle.append(num, (size_t)len);
-                b = e;
-            }
-            r.n_ops = st.nops;
-            r.cigar = (char*)malloc(rle.size() + 1);
-            if (!r.cigar) {
+            if (fill_alignment(r, states[(size_t)k], ops_h.data() + j.ops_off, xps_h.data() + j.xps_off, j.xcap,
+                               j.best, eo.want_xp) != SWH_OK) {
                 fail[(size_t)tid] = 1;
                 return;
-            }
-            memcpy(r.cigar, rle.data(), rle.size());
-            r.cigar[rle.size()] = 0;
-            r.cigar_len = (int64_t)rle.size();
-            if (eo.want_xp) {
-                const int64_t nx = std::min<int64_t>(st.nxp, j.xcap);
-                r.xpoints = (swh_xpoint*)malloc(sizeof(swh_xpoint) * (size_t)(nx + 1));
-                if (!r.xpoints) {
-                    fail[(size_t)tid] = 1;
-                    return;
-                }
-                const XPoint* xs = xps_h.data() + j.xps_off;
-                for (int64_t q = 0; q < nx; ++q) {
-                    const XPoint& xp = xs[nx - 1 - q];
-                    r.xpoints[q] = swh_xpoint{xp.row, xp.col, xp.score, xp.layer};
-                }
-                r.xpoints[nx] = swh_xpoint{r.end_row, r.end_col, j.best, 0};
-                r.n_xpoints = nx + 1;
             }
         }
     };
@@ -685,6 +616,40 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
 }
 
 void result_init(swh_result* r) { memset(r, 0, sizeof *r); }
+
+int fill_alignment(swh_result& r, const WalkState& st, const uint8_t* ops, const XPoint* xs, int64_t xcap,
+                   int32_t best, bool want_xp) {
+    r.start_row = st.start_i;
+    r.start_col = st.start_j;
+    std::string rle;
+    rle.reserve((size_t)st.nops / 4 + 16);
+    char num[32];
+    for (int64_t b = st.nops - 1; b >= 0;) {  // ops were written end -> start
+        int64_t e = b;
+        while (e >= 0 && ops[e] == ops[b]) --e;
+        const int len = snprintf(num, sizeof num, "%lld%c", (long long)(b - e), ops[b]);
+        rle.append(num, (size_t)len);
+        b = e;
+    }
+    r.n_ops = st.nops;
+    r.cigar = (char*)malloc(rle.size() + 1);
+    if (!r.cigar) return SWH_ERR_OUT_OF_MEMORY;
+    memcpy(r.cigar, rle.data(), rle.size());
+    r.cigar[rle.size()] = 0;
+    r.cigar_len = (int64_t)rle.size();
+    if (want_xp) {
+        const int64_t nx = std::min<int64_t>(st.nxp, xcap);
+        r.xpoints = (swh_xpoint*)malloc(sizeof(swh_xpoint) * (size_t)(nx + 1));
+        if (!r.xpoints) return SWH_ERR_OUT_OF_MEMORY;
+        for (int64_t q = 0; q < nx; ++q) {
+            const XPoint& xp = xs[nx - 1 - q];
+            r.xpoints[q] = swh_xpoint{xp.row, xp.col, xp.score, xp.layer};
+        }
+        r.xpoints[nx] = swh_xpoint{r.end_row, r.end_col, best, 0};
+        r.n_xpoints = nx + 1;
+    }
+    return SWH_OK;
+}
 
 EngineOpts engine_opts(const swh_opts* o, bool batch) {
     EngineOpts eo;
@@ -708,7 +673,9 @@ Job make_job(const uint8_t* d1, int64_t len1, const uint8_t* d2, int64_t len2, b
     return j;
 }
 
-}  // namespace
+}  // namespace swh_host
+
+using namespace swh_host;
 
 extern "C" {
 

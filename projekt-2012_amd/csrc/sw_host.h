@@ -1,0 +1,69 @@
+// Host-side internals shared by the C-ABI (sw_api.hip) and the request server
+// (sw_serve.hip): the context, device buffers, checkpoint plans and the sweep driver.
+// Not part of the public interface.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "../../include/swalign_hip.h"
+#include "sw_internal.h"
+
+// A grow-only device allocation.
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+
+struct swh_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipDeviceProp_t props{};
+    DevBuf seqA, seqB, rbuf, cb, cor, rowmax, srow, scol, dirs, regions, key, found, walk, ops, xps, wregs,
+        argjobs, wargs, seqs, spans;
+    uint8_t* staging = nullptr;  // pinned host staging of batch inputs
+    size_t staging_cap = 0;
+    hipEvent_t ev[8] = {};
+    size_t lds_pad[3] = {0, 0, 0};  // dynamic LDS per sweep workgroup for w workgroups per CU
+    int wps_fixed = 0;              // sweep waves per SIMD forced by the environment, 0 = model
+    hipError_t last = hipSuccess;
+};
+
+namespace swh_host {
+
+int map_hip(swh_ctx* c, hipError_t e);
+
+#define HCK(ctx, x)                                      \
+    do {                                                 \
+        hipError_t e__ = (x);                            \
+        if (e__ != hipSuccess) return swh_host::map_hip(ctx, e__); \
+    } while (0)
+
+hipError_t ensure(DevBuf& b, size_t bytes);
+int64_t ceil_div(int64_t a, int64_t b);
+int64_t next_pow2(int64_t v);
+bool scoring_ok(const swh_scoring* s);
+float elapsed(hipEvent_t a, hipEvent_t b);
+void result_init(swh_result* r);
+
+// Columns per tile: `want` if given, else one tile per resident sweep wave.
+int64_t tile_cols(swh_ctx* c, int64_t cols, int64_t want, int wps = 1);
+int choose_wps(swh_ctx* c, int64_t rows, int64_t cols);
+
+// One sweep over `regions`, one launch per tile anti-diagonal (see sw_api.hip).
+int run_sweep(swh_ctx* c, int mode, swk::SweepParams p, std::vector<swk::Region>& regions, int64_t W,
+              int64_t G_fix = 0, hipEvent_t ev_before = nullptr, hipEvent_t ev_after = nullptr, int wps = 1,
+              int64_t* launches = nullptr, int64_t tstride = 0);
+
+struct Plan {
+    int64_t K, Kc, nsr, nsc;
+};
+int64_t plan_mem(int64_t m, int64_t n, int64_t K, int64_t Kc);
+int choose_plan(int64_t m, int64_t n, const swh_opts* o, int64_t budget, Plan* pl);
+
+// Fills r's start, CIGAR (run-length, from ops written end -> start) and crosspoints.
+int fill_alignment(swh_result& r, const swk::WalkState& st, const uint8_t* ops, const swk::XPoint* xps,
+                   int64_t xcap, int32_t best, bool want_xp);
+
+}  // namespace swh_host

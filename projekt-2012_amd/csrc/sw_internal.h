@@ -36,6 +36,9 @@ struct Region {
     const uint8_t* B;  // the pair's columns (n bytes)
     int64_t m, n;
     int64_t r0, r1, c0, c1;
+    int64_t W;         // columns per tile
+    int64_t G;         // kDirs: segments per stripe in the code layout (>= tiles across the region)
+    int64_t tstride;   // kDirs: steps stored per tile
     const int2* top;   // {H,F} of row r0-1 at columns c0..c1 (index col-c0); nullptr = zero row
     const int2* left;  // {H,E} of column c0-1 indexed by row; nullptr = zero column
     int2* rb;          // 2 x (c1-c0+1) {H,F}: bottom row of stripe s, kept in slot (s+1)&1
@@ -56,12 +59,14 @@ struct Region {
     int32_t* found;    // smallest column of target_row holding the target value
 };
 
-// The tiles of one region on the launch's anti-diagonal: stripes s_lo, s_lo+1, ... are
-// tiles first_tile, first_tile+1, ... of the launch.
+// The tiles of one region on one of its anti-diagonals d: stripes s_lo, s_lo+1, ... are
+// tiles first_tile, first_tile+1, ... of the launch.  Regions of one launch may be at
+// different diagonals (pairs admitted at different times).
 struct TileSpan {
     int64_t first_tile;
     int32_t region;
     int32_t s_lo;
+    int64_t d;
 };
 
 struct SweepParams {
@@ -71,9 +76,6 @@ struct SweepParams {
     int32_t nspans;
     int32_t _pad;
     int64_t ntiles;         // tiles of this launch (grid = ceil(ntiles / kSweepWaves) workgroups)
-    int64_t G, W;           // segments per stripe (kDirs code layout), columns per segment
-    int64_t d;              // anti-diagonal of this launch
-    int64_t tstride;        // kDirs: steps stored per tile
 };
 
 struct WalkState {

@@ -272,7 +272,7 @@ __device__ void sweep_tile(const SweepParams& p, const TileCtx& T, const int2* l
     }
 }
 
-// One wave = one tile (s, g) with s + g = p.d; tile index x = wave number in the grid,
+// One wave = one tile (s, g) with s + g = d of its span; tile index x = wave number in the grid,
 // found in the launch's spans by binary search.  Waves of a workgroup are independent.
 template <int R, int MODE>
 __device__ void sweep_main(const SweepParams& p, SweepLds& lds) {
@@ -288,10 +288,11 @@ __device__ void sweep_main(const SweepParams& p, SweepLds& lds) {
     }
     const TileSpan sp = p.spans[lo];
     const int64_t s = sp.s_lo + (x - sp.first_tile);
-    const int64_t g = p.d - s;
+    const int64_t g = sp.d - s;
     const Region Rg = p.regions[sp.region];
     const int64_t rows = Rg.r1 - Rg.r0 + 1, cols = Rg.c1 - Rg.c0 + 1;
-    const int64_t nstr = (rows + HR - 1) / HR, nseg = (cols + p.W - 1) / p.W;
+    const int64_t W = Rg.W;
+    const int64_t nstr = (rows + HR - 1) / HR, nseg = (cols + W - 1) / W;
     if (s >= nstr || g >= nseg) return;  // a narrower region has fewer segments
     TileCtx T;
     T.A = Rg.A;
@@ -304,8 +305,8 @@ __device__ void sweep_main(const SweepParams& p, SweepLds& lds) {
     T.target = Rg.target;
     T.R0 = Rg.r0 + s * HR;
     T.r1 = Rg.r1;
-    T.C0 = Rg.c0 + g * p.W;
-    T.w = min(p.W, Rg.c1 - T.C0 + 1);
+    T.C0 = Rg.c0 + g * W;
+    T.w = min(W, Rg.c1 - T.C0 + 1);
     const int64_t coff = T.C0 - Rg.c0;
     T.top = s == 0 ? (Rg.top ? Rg.top + coff : nullptr) : Rg.rb + (s & 1) * cols + coff;
     T.bot = s + 1 < nstr ? Rg.rb + ((s + 1) & 1) * cols + coff : nullptr;
@@ -314,7 +315,7 @@ __device__ void sweep_main(const SweepParams& p, SweepLds& lds) {
         const int64_t brow = T.R0 + HR - 1;
         if (brow <= Rg.r1 && (brow + 1) % Rg.K == 0) T.srow = Rg.srow + ((brow + 1) / Rg.K - 1) * Rg.n + T.C0;
     }
-    T.dirs = MODE == kDirs ? Rg.dirs + (s * p.G + g) * p.tstride * 64 : nullptr;
+    T.dirs = MODE == kDirs ? Rg.dirs + (s * Rg.G + g) * Rg.tstride * 64 : nullptr;
     // Left column and corner H(R0-1, C0-1).
     const int2* left;
     int64_t left_r0;
