@@ -37,6 +37,9 @@ WORKLOADS = {
        "+2/-3/-5/-1, full traceback",
     3: "65,536 pairs, target lengths log-uniform in 100 bp - 20 kbp, query = target at 10% sub / 3% indel "
        "with flanks trimmed or extended by up to 20%, +1/-3/-5/-2, full traceback, all pairs in one batch",
+    4: "served stream: Poisson arrivals at 50/80/95% of the measured capacity; each side 70% U[1,10] kbp, "
+       "25% U[10,100] kbp, 5% U[100 kbp,1 Mbp]; queries mutated as in workload 3; +1/-3/-5/-2, full traceback; "
+       "HBM pool 64 GB",
 }
 METRIC = "GCUPS on 1 MI355X (single long pair, batched mix) + served p50/p99 vs plain"
 
@@ -163,6 +166,76 @@ def run_batch(args, torch, sw):
     ctx.close()
 
 
+def run_serve(args, torch, sw):
+    """Workload 4: a request server (swh_server_*) fed by Poisson arrivals.  Capacity is the
+    throughput of the same jobs submitted all at once; then each rate runs the stream at
+    that fraction of capacity.  A job's latency is submission to result (inputs come from
+    host memory, as a request's would).  The plain path aligns each job alone, one after
+    another (swh_align_pair), and gives the latency a job has with the device to itself."""
+    import numpy as np
+    njobs = max(20, args.steps * 20)
+    jobs, sc = sw.workload_serve(njobs)
+    cells = [len(a) * len(b) for a, b in jobs]
+    budget = 64 << 30
+
+    def run(rate):
+        rng = np.random.default_rng(7)
+        gaps = rng.exponential(1.0 / rate, len(jobs)) if rate else np.zeros(len(jobs))
+        with sw.Server(sc, budget=budget) as srv:
+            t0 = time.perf_counter()
+            ids = []
+            due = t0
+            for (a, b), g in zip(jobs, gaps):
+                due += g
+                while time.perf_counter() < due:
+                    time.sleep(min(0.001, max(0.0, due - time.perf_counter())))
+                ids.append(srv.submit(a, b))
+            lat = []
+            for j in ids:
+                st, r = srv.poll(j)
+                assert st == 0
+                lat.append(r["ms_total"])
+            stats = srv.stats()
+        return dict(rate_jobs_per_s=round(rate, 2) if rate else None, gcups=round(stats["gcups"], 2),
+                    p50_ms=round(float(np.percentile(lat, 50)), 2), p99_ms=round(float(np.percentile(lat, 99)), 2),
+                    max_ms=round(max(lat), 2), jobs=len(lat))
+
+    for k in range(args.warmup):
+        run(0)
+        print("bench: warmup done", file=sys.stderr, flush=True)
+    sat = run(0)
+    capacity = len(jobs) * sat["gcups"] * 1e9 / sum(cells)  # jobs per second at saturation
+    print("bench: saturated %.1f GCUPS, capacity %.1f jobs/s" % (sat["gcups"], capacity), file=sys.stderr, flush=True)
+    rates = {}
+    for frac in (0.5, 0.8, 0.95):
+        rates["%d%%" % round(frac * 100)] = run(frac * capacity)
+        print("bench: rate %.0f%% done" % (frac * 100), file=sys.stderr, flush=True)
+    # plain path: each job alone
+    ctx = sw.Context(0)
+    plain = []
+    t0 = time.perf_counter()
+    for a, b in jobs:
+        t1 = time.perf_counter()
+        ctx.align(a, b, sc)
+        plain.append((time.perf_counter() - t1) * 1000.0)
+    plain_gcups = sum(cells) / (time.perf_counter() - t0) / 1e9
+    ctx.close()
+    top = rates["95%"]
+    line = {
+        "metric": METRIC, "value": top["gcups"], "unit": "GCUPS", "n_gpus": 1, "steps": 1, "warmup": args.warmup,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic (numpy PCG64 lengths, targets and arrivals, seed 2016 / 7; queries by swh_synth_mutate)",
+        "config": {
+            "workload": WORKLOADS[4], "jobs": len(jobs), "cells": sum(cells),
+            "saturated": sat, "capacity_jobs_per_s": round(capacity, 2), "rates": rates,
+            "plain": {"gcups": round(plain_gcups, 2), "p50_ms": round(float(np.percentile(plain, 50)), 2),
+                      "p99_ms": round(float(np.percentile(plain, 99)), 2)},
+            "served_vs_plain_gcups": round(top["gcups"] / plain_gcups, 2),
+        },
+    }
+    print(json.dumps(line))
+
+
 def cpu_baseline_pairs(pairs, sc, budget_s=15.0):
     """The oracle's forward pass (single thread per pair, pairs over the host's cores) on a
     prefix of the batch sized to about budget_s."""
@@ -236,9 +309,9 @@ def main():
         sys.exit(2)
 
     import torch
-    if args.workload == 3:
+    if args.workload in (3, 4):
         import swalign as sw
-        return run_batch(args, torch, sw)
+        return (run_batch if args.workload == 3 else run_serve)(args, torch, sw)
     if args.copy_probe:
         src = torch.ones(args.copy_probe, dtype=torch.uint8, device="cuda:0")
         dst = torch.empty_like(src)

@@ -43,7 +43,8 @@ enum {
     SWH_ERR_OUT_OF_MEMORY = 5,   /* device allocation failed or budget too small         */
     SWH_ERR_BUDGET = 6,          /* request rejected: exceeds the HBM admission budget   */
     SWH_ERR_INTERNAL = 7,        /* traceback inconsistency (should never happen)        */
-    SWH_ERR_NOT_FOUND = 8        /* unknown job id / nothing to poll                     */
+    SWH_ERR_NOT_FOUND = 8,       /* unknown job id / nothing to poll                     */
+    SWH_PENDING = 9              /* swh_server_poll: the job has not finished (no error)  */
 };
 
 /* Mirrors `scoring` (src/SWutils.h:15-20) field for field. */
@@ -159,6 +160,41 @@ int swh_batch_align(swh_ctx* ctx, const swh_pair* pairs, int64_t npairs, const s
  * pointers on the context's device). */
 int swh_batch_align_device(swh_ctx* ctx, const swh_pair* pairs, int64_t npairs, const swh_scoring* sc,
                            const swh_opts* opts, swh_result* out, int32_t* statuses);
+
+/* ---- request server: continuous batching --------------------------------------------- */
+/* A server owns a context on `device` and a pool of hbm_budget_bytes of device memory
+ * (0 = half the free memory).  Jobs are submitted at any time from any thread; a server
+ * thread admits them while the pool has room for their working memory (sequences, row
+ * maxima, checkpoints, tile hand-off buffers, direction codes) and at most max_in_flight
+ * (0 = 1024) are in flight, then advances all of them together: every kernel launch
+ * covers the current tile anti-diagonal of every job in flight, so jobs that arrive while
+ * others run are admitted between launches and share them.  Results are bit-identical to
+ * swh_align_pair's.  No reference counterpart (the reference aligns one pair per process,
+ * src/main.cu:102-250). */
+typedef struct swh_server swh_server;
+
+typedef struct {
+    int64_t submitted, completed, failed, in_flight, queued;
+    int64_t cells_completed;
+    double gcups;             /* completed cells / (last completion - first submission)      */
+    double p50_ms, p99_ms, max_ms;  /* job latency: submission to result ready            */
+    int64_t hbm_pool_bytes, hbm_in_use_bytes;
+    int32_t loop_status;      /* SWH_OK, or the runtime failure that stopped the server       */
+    int32_t _pad;
+} swh_serve_stats;
+
+int swh_server_create(int device, const swh_scoring* sc, const swh_opts* opts, int64_t hbm_budget_bytes,
+                      int32_t max_in_flight, swh_server** out);
+/* Copies the inputs and queues the job; *job_id identifies it for swh_server_poll. */
+int swh_server_submit(swh_server* s, const uint8_t* seq1, int64_t len1, const uint8_t* seq2,
+                      int64_t len2, int64_t* job_id);
+/* SWH_PENDING if the job is still running and wait == 0; otherwise blocks until it is
+ * done, moves its result to *out (free with swh_result_free) and returns its status
+ * (SWH_ERR_BUDGET if it can never fit the pool).  out->ms_total is its latency. */
+int swh_server_poll(swh_server* s, int64_t job_id, int32_t wait, swh_result* out);
+int swh_server_stats(swh_server* s, swh_serve_stats* st);
+/* Finishes every queued and running job, then releases the server. */
+int swh_server_destroy(swh_server* s);
 
 /* ---- synthetic workloads ------------------------------------------------------------ */
 /* Generates the synthetic pair used by the benchmarks (seeded std::mt19937_64):

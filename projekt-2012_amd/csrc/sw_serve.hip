@@ -1,0 +1,696 @@
+// Request server (swh_server_*): continuous batching of alignment jobs on one device.
+//
+// A server thread runs ticks.  Each tick admits queued jobs while the HBM pool has room
+// for their working memory, then advances every job in flight by one step of its own
+// schedule in shared launches: one sweep launch per mode (stage 1, end search, traceback
+// blocks) covering the current tile anti-diagonal of every job in that mode -- jobs
+// admitted at different times sit at different diagonals of the same launch -- plus one
+// argmax launch for jobs whose stage 1 just ended and one walk launch for jobs whose
+// traceback blocks are ready.  One synchronisation per tick reads back the few bytes that
+// decide each job's next step.  A job's work therefore overlaps with every other job in
+// flight, and jobs join and leave between ticks.  No kernel waits on another workgroup.
+//
+// Each job's device memory (sequences, row maxima, checkpoints, tile hand-off buffers,
+// direction codes, ops) is one block of a pool allocated once at creation; admission is
+// bounded by the pool (the HBM budget).
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "sw_host.h"
+
+using namespace swk;
+using namespace swh_host;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// First-fit allocator over [0, size) of the device pool (host bookkeeping only).
+struct Pool {
+    int64_t size = 0;
+    std::map<int64_t, int64_t> free_;  // offset -> length
+    void init(int64_t n) {
+        size = n;
+        free_.clear();
+        free_[0] = n;
+    }
+    int64_t alloc(int64_t n) {
+        n = ceil_div(n, 256) * 256;
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= n) {
+                const int64_t off = it->first, rest = it->second - n;
+                free_.erase(it);
+                if (rest) free_[off + n] = rest;
+                return off;
+            }
+        }
+        return -1;
+    }
+    void release(int64_t off, int64_t n) {
+        n = ceil_div(n, 256) * 256;
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    int64_t in_use() const {
+        int64_t f = 0;
+        for (auto& kv : free_) f += kv.second;
+        return size - f;
+    }
+};
+
+enum Phase { kQueued, kSweep, kArgmax, kWalk, kDone };
+
+struct SJob {
+    int64_t id = 0;
+    std::vector<uint8_t> a, b;  // host copies, oriented (a = rows)
+    bool swapped = false;
+    int64_t m = 0, n = 0;
+    Plan pl{};
+    Clock::time_point t_submit, t_done;
+    // device block
+    int64_t off = -1, bytes = 0;
+    uint8_t* dA = nullptr;
+    uint8_t* dB = nullptr;
+    int32_t* rowmax = nullptr;
+    int2* srow = nullptr;
+    int2* scol = nullptr;
+    unsigned long long* key = nullptr;
+    int32_t* found = nullptr;
+    WalkState* walk = nullptr;
+    uint8_t* ops = nullptr;
+    XPoint* xps = nullptr;
+    int64_t xcap = 0;
+    uint8_t* work = nullptr;  // per-phase hand-off buffers and direction codes
+    int64_t work_bytes = 0;
+    // schedule
+    Phase phase = kQueued;
+    int mode = kStage1;
+    std::vector<Region> regs;
+    std::vector<int64_t> Sr, Gr;
+    int64_t D = 0, d = 0;
+    int32_t best = 0;
+    int64_t bi = -1, bj = -1;
+    WalkState st{};
+    unsigned long long key_h = 0;
+    int32_t found_h = -1;
+    int status = SWH_OK;
+    bool published = false;  // result visible to swh_server_poll (set under the server mutex)
+    swh_result res{};
+};
+
+}  // namespace
+
+struct swh_server {
+    swh_ctx* ctx = nullptr;
+    swh_scoring sc{};
+    swh_opts opts{};
+    int32_t max_in_flight = 0;
+    uint8_t* pool_base = nullptr;
+    Pool pool;
+    DevBuf regs_m[3], spans_m[3], argjobs, wargs, wregs;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<SJob*> queue;
+    std::map<int64_t, SJob*> jobs;  // every job not yet collected by swh_server_poll
+    std::vector<SJob*> flight;
+    int64_t next_id = 1;
+    bool stop = false;
+    std::thread loop;
+    // statistics
+    int64_t submitted = 0, completed = 0, failed = 0, cells_done = 0;
+    std::vector<double> lat_ms;
+    Clock::time_point t_first{}, t_last{};
+    bool any = false;
+    int loop_status = SWH_OK;
+};
+
+namespace {
+
+// Working bytes of one job: everything it keeps, plus the largest per-phase set.
+void layout(swh_server* s, SJob& j) {
+    const int64_t m = j.m, n = j.n;
+    auto al = [](int64_t x) { return ceil_div(x, 256) * 256; };
+    int64_t persist = al(m) + al(n) + al(4 * m) + al(8 * j.pl.nsr * n) + al(8 * j.pl.nsc * m) + al(8) + al(4) +
+                      al(sizeof(WalkState)) + al(m + n + 16);
+    j.xcap = j.pl.nsr + 2;
+    persist += al((int64_t)sizeof(XPoint) * j.xcap);
+    if (m == 0 || n == 0) {  // nothing to sweep: score 0
+        j.work_bytes = 0;
+        j.bytes = persist;
+        return;
+    }
+    const int64_t S = ceil_div(m, kStripeRows);
+    const int64_t stage1 = al(16 * n) + al(16 * m) + al(8 * S);
+    const int64_t bw = std::min(n, j.pl.Kc), bh = std::min(m, j.pl.K), nq = ceil_div(n, bw);
+    const int64_t search = al(16 * n) + nq * (al(16 * bh) + al(8 * ceil_div(bh, kStripeRows)));
+    const int64_t W = tile_cols(s->ctx, bw, s->opts.tile_cols);
+    const int64_t tstride = ceil_div(W + 63, 64) * 64, G = ceil_div(bw, W);
+    const int64_t dirs = ceil_div(bh, kStripeRows) * G * tstride * 64 * 2;
+    const int64_t tb = al(dirs) + al(16 * bw) + al(16 * bh) + al(8 * ceil_div(bh, kStripeRows));
+    j.work_bytes = std::max({stage1, search, tb});
+    j.bytes = persist + j.work_bytes;
+}
+
+void place(swh_server* s, SJob& j) {
+    uint8_t* p = s->pool_base + j.off;
+    auto take = [&](int64_t bytes) {
+        uint8_t* q = p;
+        p += ceil_div(bytes, 256) * 256;
+        return q;
+    };
+    j.dA = take(j.m);
+    j.dB = take(j.n);
+    j.rowmax = (int32_t*)take(4 * j.m);
+    j.srow = j.pl.nsr ? (int2*)take(8 * j.pl.nsr * j.n) : nullptr;
+    if (!j.pl.nsr) take(0);
+    j.scol = j.pl.nsc ? (int2*)take(8 * j.pl.nsc * j.m) : nullptr;
+    j.key = (unsigned long long*)take(8);
+    j.found = (int32_t*)take(4);
+    j.walk = (WalkState*)take(sizeof(WalkState));
+    j.ops = take(j.m + j.n + 16);
+    j.xps = (XPoint*)take((int64_t)sizeof(XPoint) * j.xcap);
+    j.work = take(j.work_bytes);
+}
+
+// Carves each region's tile hand-off buffers (and direction codes) out of the job's
+// work area and computes the schedule of the phase.
+void start_sweep(SJob& j, int mode, int64_t W, int64_t G_fix, int64_t tstride) {
+    j.phase = kSweep;
+    j.mode = mode;
+    j.d = 0;
+    j.D = 0;
+    j.Sr.assign(j.regs.size(), 0);
+    j.Gr.assign(j.regs.size(), 0);
+    uint8_t* p = j.work;
+    auto take = [&](int64_t bytes) {
+        uint8_t* q = p;
+        p += ceil_div(bytes, 256) * 256;
+        return q;
+    };
+    int64_t G = G_fix;
+    for (size_t q = 0; q < j.regs.size(); ++q) G = std::max(G, ceil_div(j.regs[q].c1 - j.regs[q].c0 + 1, W));
+    if (mode == kDirs) j.regs[0].dirs = (uint16_t*)take(ceil_div(j.regs[0].r1 - j.regs[0].r0 + 1, kStripeRows) * G *
+                                                         tstride * 64 * 2);
+    for (size_t q = 0; q < j.regs.size(); ++q) {
+        Region& r = j.regs[q];
+        const int64_t rows = r.r1 - r.r0 + 1, cols = r.c1 - r.c0 + 1;
+        j.Sr[q] = ceil_div(rows, kStripeRows);
+        j.Gr[q] = ceil_div(cols, W);
+        j.D = std::max(j.D, j.Sr[q] + j.Gr[q] - 1);
+        r.W = W;
+        r.G = G;
+        r.tstride = tstride;
+        r.rb = (int2*)take(16 * cols);
+        r.cb = (int2*)take(16 * rows);
+        r.cor = (int32_t*)take(8 * j.Sr[q]);
+    }
+}
+
+Region base_region(const SJob& j) {
+    Region r{};
+    r.A = j.dA;
+    r.B = j.dB;
+    r.m = j.m;
+    r.n = j.n;
+    r.K = j.pl.K;
+    r.Kc = j.pl.Kc;
+    return r;
+}
+
+void setup_stage1(swh_server* s, SJob& j) {
+    Region r = base_region(j);
+    r.r0 = 0;
+    r.r1 = j.m - 1;
+    r.c0 = 0;
+    r.c1 = j.n - 1;
+    r.rowmax = j.rowmax;
+    r.srow = j.srow;
+    r.scol = j.scol;
+    j.regs.assign(1, r);
+    start_sweep(j, kStage1, tile_cols(s->ctx, j.n, s->opts.tile_cols), 0, 0);
+}
+
+void setup_search(swh_server* s, SJob& j) {
+    const int64_t br0 = (j.bi / j.pl.K) * j.pl.K, bw = std::min(j.pl.Kc, j.n), nq = ceil_div(j.n, bw);
+    j.regs.clear();
+    for (int64_t q = 0; q < nq; ++q) {
+        Region r = base_region(j);
+        r.r0 = br0;
+        r.r1 = j.bi;
+        r.c0 = q * bw;
+        r.c1 = std::min(j.n, r.c0 + bw) - 1;
+        r.top = br0 > 0 ? j.srow + (br0 / j.pl.K - 1) * j.n + r.c0 : nullptr;
+        r.left = r.c0 > 0 ? j.scol + (r.c0 / j.pl.Kc - 1) * j.m : nullptr;
+        r.target_row = j.bi;
+        r.target = j.best;
+        r.found = j.found;
+        j.regs.push_back(r);
+    }
+    start_sweep(j, kSearch, tile_cols(s->ctx, bw, s->opts.tile_cols), 0, 0);
+}
+
+// The block of the walk's current cell, cut at the cell (no lookahead: one block a round).
+void setup_dirs(swh_server* s, SJob& j) {
+    const int64_t bw = std::min(j.pl.Kc, j.n);
+    const int64_t W = tile_cols(s->ctx, bw, s->opts.tile_cols);
+    const int64_t tstride = ceil_div(W + 63, 64) * 64, G = ceil_div(bw, W);
+    Region r = base_region(j);
+    r.r0 = (j.st.i / j.pl.K) * j.pl.K;
+    r.r1 = j.st.i;
+    r.c0 = (j.st.j / j.pl.Kc) * j.pl.Kc;
+    r.c1 = j.st.j;
+    r.top = r.r0 > 0 ? j.srow + (r.r0 / j.pl.K - 1) * j.n + r.c0 : nullptr;
+    r.left = r.c0 > 0 ? j.scol + (r.c0 / j.pl.Kc - 1) * j.m : nullptr;
+    j.regs.assign(1, r);
+    start_sweep(j, kDirs, W, G, tstride);
+}
+
+// Marks a job finished; its pool block is released when the loop publishes it.
+void finish(swh_server*, SJob& j, int status) {
+    j.status = status;
+    j.phase = kDone;
+    j.t_done = Clock::now();
+}
+
+const int32_t kNotFound = INT32_MAX;  // initial value of a search's found column
+
+int tick(swh_server* s) {
+    swh_ctx* c = s->ctx;
+    // ---- sweeps: one launch per mode over every job's current diagonal ---------------
+    std::vector<Region> regs[3];
+    std::vector<TileSpan> spans[3];
+    int64_t ntiles[3] = {0, 0, 0};
+    for (SJob* j : s->flight) {
+        if (j->phase != kSweep) continue;
+        const int mo = j->mode;
+        for (size_t q = 0; q < j->regs.size(); ++q) {
+            const int64_t Sq = j->Sr[q], Gq = j->Gr[q], d = j->d;
+            if (d > Sq + Gq - 2) continue;
+            const int64_t s_lo = std::max<int64_t>(0, d - (Gq - 1)), s_hi = std::min(Sq - 1, d);
+            spans[mo].push_back(TileSpan{ntiles[mo], (int32_t)regs[mo].size(), (int32_t)s_lo, d});
+            ntiles[mo] += s_hi - s_lo + 1;
+            regs[mo].push_back(j->regs[q]);
+        }
+    }
+    SweepParams p{};
+    p.sc = Score{s->sc.match, s->sc.mismatch, s->sc.gap_open, s->sc.gap_extend};
+    for (int mo = 0; mo < 3; ++mo) {
+        if (!ntiles[mo]) continue;
+        HCK(c, ensure(s->regs_m[mo], sizeof(Region) * regs[mo].size()));
+        HCK(c, ensure(s->spans_m[mo], sizeof(TileSpan) * spans[mo].size()));
+        HCK(c, hipMemcpyAsync(s->regs_m[mo].p, regs[mo].data(), sizeof(Region) * regs[mo].size(),
+                              hipMemcpyHostToDevice, c->stream));
+        HCK(c, hipMemcpyAsync(s->spans_m[mo].p, spans[mo].data(), sizeof(TileSpan) * spans[mo].size(),
+                              hipMemcpyHostToDevice, c->stream));
+        p.regions = (const Region*)s->regs_m[mo].p;
+        p.spans = (const TileSpan*)s->spans_m[mo].p;
+        p.nspans = (int32_t)spans[mo].size();
+        p.ntiles = ntiles[mo];
+        HCK(c, launch_sweep(mo, p, c->lds_pad[1], c->stream));
+    }
+    // ---- argmax for jobs whose stage 1 ended last tick ----------------------------------
+    std::vector<ArgmaxJob> aj;
+    int64_t max_m = 1;
+    for (SJob* j : s->flight)
+        if (j->phase == kArgmax) {
+            aj.push_back(ArgmaxJob{j->rowmax, j->m, j->key});
+            max_m = std::max(max_m, j->m);
+        }
+    if (!aj.empty()) {
+        for (SJob* j : s->flight)
+            if (j->phase == kArgmax) HCK(c, hipMemsetAsync(j->key, 0, 8, c->stream));
+        HCK(c, ensure(s->argjobs, sizeof(ArgmaxJob) * aj.size()));
+        HCK(c, hipMemcpyAsync(s->argjobs.p, aj.data(), sizeof(ArgmaxJob) * aj.size(), hipMemcpyHostToDevice,
+                              c->stream));
+        const int bpj = (int)std::min<int64_t>(1024, std::max<int64_t>(1, ceil_div(max_m, 16384)));
+        HCK(c, launch_argmax_rows((const ArgmaxJob*)s->argjobs.p, (int)aj.size(), bpj, c->stream));
+    }
+    // ---- walks for jobs whose block codes are ready ---------------------------------------
+    std::vector<WalkArgs> wa;
+    std::vector<WalkRegion> wr;
+    for (SJob* j : s->flight)
+        if (j->phase == kWalk) {
+            const Region& r = j->regs[0];
+            wr.push_back(WalkRegion{r.r0, r.c0, r.r1 - r.r0 + 1, r.c1 - r.c0 + 1, r.dirs});
+        }
+    if (!wr.empty()) {
+        HCK(c, ensure(s->wregs, sizeof(WalkRegion) * wr.size()));
+        size_t x = 0;
+        for (SJob* j : s->flight) {
+            if (j->phase != kWalk) continue;
+            const Region& r = j->regs[0];
+            WalkArgs a{};
+            a.A = j->dA;
+            a.B = j->dB;
+            a.regs = (const WalkRegion*)s->wregs.p + x++;
+            a.L = 0;
+            a.brow = j->st.i / j->pl.K;
+            a.bcol = j->st.j / j->pl.Kc;
+            a.tstride = r.tstride;
+            a.G = r.G;
+            a.W = r.W;
+            a.K = j->pl.K;
+            a.Kc = j->pl.Kc;
+            a.srow = j->srow;
+            a.n = j->n;
+            a.st = j->walk;
+            a.ops = j->ops;
+            a.xps = j->xps;
+            a.xcap = j->xcap;
+            wa.push_back(a);
+        }
+        HCK(c, ensure(s->wargs, sizeof(WalkArgs) * wa.size()));
+        HCK(c, hipMemcpyAsync(s->wregs.p, wr.data(), sizeof(WalkRegion) * wr.size(), hipMemcpyHostToDevice,
+                              c->stream));
+        HCK(c, hipMemcpyAsync(s->wargs.p, wa.data(), sizeof(WalkArgs) * wa.size(), hipMemcpyHostToDevice,
+                              c->stream));
+        HCK(c, launch_walk((const WalkArgs*)s->wargs.p, (int)wa.size(), c->stream));
+    }
+    // ---- read back what decides each job's next step ------------------------------------
+    for (SJob* j : s->flight) {
+        if (j->phase == kArgmax)
+            HCK(c, hipMemcpyAsync(&j->key_h, j->key, 8, hipMemcpyDeviceToHost, c->stream));
+        else if (j->phase == kSweep && j->mode == kSearch && j->d + 1 >= j->D)
+            HCK(c, hipMemcpyAsync(&j->found_h, j->found, 4, hipMemcpyDeviceToHost, c->stream));
+        else if (j->phase == kWalk)
+            HCK(c, hipMemcpyAsync(&j->st, j->walk, sizeof(WalkState), hipMemcpyDeviceToHost, c->stream));
+    }
+    HCK(c, hipStreamSynchronize(c->stream));
+    // ---- advance ------------------------------------------------------------------------
+    for (SJob* j : s->flight) {
+        switch (j->phase) {
+            case kSweep:
+                if (++j->d < j->D) break;
+                if (j->mode == kStage1) {
+                    j->phase = kArgmax;
+                } else if (j->mode == kSearch) {
+                    if (j->found_h == INT32_MAX || j->found_h < 0) {
+                        finish(s, *j, SWH_ERR_INTERNAL);
+                        break;
+                    }
+                    j->bj = j->found_h;
+                    j->res.end_row = j->bi;
+                    j->res.end_col = j->bj;
+                    if (s->opts.score_only) {
+                        finish(s, *j, SWH_OK);
+                        break;
+                    }
+                    j->st = WalkState{};
+                    j->st.i = j->bi;
+                    j->st.j = j->bj;
+                    j->st.start_i = j->bi;
+                    j->st.start_j = j->bj;
+                    j->st.last_row = j->bi;
+                    HCK(c, hipMemcpyAsync(j->walk, &j->st, sizeof(WalkState), hipMemcpyHostToDevice, c->stream));
+                    setup_dirs(s, *j);
+                } else {
+                    j->phase = kWalk;
+                }
+                break;
+            case kArgmax: {
+                j->best = (int32_t)(j->key_h >> 32);
+                j->bi = j->key_h ? (int64_t)(0xFFFFFFFFu - (uint32_t)(j->key_h & 0xFFFFFFFFu)) : -1;
+                j->res.score = j->best > 0 ? j->best : 0;
+                if (j->best <= 0) {
+                    finish(s, *j, SWH_OK);
+                    break;
+                }
+                HCK(c, hipMemcpyAsync(j->found, &kNotFound, 4, hipMemcpyHostToDevice, c->stream));
+                setup_search(s, *j);
+                break;
+            }
+            case kWalk: {
+                const Region& r = j->regs[0];
+                if (j->st.done) {
+                    std::vector<uint8_t> ops((size_t)j->st.nops + 1);
+                    std::vector<XPoint> xps((size_t)std::max<int64_t>(j->st.nxp, 1));
+                    if (j->st.nops)
+                        HCK(c, hipMemcpy(ops.data(), j->ops, (size_t)j->st.nops, hipMemcpyDeviceToHost));
+                    if (s->opts.want_xpoints && j->st.nxp)
+                        HCK(c, hipMemcpy(xps.data(), j->xps, sizeof(XPoint) * j->st.nxp, hipMemcpyDeviceToHost));
+                    const int rc = fill_alignment(j->res, j->st, ops.data(), xps.data(), j->xcap, j->best,
+                                                  s->opts.want_xpoints != 0);
+                    finish(s, *j, rc);
+                } else if (j->st.i >= r.r0 && j->st.j >= r.c0) {
+                    finish(s, *j, SWH_ERR_INTERNAL);  // no progress
+                } else {
+                    setup_dirs(s, *j);
+                }
+                break;
+            }
+            default:
+                break;
+        }
+    }
+    return SWH_OK;
+}
+
+void serve_loop(swh_server* s) {
+    if (hipSetDevice(s->ctx->device) != hipSuccess) return;
+    while (true) {
+        std::vector<SJob*> done;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            // admission, in arrival order, while the pool and the slot count allow
+            while (!s->queue.empty() && (int64_t)s->flight.size() < s->max_in_flight) {
+                SJob* j = s->queue.front();
+                if (j->bytes > s->pool.size) {
+                    s->queue.pop_front();
+                    finish(s, *j, SWH_ERR_BUDGET);
+                    done.push_back(j);
+                    continue;
+                }
+                const int64_t off = s->pool.alloc(j->bytes);
+                if (off < 0) break;
+                s->queue.pop_front();
+                j->off = off;
+                s->flight.push_back(j);
+            }
+            if (s->flight.empty() && done.empty()) {
+                if (s->stop && s->queue.empty()) return;
+                s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty(); });
+                continue;
+            }
+        }
+        int rc = SWH_OK;
+        for (SJob* j : s->flight) {
+            if (j->phase != kQueued) continue;
+            place(s, *j);
+            if (j->m)
+                rc = rc ? rc : (hipMemcpyAsync(j->dA, j->a.data(), (size_t)j->m, hipMemcpyHostToDevice,
+                                               s->ctx->stream) == hipSuccess ? SWH_OK : SWH_ERR_HIP);
+            if (j->n)
+                rc = rc ? rc : (hipMemcpyAsync(j->dB, j->b.data(), (size_t)j->n, hipMemcpyHostToDevice,
+                                               s->ctx->stream) == hipSuccess ? SWH_OK : SWH_ERR_HIP);
+            j->res.m = j->m;
+            j->res.n = j->n;
+            j->res.swapped = j->swapped ? 1 : 0;
+            j->res.end_row = j->res.end_col = j->res.start_row = j->res.start_col = -1;
+            j->res.precision_used = 32;
+            j->res.checkpoint_rows = j->pl.nsr ? j->pl.K : 0;
+            j->res.checkpoint_cols = j->pl.nsc ? j->pl.Kc : 0;
+            if (j->m == 0 || j->n == 0) finish(s, *j, SWH_OK);
+            else setup_stage1(s, *j);
+        }
+        if (!rc) rc = tick(s);
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (rc) {
+            s->loop_status = rc;  // a runtime failure ends every job in flight
+            for (SJob* j : s->flight)
+                if (j->phase != kDone) finish(s, *j, rc);
+        }
+        std::vector<SJob*> keep;
+        for (SJob* j : s->flight) (j->phase == kDone ? done : keep).push_back(j);
+        s->flight.swap(keep);
+        for (SJob* j : done) {
+            if (j->off >= 0) s->pool.release(j->off, j->bytes);
+            j->off = -1;
+            j->published = true;
+            const double ms = std::chrono::duration<double, std::milli>(j->t_done - j->t_submit).count();
+            j->res.ms_total = ms;
+            if (j->status == SWH_OK) {
+                ++s->completed;
+                s->cells_done += j->m * j->n;
+                s->lat_ms.push_back(ms);
+            } else {
+                ++s->failed;
+            }
+            s->t_last = j->t_done;
+        }
+        if (!done.empty()) s->cv_done.notify_all();
+        if (rc) return;
+    }
+}
+
+double percentile(std::vector<double> v, double q) {
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    const size_t k = (size_t)std::min<double>((double)v.size() - 1, q * (double)(v.size() - 1) + 0.5);
+    return v[k];
+}
+
+}  // namespace
+
+extern "C" {
+
+int swh_server_create(int device, const swh_scoring* sc, const swh_opts* opts, int64_t hbm_budget_bytes,
+                      int32_t max_in_flight, swh_server** out) {
+    if (!out) return SWH_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (!scoring_ok(sc)) return SWH_ERR_SCORING;
+    swh_ctx* c = nullptr;
+    int rc = swh_ctx_create(device, &c);
+    if (rc) return rc;
+    swh_server* s = new swh_server();
+    s->ctx = c;
+    s->sc = *sc;
+    if (opts) s->opts = *opts;
+    s->max_in_flight = max_in_flight > 0 ? max_in_flight : 1024;
+    size_t freeb = 0, totalb = 0;
+    (void)hipMemGetInfo(&freeb, &totalb);
+    int64_t budget = hbm_budget_bytes > 0 ? hbm_budget_bytes : (int64_t)(freeb / 2);
+    budget = std::min<int64_t>(budget, (int64_t)(freeb * 0.9));
+    if (hipMalloc((void**)&s->pool_base, (size_t)budget) != hipSuccess) {
+        (void)hipGetLastError();
+        swh_ctx_destroy(c);
+        delete s;
+        return SWH_ERR_OUT_OF_MEMORY;
+    }
+    s->pool.init(budget);
+    s->loop = std::thread(serve_loop, s);
+    *out = s;
+    return SWH_OK;
+}
+
+int swh_server_submit(swh_server* s, const uint8_t* seq1, int64_t len1, const uint8_t* seq2, int64_t len2,
+                      int64_t* job_id) {
+    if (!s || !job_id || len1 < 0 || len2 < 0 || (len1 && !seq1) || (len2 && !seq2)) return SWH_ERR_INVALID_ARG;
+    SJob* j = new SJob();
+    j->swapped = len1 < len2;  // src/SWutils.cpp:84-88
+    const uint8_t* A = j->swapped ? seq2 : seq1;
+    const uint8_t* B = j->swapped ? seq1 : seq2;
+    j->m = j->swapped ? len2 : len1;
+    j->n = j->swapped ? len1 : len2;
+    j->a.assign(A, A + j->m);
+    j->b.assign(B, B + j->n);
+    // checkpoints from 1024 rows (or the options), doubled until they take at most an
+    // eighth of the pool
+    swh_opts o = s->opts;
+    if (!o.score_only && o.checkpoint_rows <= 0) {
+        int64_t K = 1024;
+        while (plan_mem(j->m, j->n, K, K) > s->pool.size / 8 && K < (INT64_C(1) << 40)) K *= 2;
+        o.checkpoint_rows = K;
+        o.checkpoint_cols = K;
+    }
+    if (choose_plan(j->m, j->n, &o, INT64_MAX / 4, &j->pl) != SWH_OK) {
+        delete j;
+        return SWH_ERR_BUDGET;
+    }
+    layout(s, *j);
+    j->t_submit = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stop || s->loop_status) {
+            const int st = s->loop_status ? s->loop_status : SWH_ERR_INVALID_ARG;
+            delete j;
+            return st;
+        }
+        j->id = s->next_id++;
+        *job_id = j->id;
+        s->jobs[j->id] = j;
+        s->queue.push_back(j);
+        ++s->submitted;
+        if (!s->any) {
+            s->any = true;
+            s->t_first = j->t_submit;
+        }
+    }
+    s->cv_work.notify_one();
+    return SWH_OK;
+}
+
+int swh_server_poll(swh_server* s, int64_t job_id, int32_t wait, swh_result* out) {
+    if (!s || !out) return SWH_ERR_INVALID_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->jobs.find(job_id);
+    if (it == s->jobs.end()) return SWH_ERR_NOT_FOUND;
+    SJob* j = it->second;
+    if (!j->published) {
+        if (!wait) return SWH_PENDING;
+        s->cv_done.wait(lk, [&] { return j->published; });
+    }
+    *out = j->res;
+    const int st = j->status;
+    s->jobs.erase(it);
+    delete j;
+    return st;
+}
+
+int swh_server_stats(swh_server* s, swh_serve_stats* st) {
+    if (!s || !st) return SWH_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    memset(st, 0, sizeof *st);
+    st->submitted = s->submitted;
+    st->completed = s->completed;
+    st->failed = s->failed;
+    st->in_flight = (int64_t)s->flight.size();
+    st->queued = (int64_t)s->queue.size();
+    st->cells_completed = s->cells_done;
+    const double sec = std::chrono::duration<double>(s->t_last - s->t_first).count();
+    st->gcups = sec > 0 ? (double)s->cells_done / sec / 1e9 : 0.0;
+    st->p50_ms = percentile(s->lat_ms, 0.50);
+    st->p99_ms = percentile(s->lat_ms, 0.99);
+    st->max_ms = s->lat_ms.empty() ? 0.0 : *std::max_element(s->lat_ms.begin(), s->lat_ms.end());
+    st->hbm_pool_bytes = s->pool.size;
+    st->hbm_in_use_bytes = s->pool.in_use();
+    st->loop_status = s->loop_status;
+    return SWH_OK;
+}
+
+int swh_server_destroy(swh_server* s) {
+    if (!s) return SWH_ERR_INVALID_ARG;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->loop.joinable()) s->loop.join();
+    for (auto& kv : s->jobs) {
+        swh_result_free(&kv.second->res);
+        delete kv.second;
+    }
+    for (SJob* j : s->queue)
+        if (s->jobs.find(j->id) == s->jobs.end()) delete j;
+    DevBuf* bufs[] = {&s->regs_m[0], &s->regs_m[1], &s->regs_m[2], &s->spans_m[0], &s->spans_m[1],
+                      &s->spans_m[2], &s->argjobs, &s->wargs, &s->wregs};
+    for (DevBuf* b : bufs)
+        if (b->p) (void)hipFree(b->p);
+    if (s->pool_base) (void)hipFree(s->pool_base);
+    swh_ctx_destroy(s->ctx);
+    delete s;
+    return SWH_OK;
+}
+
+}  // extern "C"

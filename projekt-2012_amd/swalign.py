@@ -76,6 +76,13 @@ EXPORTS = [
                                   C.POINTER(Opts), C.POINTER(Result), C.POINTER(C.c_int32)]),
     ("swh_batch_align_device", C.c_int, [C.c_void_p, C.POINTER(Pair), C.c_int64, C.POINTER(Scoring),
                                          C.POINTER(Opts), C.POINTER(Result), C.POINTER(C.c_int32)]),
+    ("swh_server_create", C.c_int, [C.c_int, C.POINTER(Scoring), C.POINTER(Opts), C.c_int64, C.c_int32,
+                                    C.POINTER(C.c_void_p)]),
+    ("swh_server_submit", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
+                                    C.POINTER(C.c_int64)]),
+    ("swh_server_poll", C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.POINTER(Result)]),
+    ("swh_server_stats", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("swh_server_destroy", C.c_int, [C.c_void_p]),
     ("swh_synth_pair", C.c_int, [C.c_uint64, C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_double,
                                  C.c_double, C.c_double, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]),
     ("swh_synth_mutate", C.c_int, [C.c_uint64, C.c_void_p, C.c_int64, C.c_double, C.c_double,
@@ -291,6 +298,64 @@ def synth_mutate(seed, src, p_sub, p_indel, flank_frac):
         lib().swh_free(out)
 
 
+class ServeStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64),
+                ("in_flight", C.c_int64), ("queued", C.c_int64), ("cells_completed", C.c_int64),
+                ("gcups", C.c_double), ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("max_ms", C.c_double),
+                ("hbm_pool_bytes", C.c_int64), ("hbm_in_use_bytes", C.c_int64), ("loop_status", C.c_int32),
+                ("_pad", C.c_int32)]
+
+
+SWH_PENDING = 9
+
+
+class Server:
+    """Continuous-batching request server (swh_server_*)."""
+
+    def __init__(self, sc, device=0, budget=0, max_in_flight=0, **kw):
+        self._h = C.c_void_p()
+        self._sc = sc
+        o = make_opts(**kw)
+        rc = lib().swh_server_create(device, C.byref(sc), C.byref(o), budget, max_in_flight, C.byref(self._h))
+        if rc != SWH_OK:
+            raise SWError(rc, "swh_server_create")
+
+    def submit(self, seq1, seq2):
+        s1, s2 = _bytes(seq1), _bytes(seq2)
+        jid = C.c_int64()
+        rc = lib().swh_server_submit(self._h, s1, len(s1), s2, len(s2), C.byref(jid))
+        if rc != SWH_OK:
+            raise SWError(rc, "swh_server_submit")
+        return jid.value
+
+    def poll(self, job_id, wait=True):
+        """(status, result dict or None); status SWH_PENDING when not finished and wait=False."""
+        r = Result()
+        rc = lib().swh_server_poll(self._h, job_id, 1 if wait else 0, C.byref(r))
+        if rc == SWH_PENDING:
+            return rc, None
+        try:
+            return rc, (_result_dict(r) if rc == SWH_OK else None)
+        finally:
+            lib().swh_result_free(C.byref(r))
+
+    def stats(self):
+        st = ServeStats()
+        lib().swh_server_stats(self._h, C.byref(st))
+        return {k: getattr(st, k) for k, _ in ServeStats._fields_ if not k.startswith("_")}
+
+    def close(self):
+        if self._h:
+            lib().swh_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
 class DeviceBuffer:
     """Device memory holding a copy of `data`, allocated with the HIP runtime the library
     itself uses (for callers and tests that have no other device allocator at hand)."""
@@ -326,6 +391,23 @@ class DeviceBuffer:
 
 
 # Synthetic workloads (seed = 2012 + workload index).
+def workload_serve(njobs, seed=2016):
+    """The served stream's jobs: each side's length drawn 70% U[1, 10] kbp, 25% U[10, 100]
+    kbp, 5% U[100 kbp, 1 Mbp]; the query is the target mutated as in workload_batch.
+    Returns ([(target, query)], scoring)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    alpha = np.frombuffer(b"ACGT", np.uint8)
+    jobs = []
+    for k in range(njobs):
+        u = rng.random()
+        lo, hi = (1_000, 10_000) if u < 0.70 else ((10_000, 100_000) if u < 0.95 else (100_000, 1_000_000))
+        L = int(rng.integers(lo, hi + 1))
+        t = alpha[rng.integers(0, 4, L)].tobytes()
+        jobs.append((t, synth_mutate(seed * 1_000_003 + k, t, 0.10, 0.03, 0.20)))
+    return jobs, scoring(1, -3, -5, -2)
+
+
 def workload_batch(npairs=65536, seed=2015, lmin=100, lmax=20000):
     """The batched workload: target lengths log-uniform in [lmin, lmax] (numpy PCG64 from
     `seed`), uniform ACGT targets; each query is its target mutated at 10% substitutions
