@@ -296,28 +296,49 @@ void finish(swh_server*, SJob& j, int status) {
 
 const int32_t kNotFound = INT32_MAX;  // initial value of a search's found column
 
+// Diagonals each sweeping job advances per tick: the host's per-tick work (uploads, the
+// synchronisation) is shared by that many launches per mode.
+constexpr int kDiagonalsPerTick = 8;
+
 int tick(swh_server* s) {
     swh_ctx* c = s->ctx;
-    // ---- sweeps: one launch per mode over every job's current diagonal ---------------
+    // ---- sweeps: per mode, kDiagonalsPerTick launches, each over every job's next diagonal
     std::vector<Region> regs[3];
     std::vector<TileSpan> spans[3];
-    int64_t ntiles[3] = {0, 0, 0};
-    for (SJob* j : s->flight) {
+    std::vector<int64_t> span_at[3], tiles_at[3];
+    std::vector<size_t> first_reg(s->flight.size(), 0);
+    for (size_t x = 0; x < s->flight.size(); ++x) {
+        SJob* j = s->flight[x];
         if (j->phase != kSweep) continue;
-        const int mo = j->mode;
-        for (size_t q = 0; q < j->regs.size(); ++q) {
-            const int64_t Sq = j->Sr[q], Gq = j->Gr[q], d = j->d;
-            if (d > Sq + Gq - 2) continue;
-            const int64_t s_lo = std::max<int64_t>(0, d - (Gq - 1)), s_hi = std::min(Sq - 1, d);
-            spans[mo].push_back(TileSpan{ntiles[mo], (int32_t)regs[mo].size(), (int32_t)s_lo, d});
-            ntiles[mo] += s_hi - s_lo + 1;
-            regs[mo].push_back(j->regs[q]);
+        first_reg[x] = regs[j->mode].size();
+        for (const Region& r : j->regs) regs[j->mode].push_back(r);
+    }
+    for (int mo = 0; mo < 3; ++mo) {
+        if (regs[mo].empty()) continue;
+        for (int step = 0; step < kDiagonalsPerTick; ++step) {
+            span_at[mo].push_back((int64_t)spans[mo].size());
+            int64_t t = 0;
+            for (size_t x = 0; x < s->flight.size(); ++x) {
+                SJob* j = s->flight[x];
+                if (j->phase != kSweep || j->mode != mo) continue;
+                const int64_t d = j->d + step;
+                if (d >= j->D) continue;
+                for (size_t q = 0; q < j->regs.size(); ++q) {
+                    const int64_t Sq = j->Sr[q], Gq = j->Gr[q];
+                    if (d > Sq + Gq - 2) continue;
+                    const int64_t s_lo = std::max<int64_t>(0, d - (Gq - 1)), s_hi = std::min(Sq - 1, d);
+                    spans[mo].push_back(TileSpan{t, (int32_t)(first_reg[x] + q), (int32_t)s_lo, d});
+                    t += s_hi - s_lo + 1;
+                }
+            }
+            tiles_at[mo].push_back(t);
         }
+        span_at[mo].push_back((int64_t)spans[mo].size());
     }
     SweepParams p{};
     p.sc = Score{s->sc.match, s->sc.mismatch, s->sc.gap_open, s->sc.gap_extend};
     for (int mo = 0; mo < 3; ++mo) {
-        if (!ntiles[mo]) continue;
+        if (regs[mo].empty() || spans[mo].empty()) continue;
         HCK(c, ensure(s->regs_m[mo], sizeof(Region) * regs[mo].size()));
         HCK(c, ensure(s->spans_m[mo], sizeof(TileSpan) * spans[mo].size()));
         HCK(c, hipMemcpyAsync(s->regs_m[mo].p, regs[mo].data(), sizeof(Region) * regs[mo].size(),
@@ -325,10 +346,13 @@ int tick(swh_server* s) {
         HCK(c, hipMemcpyAsync(s->spans_m[mo].p, spans[mo].data(), sizeof(TileSpan) * spans[mo].size(),
                               hipMemcpyHostToDevice, c->stream));
         p.regions = (const Region*)s->regs_m[mo].p;
-        p.spans = (const TileSpan*)s->spans_m[mo].p;
-        p.nspans = (int32_t)spans[mo].size();
-        p.ntiles = ntiles[mo];
-        HCK(c, launch_sweep(mo, p, c->lds_pad[1], c->stream));
+        for (int step = 0; step < kDiagonalsPerTick; ++step) {
+            if (!tiles_at[mo][(size_t)step]) continue;
+            p.spans = (const TileSpan*)s->spans_m[mo].p + span_at[mo][(size_t)step];
+            p.nspans = (int32_t)(span_at[mo][(size_t)step + 1] - span_at[mo][(size_t)step]);
+            p.ntiles = tiles_at[mo][(size_t)step];
+            HCK(c, launch_sweep(mo, p, c->lds_pad[1], c->stream));
+        }
     }
     // ---- argmax for jobs whose stage 1 ended last tick ----------------------------------
     std::vector<ArgmaxJob> aj;
@@ -392,7 +416,7 @@ int tick(swh_server* s) {
     for (SJob* j : s->flight) {
         if (j->phase == kArgmax)
             HCK(c, hipMemcpyAsync(&j->key_h, j->key, 8, hipMemcpyDeviceToHost, c->stream));
-        else if (j->phase == kSweep && j->mode == kSearch && j->d + 1 >= j->D)
+        else if (j->phase == kSweep && j->mode == kSearch && j->d + kDiagonalsPerTick >= j->D)
             HCK(c, hipMemcpyAsync(&j->found_h, j->found, 4, hipMemcpyDeviceToHost, c->stream));
         else if (j->phase == kWalk)
             HCK(c, hipMemcpyAsync(&j->st, j->walk, sizeof(WalkState), hipMemcpyDeviceToHost, c->stream));
@@ -402,7 +426,8 @@ int tick(swh_server* s) {
     for (SJob* j : s->flight) {
         switch (j->phase) {
             case kSweep:
-                if (++j->d < j->D) break;
+                j->d = std::min(j->D, j->d + kDiagonalsPerTick);
+                if (j->d < j->D) break;
                 if (j->mode == kStage1) {
                     j->phase = kArgmax;
                 } else if (j->mode == kSearch) {
