@@ -82,6 +82,18 @@ struct Pool {
 
 enum Phase { kQueued, kSweep, kArgmax, kWalk, kDone };
 
+// Device and host copies of one tick's sweep descriptors.  Ticks that need no read-back
+// are not synchronised, so the host runs up to kTickRing ticks ahead of the device; a
+// slot is reused once the event recorded after its launches has completed.
+struct TickSlot {
+    DevBuf regs[3], spans[3];
+    std::vector<Region> hregs[3];
+    std::vector<TileSpan> hspans[3];
+    hipEvent_t done = nullptr;
+    bool pending = false;
+};
+constexpr int kTickRing = 4;
+
 struct SJob {
     int64_t id = 0;
     std::vector<uint8_t> a, b;  // host copies, oriented (a = rows)
@@ -129,7 +141,9 @@ struct swh_server {
     int32_t max_in_flight = 0;
     uint8_t* pool_base = nullptr;
     Pool pool;
-    DevBuf regs_m[3], spans_m[3], argjobs, wargs, wregs;
+    TickSlot ring[kTickRing];
+    int ring_at = 0;
+    DevBuf argjobs, wargs, wregs;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<SJob*> queue;
@@ -302,9 +316,19 @@ constexpr int kDiagonalsPerTick = 8;
 
 int tick(swh_server* s) {
     swh_ctx* c = s->ctx;
+    TickSlot& slot = s->ring[s->ring_at];
+    s->ring_at = (s->ring_at + 1) % kTickRing;
+    if (slot.pending) {
+        HCK(c, hipEventSynchronize(slot.done));
+        slot.pending = false;
+    }
     // ---- sweeps: per mode, kDiagonalsPerTick launches, each over every job's next diagonal
-    std::vector<Region> regs[3];
-    std::vector<TileSpan> spans[3];
+    std::vector<Region>* regs = slot.hregs;
+    std::vector<TileSpan>* spans = slot.hspans;
+    for (int mo = 0; mo < 3; ++mo) {
+        regs[mo].clear();
+        spans[mo].clear();
+    }
     std::vector<int64_t> span_at[3], tiles_at[3];
     std::vector<size_t> first_reg(s->flight.size(), 0);
     for (size_t x = 0; x < s->flight.size(); ++x) {
@@ -339,21 +363,23 @@ int tick(swh_server* s) {
     p.sc = Score{s->sc.match, s->sc.mismatch, s->sc.gap_open, s->sc.gap_extend};
     for (int mo = 0; mo < 3; ++mo) {
         if (regs[mo].empty() || spans[mo].empty()) continue;
-        HCK(c, ensure(s->regs_m[mo], sizeof(Region) * regs[mo].size()));
-        HCK(c, ensure(s->spans_m[mo], sizeof(TileSpan) * spans[mo].size()));
-        HCK(c, hipMemcpyAsync(s->regs_m[mo].p, regs[mo].data(), sizeof(Region) * regs[mo].size(),
+        HCK(c, ensure(slot.regs[mo], sizeof(Region) * regs[mo].size()));
+        HCK(c, ensure(slot.spans[mo], sizeof(TileSpan) * spans[mo].size()));
+        HCK(c, hipMemcpyAsync(slot.regs[mo].p, regs[mo].data(), sizeof(Region) * regs[mo].size(),
                               hipMemcpyHostToDevice, c->stream));
-        HCK(c, hipMemcpyAsync(s->spans_m[mo].p, spans[mo].data(), sizeof(TileSpan) * spans[mo].size(),
+        HCK(c, hipMemcpyAsync(slot.spans[mo].p, spans[mo].data(), sizeof(TileSpan) * spans[mo].size(),
                               hipMemcpyHostToDevice, c->stream));
-        p.regions = (const Region*)s->regs_m[mo].p;
+        p.regions = (const Region*)slot.regs[mo].p;
         for (int step = 0; step < kDiagonalsPerTick; ++step) {
             if (!tiles_at[mo][(size_t)step]) continue;
-            p.spans = (const TileSpan*)s->spans_m[mo].p + span_at[mo][(size_t)step];
+            p.spans = (const TileSpan*)slot.spans[mo].p + span_at[mo][(size_t)step];
             p.nspans = (int32_t)(span_at[mo][(size_t)step + 1] - span_at[mo][(size_t)step]);
             p.ntiles = tiles_at[mo][(size_t)step];
             HCK(c, launch_sweep(mo, p, c->lds_pad[1], c->stream));
         }
     }
+    HCK(c, hipEventRecord(slot.done, c->stream));
+    slot.pending = true;
     // ---- argmax for jobs whose stage 1 ended last tick ----------------------------------
     std::vector<ArgmaxJob> aj;
     int64_t max_m = 1;
@@ -412,8 +438,14 @@ int tick(swh_server* s) {
                               c->stream));
         HCK(c, launch_walk((const WalkArgs*)s->wargs.p, (int)wa.size(), c->stream));
     }
-    // ---- read back what decides each job's next step ------------------------------------
+    // ---- read back what decides each job's next step (only when some job needs it) -------
+    bool readback = false;
     for (SJob* j : s->flight) {
+        readback = readback || j->phase == kArgmax || j->phase == kWalk ||
+                   (j->phase == kSweep && j->mode == kSearch && j->d + kDiagonalsPerTick >= j->D);
+    }
+    for (SJob* j : s->flight) {
+        if (!readback) break;
         if (j->phase == kArgmax)
             HCK(c, hipMemcpyAsync(&j->key_h, j->key, 8, hipMemcpyDeviceToHost, c->stream));
         else if (j->phase == kSweep && j->mode == kSearch && j->d + kDiagonalsPerTick >= j->D)
@@ -421,7 +453,10 @@ int tick(swh_server* s) {
         else if (j->phase == kWalk)
             HCK(c, hipMemcpyAsync(&j->st, j->walk, sizeof(WalkState), hipMemcpyDeviceToHost, c->stream));
     }
-    HCK(c, hipStreamSynchronize(c->stream));
+    if (readback) {
+        HCK(c, hipStreamSynchronize(c->stream));
+        for (TickSlot& t : s->ring) t.pending = false;
+    }
     // ---- advance ------------------------------------------------------------------------
     for (SJob* j : s->flight) {
         switch (j->phase) {
@@ -604,6 +639,13 @@ int swh_server_create(int device, const swh_scoring* sc, const swh_opts* opts, i
         return SWH_ERR_OUT_OF_MEMORY;
     }
     s->pool.init(budget);
+    for (TickSlot& t : s->ring) {
+        if (hipEventCreateWithFlags(&t.done, hipEventDisableTiming) != hipSuccess) {
+            (void)hipGetLastError();
+            swh_server_destroy(s);
+            return SWH_ERR_HIP;
+        }
+    }
     s->loop = std::thread(serve_loop, s);
     *out = s;
     return SWH_OK;
@@ -620,11 +662,12 @@ int swh_server_submit(swh_server* s, const uint8_t* seq1, int64_t len1, const ui
     j->n = j->swapped ? len1 : len2;
     j->a.assign(A, A + j->m);
     j->b.assign(B, B + j->n);
-    // checkpoints from 1024 rows (or the options), doubled until they take at most an
-    // eighth of the pool
+    // checkpoints every max(m, n)/128 rows (1024 to 8192, or the options: short jobs keep
+    // their traceback blocks small, long ones need fewer rounds), doubled until they take
+    // at most an eighth of the pool
     swh_opts o = s->opts;
     if (!o.score_only && o.checkpoint_rows <= 0) {
-        int64_t K = 1024;
+        int64_t K = std::min<int64_t>(8192, std::max<int64_t>(1024, next_pow2(std::max(j->m, j->n) / 128)));
         while (plan_mem(j->m, j->n, K, K) > s->pool.size / 8 && K < (INT64_C(1) << 40)) K *= 2;
         o.checkpoint_rows = K;
         o.checkpoint_cols = K;
@@ -708,8 +751,15 @@ int swh_server_destroy(swh_server* s) {
     }
     for (SJob* j : s->queue)
         if (s->jobs.find(j->id) == s->jobs.end()) delete j;
-    DevBuf* bufs[] = {&s->regs_m[0], &s->regs_m[1], &s->regs_m[2], &s->spans_m[0], &s->spans_m[1],
-                      &s->spans_m[2], &s->argjobs, &s->wargs, &s->wregs};
+    (void)hipStreamSynchronize(s->ctx->stream);
+    for (TickSlot& t : s->ring) {
+        for (int mo = 0; mo < 3; ++mo) {
+            if (t.regs[mo].p) (void)hipFree(t.regs[mo].p);
+            if (t.spans[mo].p) (void)hipFree(t.spans[mo].p);
+        }
+        if (t.done) (void)hipEventDestroy(t.done);
+    }
+    DevBuf* bufs[] = {&s->argjobs, &s->wargs, &s->wregs};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     if (s->pool_base) (void)hipFree(s->pool_base);
