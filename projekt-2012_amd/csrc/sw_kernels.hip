@@ -386,6 +386,14 @@ struct WalkLds {
     WalkState sst;
 };
 
+// Wave-uniform copies (lane 0's value, in scalar registers).
+__device__ __forceinline__ int uni(int v) { return (int)__builtin_amdgcn_readfirstlane((uint32_t)v); }
+__device__ __forceinline__ int64_t uni64(int64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 // Region of the round holding cell (i, j), or nullptr.
 __device__ __forceinline__ const WalkRegion* walk_region(const WalkArgs& a, int64_t i, int64_t j) {
     if (i < 0 || j < 0) return nullptr;
@@ -405,6 +413,9 @@ __device__ void walk_main(const WalkArgs& a, WalkLds<R>& lds) {
     __syncthreads();
     while (true) {
         WalkState st = sst;
+        st.i = uni64(st.i);  // every lane holds the same state; keep it in scalar registers
+        st.j = uni64(st.j);
+        st.done = uni(st.done);
         if (st.done) break;
         const WalkRegion* rg = walk_region(a, st.i, st.j);
         if (!rg) break;  // outside the round's regions
@@ -412,7 +423,7 @@ __device__ void walk_main(const WalkArgs& a, WalkLds<R>& lds) {
         // tile of the current cell and the window's steps [t0, t1]
         const int64_t il = st.i - rgn.r0, jl = st.j - rgn.c0;
         const int64_t s = il / HR, sg = jl / a.W;
-        const int64_t R0 = rgn.r0 + s * HR, C0 = rgn.c0 + sg * a.W;  // tile origin (absolute)
+        const int64_t R0 = uni64(rgn.r0 + s * HR), C0 = uni64(rgn.c0 + sg * a.W);  // tile origin (absolute)
         const int ri0 = (int)(st.i - R0), cj0 = (int)(st.j - C0);
         const int t1 = cj0 + ri0 / R;
         const int t0 = t1 >= kWalkSteps ? t1 - kWalkSteps + 1 : 0;
@@ -432,70 +443,88 @@ __device__ void walk_main(const WalkArgs& a, WalkLds<R>& lds) {
             lds.b[k] = (cj >= 0 && cj <= t1) ? a.B[C0 + cj] : 0;
         }
         __syncthreads();
-        if (lane == 0) {
-            // Lane 0 walks.  Ops and the crosspoint go to LDS and are flushed by the whole
-            // wave after the window, so the loop issues no global memory operation (a
-            // pending one would make every LDS read wait for it).
+        {
+            // The whole wave walks in lockstep on wave-uniform values: each step's code (and
+            // whether the pair on the diagonal matches) is read once from LDS and broadcast
+            // with readfirstlane, so the loop's control flow is scalar.  Lane 0 alone writes
+            // the ops and the crosspoint, to LDS; they are flushed after the window, so the
+            // loop issues no global memory operation.
             static_assert(64 * R + kWalkSteps + 64 <= kWalkOps, "window ops buffer");
-            int ri = ri0, cj = cj0;  // tile-local row and column
-            int layer = st.layer;
-            int nw = 0, nxw = 0;
-            int64_t last_row = st.last_row;
-            int64_t xrow = ((st.i + 1) / a.K) * a.K - 1;  // checkpoint row at or above the row
+            int ri = uni(ri0), cj = uni(cj0);  // tile-local row and column
+            int layer = uni(st.layer);
+            int nw = 0, nxw = 0, done = 0;
+            int64_t last_row = uni64(st.last_row);
+            int64_t si = uni64(st.start_i), sj = uni64(st.start_j);
+            int64_t xrow = ((uni64(st.i) + 1) / a.K) * a.K - 1;  // checkpoint row at or above the row
+            const int bofs = t0 - 63;
             while (true) {
                 if (ri < 0 || cj < 0) {
-                    if (R0 + ri < 0 || C0 + cj < 0) st.done = 1;  // walked past row or column 0
+                    if (R0 + ri < 0 || C0 + cj < 0) done = 1;  // walked past row or column 0
                     break;
                 }
                 const int l = ri / R, t = cj + l;
                 if (t < t0) break;  // left the window
-                const uint32_t d = (lds.win[(t - t0) * 64 + l] >> (4 * (ri % R))) & 15u;
+                const uint32_t word = lds.win[(t - t0) * 64 + l];
+                const uint32_t eq = lds.a[ri] == lds.b[cj - bofs] ? 16u : 0u;
+                const uint32_t v = __builtin_amdgcn_readfirstlane(((word >> (4 * (ri % R))) & 15u) | eq);
                 const int64_t i = R0 + ri;
                 if (i != last_row) {
                     last_row = i;
                     if (xrow > i) xrow -= a.K;
                     if (i == xrow && nxw == 0) {
-                        lds.xp.row = i;
-                        lds.xp.col = C0 + cj;
-                        lds.xp.layer = layer;
+                        if (lane == 0) {
+                            lds.xp.row = i;
+                            lds.xp.col = C0 + cj;
+                            lds.xp.layer = layer;
+                        }
                         nxw = 1;
                     }
                 }
+                uint32_t op;
                 if (layer == 0) {
-                    const uint32_t h = d & 3u;
+                    const uint32_t h = v & 3u;
                     if (h == 1u) {
-                        lds.ops[nw++] = (lds.a[ri] == lds.b[cj - (t0 - 63)]) ? (uint8_t)'=' : (uint8_t)'X';
-                        st.start_i = i;
-                        st.start_j = C0 + cj;
+                        op = (v & 16u) ? (uint32_t)'=' : (uint32_t)'X';
+                        si = i;
+                        sj = C0 + cj;
                         --ri;
                         --cj;
                     } else if (h == 2u) {
                         layer = 2;
+                        continue;
                     } else if (h == 3u) {
                         layer = 1;
+                        continue;
                     } else {
-                        st.done = 1;
+                        done = 1;
                         break;
                     }
                 } else if (layer == 2) {
-                    lds.ops[nw++] = (uint8_t)'D';
-                    layer = (d >> 3) & 1u ? 2 : 0;
+                    op = 'D';
+                    layer = (v >> 3) & 1u ? 2 : 0;
                     --ri;
                 } else {
-                    lds.ops[nw++] = (uint8_t)'I';
-                    layer = (d >> 2) & 1u ? 1 : 0;
+                    op = 'I';
+                    layer = (v >> 2) & 1u ? 1 : 0;
                     --cj;
                 }
+                if (lane == 0) lds.ops[nw] = (uint8_t)op;
+                ++nw;
             }
-            lds.nw = nw;
-            lds.nxw = nxw;
-            lds.ops_at = st.nops;
-            st.i = R0 + ri;
-            st.j = C0 + cj;
-            st.layer = layer;
-            st.nops += nw;
-            st.last_row = last_row;
-            sst = st;
+            if (lane == 0) {
+                lds.nw = nw;
+                lds.nxw = nxw;
+                lds.ops_at = st.nops;
+                st.i = R0 + ri;
+                st.j = C0 + cj;
+                st.layer = layer;
+                st.nops += nw;
+                st.last_row = last_row;
+                st.start_i = si;
+                st.start_j = sj;
+                st.done = done;
+                sst = st;
+            }
         }
         __syncthreads();
         {
