@@ -107,7 +107,7 @@ int choose_wps(swh_ctx* c, int64_t rows, int64_t cols) {
 // widest region's, or G_fix when given: kDirs lays codes out by it).  Optional events are
 // recorded immediately before the first and after the last launch.
 int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions, int64_t W, int64_t G_fix,
-              hipEvent_t ev_before, hipEvent_t ev_after, int wps, int64_t* launches, int64_t tstride) {
+              hipEvent_t ev_before, hipEvent_t ev_after, int wps, int64_t* launches, int64_t tstride, bool packed) {
     const int64_t nr = (int64_t)regions.size();
     std::vector<int64_t> Sr((size_t)nr), Gr((size_t)nr);
     int64_t G = 0, D = 0, cb_elems = 0, rb_elems = 0, cor_elems = 0;
@@ -177,13 +177,20 @@ int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions,
         p.spans = (const TileSpan*)c->spans.p + span_at[(size_t)d];
         p.nspans = (int32_t)(span_at[(size_t)d + 1] - span_at[(size_t)d]);
         p.ntiles = tiles_at[(size_t)d];
-        HCK(c, launch_sweep(mode, p, c->lds_pad[wps == 2 ? 2 : 1], c->stream));
+        if (packed && mode == kStage1) HCK(c, launch_sweep_pk(p, c->lds_pad[wps == 2 ? 2 : 1], c->stream));
+        else HCK(c, launch_sweep(mode, p, c->lds_pad[wps == 2 ? 2 : 1], c->stream));
     }
     if (ev_after) HCK(c, hipEventRecord(ev_after, c->stream));
     if (launches) *launches = D;
     // the host vectors above fed asynchronous copies: keep them alive until those finish
     HCK(c, hipStreamSynchronize(c->stream));
     return SWH_OK;
+}
+
+bool fits_int16(int64_t m, int64_t n, const swh_scoring* sc) {
+    const int64_t M = sc->match, X = -(int64_t)sc->mismatch, O = -(int64_t)sc->gap_open, E = -(int64_t)sc->gap_extend;
+    if (M > 1000 || X > 1000 || O > 1000 || E > 1000) return false;
+    return M * std::min(m, n) + M + O + X + E <= 32000;
 }
 
 int64_t plan_mem(int64_t m, int64_t n, int64_t K, int64_t Kc) {
@@ -241,6 +248,7 @@ struct Job {
 
 struct EngineOpts {
     int64_t tile_cols = 0;   // 0 = auto
+    int precision = 0;       // 0 = packed 16-bit stage 1 where it cannot overflow, 32 = int32 only
     int lookahead = 3;       // traceback blocks ahead per direction (0 = none)
     bool want_xp = false;
     bool batch = false;      // many pairs: fixed tile width, two waves per SIMD
@@ -318,18 +326,22 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
         live.push_back(k);
     }
     if (live.empty()) return SWH_OK;
+    // packed 16-bit stage 1 when no pair of the sweep can leave int16 (two tiles per wave)
+    bool packed = eo.precision != 32;
+    for (int64_t k : live) packed = packed && fits_int16(jobs[k].m, jobs[k].n, sc);
     int wps = 1;
     int64_t W = 0;
     if (eo.batch) {
         wps = c->wps_fixed ? c->wps_fixed : (c->lds_pad[2] ? 2 : 1);
         W = eo.tile_cols > 0 ? eo.tile_cols : kBatchTileCols;
     } else {
-        wps = eo.tile_cols > 0 ? 1 : choose_wps(c, max_m, max_n);
-        W = tile_cols(c, max_n, eo.tile_cols, wps);
+        wps = eo.tile_cols > 0 || packed ? 1 : choose_wps(c, max_m, max_n);
+        W = tile_cols(c, max_n, eo.tile_cols, packed ? 2 : wps);
     }
     int64_t launches = 0;
-    int rc = run_sweep(c, kStage1, p, regions, W, 0, c->ev[0], c->ev[1], wps, &launches);
+    int rc = run_sweep(c, kStage1, p, regions, W, 0, c->ev[0], c->ev[1], wps, &launches, 0, packed);
     if (rc) return rc;
+    for (int64_t k : live) out[k].precision_used = packed ? 16 : 32;
 
     // ---- argmax of the row maxima, per pair ---------------------------------------
     std::vector<ArgmaxJob> aj;  // source of an asynchronous copy: lives until the sync below
@@ -654,6 +666,7 @@ int fill_alignment(swh_result& r, const WalkState& st, const uint8_t* ops, const
 EngineOpts engine_opts(const swh_opts* o, bool batch) {
     EngineOpts eo;
     eo.tile_cols = o ? o->tile_cols : 0;
+    eo.precision = o ? o->precision : 0;
     eo.want_xp = o && o->want_xpoints;
     eo.batch = batch;
     eo.lookahead = batch ? 0 : 3;

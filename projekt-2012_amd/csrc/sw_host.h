@@ -54,7 +54,10 @@ int choose_wps(swh_ctx* c, int64_t rows, int64_t cols);
 // One sweep over `regions`, one launch per tile anti-diagonal (see sw_api.hip).
 int run_sweep(swh_ctx* c, int mode, swk::SweepParams p, std::vector<swk::Region>& regions, int64_t W,
               int64_t G_fix = 0, hipEvent_t ev_before = nullptr, hipEvent_t ev_after = nullptr, int wps = 1,
-              int64_t* launches = nullptr, int64_t tstride = 0);
+              int64_t* launches = nullptr, int64_t tstride = 0, bool packed = false);
+
+// True if a pair of m x n under scoring sc cannot leave int16 in the packed stage 1.
+bool fits_int16(int64_t m, int64_t n, const swh_scoring* sc);
 
 struct Plan {
     int64_t K, Kc, nsr, nsc;

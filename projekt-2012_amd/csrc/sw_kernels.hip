@@ -274,12 +274,25 @@ __device__ void sweep_tile(const SweepParams& p, const TileCtx& T, const int2* l
 
 // One wave = one tile (s, g) with s + g = d of its span; tile index x = wave number in the grid,
 // found in the launch's spans by binary search.  Waves of a workgroup are independent.
+// Everything a wave needs to sweep tile x of the launch (see sweep_main).
+struct TileSetup {
+    TileCtx T;
+    const int2* left;
+    int64_t left_r0;
+    int corner;
+    int2* right;
+    int64_t right_r0;
+    bool first_segment;
+    int32_t* found;
+};
+
+// Locates tile x of the launch (binary search over the launch's spans) and its inputs and
+// outputs; false if the tile does not exist.  Hands the next segment of the stripe its
+// corner (the top boundary at this tile's last column).
 template <int R, int MODE>
-__device__ void sweep_main(const SweepParams& p, SweepLds& lds) {
+__device__ bool make_tile(const SweepParams& p, int64_t x, TileSetup& ts) {
     constexpr int64_t HR = 64 * R;
-    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wave
-    const int64_t x = (int64_t)blockIdx.x * kSweepWaves + wid;
-    if (x >= p.ntiles) return;
+    if (x >= p.ntiles) return false;
     int lo = 0, hi = p.nspans - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -293,8 +306,8 @@ __device__ void sweep_main(const SweepParams& p, SweepLds& lds) {
     const int64_t rows = Rg.r1 - Rg.r0 + 1, cols = Rg.c1 - Rg.c0 + 1;
     const int64_t W = Rg.W;
     const int64_t nstr = (rows + HR - 1) / HR, nseg = (cols + W - 1) / W;
-    if (s >= nstr || g >= nseg) return;  // a narrower region has fewer segments
-    TileCtx T;
+    if (s >= nstr || g >= nseg) return false;  // a narrower region has fewer segments
+    TileCtx& T = ts.T;
     T.A = Rg.A;
     T.B = Rg.B;
     T.m = Rg.m;
@@ -317,23 +330,275 @@ __device__ void sweep_main(const SweepParams& p, SweepLds& lds) {
     }
     T.dirs = MODE == kDirs ? Rg.dirs + (s * Rg.G + g) * Rg.tstride * 64 : nullptr;
     // Left column and corner H(R0-1, C0-1).
-    const int2* left;
-    int64_t left_r0;
-    int corner;
     if (g == 0) {
-        left = Rg.left;
-        left_r0 = 0;
-        corner = (Rg.left && T.R0 >= 1) ? Rg.left[T.R0 - 1].x : 0;
+        ts.left = Rg.left;
+        ts.left_r0 = 0;
+        ts.corner = (Rg.left && T.R0 >= 1) ? Rg.left[T.R0 - 1].x : 0;
     } else {
-        left = Rg.cb + (g & 1) * rows;
-        left_r0 = Rg.r0;
-        if (s == 0) corner = Rg.top ? Rg.top[coff - 1].x : 0;
-        else corner = Rg.cor[2 * s + (g & 1)];
+        ts.left = Rg.cb + (g & 1) * rows;
+        ts.left_r0 = Rg.r0;
+        if (s == 0) ts.corner = Rg.top ? Rg.top[coff - 1].x : 0;
+        else ts.corner = Rg.cor[2 * s + (g & 1)];
     }
-    // Hand the next segment its corner: the top boundary at this tile's last column.
     if (s > 0 && g + 1 < nseg && wave_lane() == 0) Rg.cor[2 * s + ((g + 1) & 1)] = T.top[T.w - 1].x;
-    int2* right = g + 1 < nseg ? Rg.cb + ((g + 1) & 1) * rows : nullptr;
-    sweep_tile<R, MODE>(p, T, left, left_r0, corner, right, Rg.r0, g == 0, Rg.found, lds);
+    ts.right = g + 1 < nseg ? Rg.cb + ((g + 1) & 1) * rows : nullptr;
+    ts.right_r0 = Rg.r0;
+    ts.first_segment = g == 0;
+    ts.found = Rg.found;
+    return true;
+}
+
+// One wave = one tile (s, g) with s + g = d of its span; tile index x = wave number in the
+// grid.  Waves of a workgroup are independent.
+template <int R, int MODE>
+__device__ void sweep_main(const SweepParams& p, SweepLds& lds) {
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wave
+    const int64_t x = (int64_t)blockIdx.x * kSweepWaves + wid;
+    TileSetup ts;
+    if (!make_tile<R, MODE>(p, x, ts)) return;
+    sweep_tile<R, MODE>(p, ts.T, ts.left, ts.left_r0, ts.corner, ts.right, ts.right_r0, ts.first_segment, ts.found,
+                        lds);
+}
+
+// ---- packed 16-bit stage 1: two tiles per wave --------------------------------------
+// The low 16 bits of every value belong to tile 0 of the wave and the high 16 bits to tile
+// 1 (any two tiles of the launch: tiles of one anti-diagonal are independent), so one
+// packed operation (v_pk_*) serves two cells.  Used only when no value can leave int16:
+// every H is at most match * min(m, n), every E, F at least gap_open, and the host checks
+// that these bounds, plus the scoring values, fit.  Inputs and outputs in memory stay int32.
+typedef short s16x2 __attribute__((vector_size(4)));
+
+__device__ __forceinline__ int pk_bits(s16x2 v) { return __builtin_bit_cast(int, v); }
+__device__ __forceinline__ s16x2 pk_from(int v) { return __builtin_bit_cast(s16x2, v); }
+__device__ __forceinline__ s16x2 pk2(int lo, int hi) {
+    s16x2 v;
+    v[0] = (short)lo;
+    v[1] = (short)hi;
+    return v;
+}
+__device__ __forceinline__ s16x2 pk_max(s16x2 a, s16x2 b) { return __builtin_elementwise_max(a, b); }
+__device__ __forceinline__ s16x2 pk_min(s16x2 a, s16x2 b) { return __builtin_elementwise_min(a, b); }
+// Per-half select: halves whose mask bits are set take `a`, the others keep `b`.
+__device__ __forceinline__ s16x2 pk_sel(int mask, s16x2 a, s16x2 b) {
+    return pk_from((pk_bits(a) & mask) | (pk_bits(b) & ~mask));
+}
+
+// 64 steps of two tiles (see chunk_steps).  EDGE: per-half validity masks.
+template <int R, bool EDGE, bool DUMP>
+__device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileCtx (&T)[2], int64_t c,
+                                               const int64_t (&row0)[2], const int (&a)[R], s16x2 (&Ho)[R],
+                                               s16x2 (&E)[R], s16x2 (&rmax)[R], s16x2& Ho_up_prev, s16x2& Ho_bot,
+                                               s16x2& Fe_bot, int& Qh, int& Qf, int& Qb, int& bl, int2* s_out) {
+    const int lane = wave_lane();
+    const s16x2 vO = pk2(p.sc.O, p.sc.O), vE = pk2(p.sc.E, p.sc.E), zero = pk2(0, 0), one = pk2(1, 1);
+    const s16x2 vMO = pk2(p.sc.M - p.sc.O, p.sc.M - p.sc.O);
+    const s16x2 vD = pk2(p.sc.M - p.sc.X, p.sc.M - p.sc.X);  // match - mismatch
+    SWK_UNROLL(UNROLL_STEPS)
+    for (int s = 0; s < 64; ++s) {
+        s16x2 Ho_up = pk_from(shr1(Qh, pk_bits(Ho_bot)));
+        s16x2 Fe_up = pk_from(shr1(Qf, pk_bits(Fe_bot)));
+        bl = shr1(Qb, bl);
+        Qh = shl1(Qh);
+        Qf = shl1(Qf);
+        Qb = shl1(Qb);
+        int vmask = -1;
+        if (EDGE) {
+            const int64_t col = 64 * c + s - lane;
+            vmask = ((col >= 0 && col < T[0].w) ? 0x0000FFFF : 0) | ((col >= 0 && col < T[1].w) ? (int)0xFFFF0000 : 0);
+        }
+        s16x2 diag = Ho_up_prev;
+        Ho_up_prev = EDGE ? pk_sel(vmask, Ho_up, Ho_up_prev) : Ho_up;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            // substitution: the halves of a ^ b are zero exactly where the codes agree
+            const s16x2 ne = pk_min(pk_from(a[k] ^ bl), one);
+            const s16x2 Hd = diag + vMO - ne * vD;  // H(i-1,j-1) + s(i,j)
+            diag = Ho[k];
+            const s16x2 Ee = E[k] + vE;
+            const s16x2 En = pk_max(Ee, Ho[k]);
+            const s16x2 F = pk_max(Fe_up, Ho_up);
+            const s16x2 H = pk_max(pk_max(Hd, zero), pk_max(En, F));
+            const s16x2 Hon = H + vO;
+            if (EDGE) {
+                E[k] = pk_sel(vmask, En, E[k]);
+                Ho[k] = pk_sel(vmask, Hon, Ho[k]);
+                rmax[k] = pk_sel(vmask, pk_max(rmax[k], H), rmax[k]);
+            } else {
+                E[k] = En;
+                Ho[k] = Hon;
+                rmax[k] = pk_max(rmax[k], H);
+            }
+            Ho_up = Hon;
+            Fe_up = F + vE;
+        }
+        Ho_bot = Ho_up;
+        Fe_bot = Fe_up;
+        if (lane == 63) s_out[s] = make_int2(pk_bits(Ho_bot), pk_bits(Fe_bot));
+        if (DUMP) {
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int64_t x = 64 * c + s - lane;
+                const int64_t X = T[t].C0 + x;
+                if (T[t].scol && x >= 0 && x < T[t].w && ((X + 1) & (T[t].Kc - 1)) == 0) {
+                    int2* dst = T[t].scol + ((X + 1) / T[t].Kc - 1) * T[t].m;
+#pragma unroll
+                    for (int k = 0; k < R; ++k)
+                        if (row0[t] + k <= T[t].r1) dst[row0[t] + k] = make_int2(Ho[k][t] - p.sc.O, E[k][t]);
+                }
+            }
+        }
+    }
+}
+
+// Stage 1 over two tiles at once (has[1] false: the second half runs on empty input and
+// stores nothing).
+template <int R>
+__device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], SweepLds& lds) {
+    int2* s_out = lds.out;
+    const int lane = wave_lane();
+    const int O = p.sc.O, Ev = p.sc.E;
+    TileCtx T[2] = {ts[0].T, ts[1].T};
+    const int64_t row0[2] = {T[0].R0 + (int64_t)lane * R, T[1].R0 + (int64_t)lane * R};
+    int a[R];
+    s16x2 Ho[R], E[R], rmax[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        int av[2], hv[2], ev[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int64_t row = row0[t] + k;
+            const bool in = row <= T[t].r1;
+            av[t] = in ? (int)T[t].A[row] : 0x200;
+            if (ts[t].left && in) {
+                const int2 v = ts[t].left[row - ts[t].left_r0];
+                hv[t] = v.x + O;
+                ev[t] = v.y;
+            } else {
+                hv[t] = O;
+                ev[t] = 0;
+            }
+        }
+        a[k] = (av[0] & 0xFFFF) | (av[1] << 16);
+        Ho[k] = pk2(hv[0], hv[1]);
+        E[k] = pk2(ev[0], ev[1]);
+        rmax[k] = pk2(0, 0);
+    }
+    int up[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        up[t] = ts[t].corner + O;
+        if (lane > 0)
+            up[t] = (ts[t].left && row0[t] - 1 <= T[t].r1) ? ts[t].left[row0[t] - 1 - ts[t].left_r0].x + O : O;
+    }
+    s16x2 Ho_up_prev = pk2(up[0], up[1]);
+    s16x2 Ho_bot = pk2(O, O), Fe_bot = pk2(Ev, Ev);
+
+    const int64_t w_max = max(T[0].w, T[1].w);
+    const int64_t nchunks = (w_max + 63 + 63) / 64;
+    const int64_t last_interior = (min(T[0].w, T[1].w) - 64) / 64;
+
+    auto load = [&](int64_t cc, int& qh, int& qf, int& qb) {
+        int h[2], f[2], b[2];
+        const int64_t col = 64 * cc + lane;
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            int H = 0, F = 0;
+            if (col < T[t].w && T[t].top) {
+                const int2 v = T[t].top[col];
+                H = v.x;
+                F = v.y;
+            }
+            h[t] = H + O;
+            f[t] = F + Ev;
+            b[t] = (col < T[t].w) ? (int)T[t].B[T[t].C0 + col] : 0x100;
+        }
+        qh = pk_bits(pk2(h[0], h[1]));
+        qf = pk_bits(pk2(f[0], f[1]));
+        qb = (b[0] & 0xFFFF) | (b[1] << 16);
+    };
+    int Qh, Qf, Qb, nh, nf, nb, bl = 0x01000100;
+    load(0, Qh, Qf, Qb);
+    for (int64_t c = 0; c < nchunks; ++c) {
+        if (c + 1 < nchunks) load(c + 1, nh, nf, nb);
+        const bool interior = c >= 1 && c <= last_interior;
+        bool dump = false;
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            if (T[t].scol && T[t].w > 0) {
+                const int64_t lo = T[t].C0 + max(64 * c - 63, (int64_t)0), hi = T[t].C0 + min(64 * c + 63, T[t].w - 1);
+                const int64_t x = ((lo + T[t].Kc) / T[t].Kc) * T[t].Kc - 1;
+                dump = dump || x <= hi;
+            }
+        }
+        if (!dump) {
+            if (interior)
+                chunk_steps_pk<R, false, false>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot, Qh, Qf, Qb,
+                                                bl, s_out);
+            else
+                chunk_steps_pk<R, true, false>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot, Qh, Qf, Qb,
+                                               bl, s_out);
+        } else {
+            chunk_steps_pk<R, true, true>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot, Qh, Qf, Qb, bl,
+                                          s_out);
+        }
+        __builtin_amdgcn_wave_barrier();
+        {
+            const int2 o = s_out[lane];
+            const int64_t col = 64 * c + lane - 63;
+            const s16x2 oh = pk_from(o.x), of = pk_from(o.y);
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                if (col >= 0 && col < T[t].w) {
+                    const int2 v = make_int2(oh[t] - O, of[t] - Ev);
+                    if (T[t].bot) T[t].bot[col] = v;
+                    if (T[t].srow) T[t].srow[col] = v;
+                }
+            }
+        }
+        Qh = nh;
+        Qf = nf;
+        Qb = nb;
+        __builtin_amdgcn_wave_barrier();
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            const int64_t row = row0[t] + k;
+            if (row > T[t].r1) continue;
+            if (ts[t].right) ts[t].right[row - ts[t].right_r0] = make_int2(Ho[k][t] - O, E[k][t]);
+            const int r = rmax[k][t];
+            T[t].rowmax[row] = ts[t].first_segment ? r : max(T[t].rowmax[row], r);
+        }
+    }
+}
+
+// Packed stage 1: wave x takes tiles 2x and 2x+1 of the launch.
+template <int R>
+__device__ void sweep_main_pk(const SweepParams& p, SweepLds& lds) {
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t x = (int64_t)blockIdx.x * kSweepWaves + wid;
+    TileSetup ts[2];
+    if (!make_tile<R, kStage1>(p, 2 * x, ts[0])) return;
+    if (!make_tile<R, kStage1>(p, 2 * x + 1, ts[1])) {
+        // no second tile: an empty one (no rows, no columns, nothing stored)
+        ts[1] = ts[0];
+        ts[1].T.w = 0;
+        ts[1].T.r1 = ts[1].T.R0 - 1;
+        ts[1].T.top = nullptr;
+        ts[1].T.bot = nullptr;
+        ts[1].T.srow = nullptr;
+        ts[1].T.scol = nullptr;
+        ts[1].left = nullptr;
+        ts[1].right = nullptr;
+    }
+    sweep_tile_pk<R>(p, ts, lds);
+}
+
+template <int R>
+__global__ __launch_bounds__(64 * kSweepWaves) void k_sweep_pk(SweepParams p) {
+    __shared__ SweepLds lds[kSweepWaves];
+    sweep_main_pk<R>(p, lds[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)]);
 }
 
 template <int R, int MODE>
@@ -566,6 +831,13 @@ hipError_t launch_sweep(int mode, const SweepParams& p, size_t lds_pad, hipStrea
     return hipGetLastError();
 }
 
+hipError_t launch_sweep_pk(const SweepParams& p, size_t lds_pad, hipStream_t s) {
+    const int64_t pairs = (p.ntiles + 1) / 2;
+    const dim3 grid((unsigned)((pairs + kSweepWaves - 1) / kSweepWaves)), block(64 * kSweepWaves);
+    hipLaunchKernelGGL((k_sweep_pk<kRowsPerLane>), grid, block, lds_pad, s, p);
+    return hipGetLastError();
+}
+
 size_t sweep_static_lds() { return sizeof(SweepLds) * kSweepWaves; }
 
 hipError_t sweep_allow_lds(size_t dyn_bytes) {
@@ -576,6 +848,8 @@ hipError_t sweep_allow_lds(size_t dyn_bytes) {
         e = hipFuncSetAttribute((const void*)k_sweep<R, kDirs>, hipFuncAttributeMaxDynamicSharedMemorySize, v);
     if (e == hipSuccess)
         e = hipFuncSetAttribute((const void*)k_sweep<R, kSearch>, hipFuncAttributeMaxDynamicSharedMemorySize, v);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)k_sweep_pk<R>, hipFuncAttributeMaxDynamicSharedMemorySize, v);
     return e;
 }
 

@@ -131,3 +131,28 @@ def test_tile_width_shapes(sw, orc, ctx, W, L):
     got = ctx.align(a, b, sc, K=512, Kc=256, xpoints=True, tile_cols=W, lookahead=L)
     _cmp(got, ref)
     assert got["xpoints"] == [tuple(x) for x in ref["xpoints"]]
+
+
+@pytest.mark.parametrize("m,n,W", [(3000, 2000, 0), (1300, 1250, 64), (700, 690, 37), (5000, 300, 0)])
+def test_packed16_equals_int32(sw, orc, ctx, m, n, W):
+    """Stage 1 in packed 16-bit arithmetic (two tiles per wave) gives exactly the int32
+    result: score, end, start, CIGAR, crosspoints."""
+    (a, b), sc = sw.synth_pair(m + n + W, m, n, min(m, n) // 2, 0.1, 0.02), sw.scoring(1, -3, -5, -2)
+    r16 = ctx.align(a, b, sc, K=512, Kc=256, xpoints=True, tile_cols=W)
+    r32 = ctx.align(a, b, sc, K=512, Kc=256, xpoints=True, tile_cols=W, precision=32)
+    assert r16["precision"] == 16 and r32["precision"] == 32
+    for k in ("score", "end", "start", "cigar", "xpoints"):
+        assert r16[k] == r32[k], k
+    A, B = orient(a, b)
+    _cmp(r16, orc.align_linear(A, B, orc.scoring(1, -3, -5, -2), 512, 256))
+
+
+@pytest.mark.parametrize("L,precision", [(31000, 16), (33000, 32)])
+def test_int16_bound_selects_precision(sw, ctx, L, precision):
+    """Identical sequences score L*match: up to the int16 bound the packed path runs and is
+    exact at its largest values; beyond it the int32 path is chosen."""
+    rng = random.Random(L)
+    s = bytes(rng.choice(b"ACGT") for _ in range(L))
+    r = ctx.align(s, s, sw.scoring(1, -3, -5, -2), score_only=True)
+    assert r["precision"] == precision
+    assert r["score"] == L and r["end"] == (L - 1, L - 1)
