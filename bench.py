@@ -28,6 +28,7 @@ LANES_PER_CU_CLK = 64       # 4 SIMD x 16 lanes; one 32-bit VALU op per lane per
 PEAK_CUS = 256
 PEAK_CLOCK_GHZ = 2.4
 PEAK_INT32_TOPS = PEAK_CUS * LANES_PER_CU_CLK * PEAK_CLOCK_GHZ * 1e9 / 1e12   # 39.32
+PEAK_INT16X2_TOPS = 2 * PEAK_INT32_TOPS   # packed 16-bit: two operations per lane per clock
 
 WORKLOADS = {
     0: "10 kbp x 10 kbp pair, 3 kbp homology (10% sub, 2% indel), +1/-3/-5/-2, full traceback",
@@ -85,8 +86,11 @@ def algorithmic_bytes(cells, n_cols_per_tile, K, Kc):
     return cells * per_cell
 
 
-def roofline(cells, fwd_ms, launches, workload=None, W=None, K=None, Kc=None):
+def roofline(cells, fwd_ms, launches, workload=None, W=None, K=None, Kc=None, precision=32):
+    """Stage-1 sweep against the integer-VALU peak of the precision it ran in (int32, or
+    packed int16 pairs at twice the operation rate)."""
     achieved = cells * OPS_PER_CELL / (fwd_ms / 1000.0) / 1e12
+    peak = PEAK_INT16X2_TOPS if precision == 16 else PEAK_INT32_TOPS
     traffic, source = measured_traffic(workload)
     extra = {}
     if W and K and Kc:
@@ -95,11 +99,12 @@ def roofline(cells, fwd_ms, launches, workload=None, W=None, K=None, Kc=None):
         extra["traffic_source"] = source
     return {
         "bound": "valu",
-        "kernel": "k_sweep<4, kStage1> (forward sweep with checkpoints)",
+        "kernel": ("k_sweep_pk<4> (forward sweep, packed 16-bit)" if precision == 16
+                   else "k_sweep<4, kStage1> (forward sweep with checkpoints)"),
         "achieved": round(achieved, 3),
-        "peak": round(PEAK_INT32_TOPS, 2),
+        "peak": round(peak, 2),
         "unit": "TOP/s",
-        "frac": round(achieved / PEAK_INT32_TOPS, 4),
+        "frac": round(achieved / peak, 4),
         "traffic": traffic,
         **extra,
         "launches": launches,
@@ -148,7 +153,8 @@ def run_batch(args, torch, sw):
     line = {
         "metric": METRIC, "value": round(gcups, 2), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(sec * 1000.0, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "int16x2 stage 1, int32 end search and traceback" if r0["precision"] == 16 else "int32",
         "data": "synthetic (numpy PCG64 lengths and targets, seed 2015; queries by swh_synth_mutate)",
         "config": {
             "workload": WORKLOADS[3], "pairs": len(pairs), "cells": cells,
@@ -158,7 +164,7 @@ def run_batch(args, torch, sw):
             "plain_per_pair_gcups": round(plain_gcups, 2), "plain_sample_pairs": nplain,
             "batch_vs_plain": round(gcups / plain_gcups, 2),
         },
-        "roofline": roofline(cells, r0["ms_forward"], r0["forward_launches"]),
+        "roofline": roofline(cells, r0["ms_forward"], r0["forward_launches"], precision=r0["precision"]),
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline_pairs(pairs, sc)
@@ -367,7 +373,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int32",
+        "dtype": "int16x2 stage 1, int32 end search and traceback" if r0["precision"] == 16 else "int32",
         "data": "synthetic (seeded std::mt19937_64, seed %d)" % (2012 + args.workload),
         "config": {
             "workload": WORKLOADS[args.workload],
@@ -383,7 +389,7 @@ def main():
             "forward_gcups": round(cells / (fwd_ms / 1000.0) / 1e9, 2),
             "device": info["name"], "cus": info["cus"], "clock_mhz": round(clock_ghz * 1000),
         },
-        "roofline": roofline(cells, fwd_ms, r0["forward_launches"], args.workload,
+        "roofline": roofline(cells, fwd_ms, r0["forward_launches"], args.workload, precision=r0["precision"],
                              W=-(-n // (4 * int(info["cus"] or 256))), K=r0["checkpoint_rows"] or m,
                              Kc=r0["checkpoint_cols"] or n),
     }
