@@ -127,12 +127,22 @@ def run_batch(args, torch, sw):
         off += len(a) + len(b)
     cells = sum(len(a) * len(b) for a, b in pairs)
     torch.cuda.synchronize()
+    # the autotuner picks tile width, waves per SIMD and checkpoint spacing on an evenly
+    # spaced eighth of the batch (untimed, like the reference's own launch set-up)
+    kw, tuning = {}, None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tuning = ctx.batch_tune(dp, sc, sample=len(dp) // 8)
+        tuning["tune_s"] = round(time.perf_counter() - t0, 2)
+        kw = dict(tile_cols=tuning["tile_cols"], waves_per_simd=tuning["waves_per_simd"],
+                  K=tuning["checkpoint_rows"], Kc=tuning["checkpoint_rows"])
+        print("bench: autotune %s" % tuning, file=sys.stderr, flush=True)
     for k in range(args.warmup):
-        ctx.batch_device(dp, sc)
+        ctx.batch_device(dp, sc, **kw)
         print("bench: warmup step %d done in %.2f s" % (k, ctx.last_call_s), file=sys.stderr, flush=True)
     times, res = [], None
     for k in range(args.steps):
-        res, st = ctx.batch_device(dp, sc)
+        res, st = ctx.batch_device(dp, sc, **kw)
         assert all(x == 0 for x in st)
         times.append(ctx.last_call_s)
         print("bench: step %d done in %.2f s" % (k, ctx.last_call_s), file=sys.stderr, flush=True)
@@ -163,6 +173,7 @@ def run_batch(args, torch, sw):
             "forward_gcups": round(cells / (r0["ms_forward"] / 1000.0) / 1e9, 2),
             "plain_per_pair_gcups": round(plain_gcups, 2), "plain_sample_pairs": nplain,
             "batch_vs_plain": round(gcups / plain_gcups, 2),
+            "autotune": tuning and {k: (round(v, 3) if isinstance(v, float) else v) for k, v in tuning.items()},
         },
         "roofline": roofline(cells, r0["ms_forward"], r0["forward_launches"], precision=r0["precision"]),
     }
@@ -305,6 +316,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", type=int, default=1, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="workload 3: default batch settings")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES device-to-device once (calibrates the FETCH_SIZE / WRITE_SIZE "
                          "counters against a transfer of known size) and exit")

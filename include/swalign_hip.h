@@ -67,7 +67,9 @@ typedef struct {
                                /* 0 = auto (about four tiles per CU across the row, >= 256)   */
     int32_t tb_lookahead;      /* traceback: checkpoint blocks recomputed ahead of the walk in  */
                                /* each direction per round; 0 = auto (3), -1 = none            */
-    int32_t reserved[3];
+    int32_t waves_per_simd;    /* sweep waves resident per SIMD (1 or 2), 0 = auto             */
+    int32_t autotune;          /* batch: 1 = time candidate settings on a sample first          */
+    int32_t reserved[1];
 } swh_opts;
 
 /* Path node on a checkpoint row (the analogue of the reference's crosspoints,
@@ -160,6 +162,20 @@ int swh_batch_align(swh_ctx* ctx, const swh_pair* pairs, int64_t npairs, const s
  * pointers on the context's device). */
 int swh_batch_align_device(swh_ctx* ctx, const swh_pair* pairs, int64_t npairs, const swh_scoring* sc,
                            const swh_opts* opts, swh_result* out, int32_t* statuses);
+
+/* Batch autotuner: times every combination of tile width {256, 384, 512, 768, 1024},
+ * sweep waves per SIMD {1, 2} and checkpoint spacing {512, 1024, 2048} on an evenly spaced
+ * sample of `sample` pairs (device-resident pairs if `device` != 0) and reports the
+ * fastest, to be passed back in swh_opts (tile_cols, waves_per_simd, checkpoint_rows and
+ * checkpoint_cols).  swh_opts.autotune = 1 makes the batch entry points do this first. */
+typedef struct {
+    int32_t tile_cols, waves_per_simd;
+    int64_t checkpoint_rows;
+    double ms_best, ms_default;   /* sample time of the best and of the default settings */
+    int32_t candidates, _pad;
+} swh_batch_tuning;
+int swh_batch_tune(swh_ctx* ctx, const swh_pair* pairs, int64_t npairs, int32_t device, const swh_scoring* sc,
+                   const swh_opts* opts, int64_t sample, swh_batch_tuning* out);
 
 /* ---- request server: continuous batching --------------------------------------------- */
 /* A server owns a context on `device` and a pool of hbm_budget_bytes of device memory

@@ -249,6 +249,7 @@ struct Job {
 struct EngineOpts {
     int64_t tile_cols = 0;   // 0 = auto
     int precision = 0;       // 0 = packed 16-bit stage 1 where it cannot overflow, 32 = int32 only
+    int wps = 0;             // sweep waves per SIMD, 0 = auto
     int lookahead = 3;       // traceback blocks ahead per direction (0 = none)
     bool want_xp = false;
     bool batch = false;      // many pairs: fixed tile width, two waves per SIMD
@@ -332,7 +333,8 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
     int wps = 1;
     int64_t W = 0;
     if (eo.batch) {
-        wps = c->wps_fixed ? c->wps_fixed : (c->lds_pad[2] ? 2 : 1);
+        wps = eo.wps ? eo.wps : (c->wps_fixed ? c->wps_fixed : (c->lds_pad[2] ? 2 : 1));
+        if (wps == 2 && !c->lds_pad[2]) wps = 1;
         W = eo.tile_cols > 0 ? eo.tile_cols : kBatchTileCols;
     } else {
         wps = eo.tile_cols > 0 || packed ? 1 : choose_wps(c, max_m, max_n);
@@ -667,6 +669,7 @@ EngineOpts engine_opts(const swh_opts* o, bool batch) {
     EngineOpts eo;
     eo.tile_cols = o ? o->tile_cols : 0;
     eo.precision = o ? o->precision : 0;
+    eo.wps = o && (o->waves_per_simd == 1 || o->waves_per_simd == 2) ? o->waves_per_simd : 0;
     eo.want_xp = o && o->want_xpoints;
     eo.batch = batch;
     eo.lookahead = batch ? 0 : 3;
@@ -848,12 +851,26 @@ int swh_forward(swh_ctx* c, const uint8_t* A, int64_t m, const uint8_t* B, int64
 
 namespace {
 
+int tune_jobs(swh_ctx* c, const std::vector<Job>& jobs, const swh_scoring* sc, const swh_opts* o, int64_t sample,
+              swh_batch_tuning* out);
+
 // Shared by the batch entry points: jobs over device-resident pairs, one checkpoint
-// spacing for the batch (from 1024 rows, or the options, doubled to fit the budget).
+// spacing for the batch (from 1024 rows, or the options, doubled to fit the budget); with
+// opts->autotune the settings are first chosen by timing candidates on a sample.
 int batch_run(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const swh_opts* o, swh_result* out,
               int32_t* statuses) {
     const int64_t budget = default_budget(o);
     swh_opts ob = o ? *o : swh_opts{};
+    if (ob.autotune) {
+        swh_batch_tuning t{};
+        const int rc = tune_jobs(c, jobs, sc, &ob, 2048, &t);
+        if (rc) return rc;
+        ob.autotune = 0;
+        ob.tile_cols = t.tile_cols;
+        ob.waves_per_simd = t.waves_per_simd;
+        ob.checkpoint_rows = t.checkpoint_rows;
+        ob.checkpoint_cols = t.checkpoint_rows;
+    }
     if (!ob.score_only) {
         int64_t K = ob.checkpoint_rows > 0 ? ceil_div(ob.checkpoint_rows, kStripeRows) * kStripeRows : 1024;
         int64_t Kc = ob.checkpoint_cols > 0 ? next_pow2(ob.checkpoint_cols) : next_pow2(K);
@@ -873,15 +890,121 @@ int batch_run(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const s
         const int rc = choose_plan(j.m, j.n, &ob, INT64_MAX / 4, &j.pl);
         if (rc) return rc;
     }
-    const int rc = run_jobs(c, jobs, sc, engine_opts(o, true), out);
+    const int rc = run_jobs(c, jobs, sc, engine_opts(&ob, true), out);
     if (statuses)
         for (size_t k = 0; k < jobs.size(); ++k) statuses[k] = rc ? rc : jobs[k].status;
     return rc;
 }
 
+// Times each candidate setting on an evenly spaced sample of the jobs (full alignments,
+// results discarded) and keeps the fastest.
+int tune_jobs(swh_ctx* c, const std::vector<Job>& jobs, const swh_scoring* sc, const swh_opts* o, int64_t sample,
+              swh_batch_tuning* out) {
+    const int64_t n = (int64_t)jobs.size();
+    sample = std::max<int64_t>(1, std::min(sample > 0 ? sample : 2048, n));
+    std::vector<Job> base;
+    for (int64_t k = 0; k < sample; ++k) base.push_back(jobs[(size_t)(k * n / sample)]);
+    const int Ws[] = {256, 384, 512, 768, 1024};
+    const int Ps[] = {1, 2};
+    const int64_t Ks[] = {512, 1024, 2048};
+    std::vector<swh_result> res((size_t)sample);
+    double best = 1e300;
+    *out = swh_batch_tuning{};
+    {  // one untimed run first, so that buffer growth is not charged to the first candidate
+        std::vector<Job> js = base;
+        swh_opts ot = o ? *o : swh_opts{};
+        ot.autotune = 0;
+        ot.checkpoint_rows = ot.checkpoint_cols = 2048;
+        for (Job& j : js) {
+            const int rc = choose_plan(j.m, j.n, &ot, INT64_MAX / 4, &j.pl);
+            if (rc) return rc;
+        }
+        for (auto& r : res) result_init(&r);
+        const int rc = run_jobs(c, js, sc, engine_opts(&ot, true), res.data());
+        for (auto& r : res) swh_result_free(&r);
+        if (rc) return rc;
+    }
+    int cand = 0;
+    for (int64_t K : Ks) {
+        for (int P : Ps) {
+            if (P == 2 && !c->lds_pad[2]) continue;
+            for (int W : Ws) {
+                std::vector<Job> js = base;
+                swh_opts ot = o ? *o : swh_opts{};
+                ot.autotune = 0;
+                ot.tile_cols = W;
+                ot.waves_per_simd = P;
+                ot.checkpoint_rows = K;
+                ot.checkpoint_cols = K;
+                for (Job& j : js) {
+                    const int rc = choose_plan(j.m, j.n, &ot, INT64_MAX / 4, &j.pl);
+                    if (rc) return rc;
+                }
+                for (auto& r : res) result_init(&r);
+                HCK(c, hipStreamSynchronize(c->stream));
+                const auto t0 = std::chrono::steady_clock::now();
+                const int rc = run_jobs(c, js, sc, engine_opts(&ot, true), res.data());
+                const double ms =
+                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                for (auto& r : res) swh_result_free(&r);
+                if (rc) return rc;
+                ++cand;
+                if (W == kBatchTileCols && K == 1024 && P == (c->lds_pad[2] ? 2 : 1)) out->ms_default = ms;
+                if (ms < best) {
+                    best = ms;
+                    out->tile_cols = W;
+                    out->waves_per_simd = P;
+                    out->checkpoint_rows = K;
+                    out->ms_best = ms;
+                }
+            }
+        }
+    }
+    out->candidates = cand;
+    return SWH_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int swh_batch_tune(swh_ctx* c, const swh_pair* pairs, int64_t npairs, int32_t device, const swh_scoring* sc,
+                   const swh_opts* o, int64_t sample, swh_batch_tuning* out) {
+    if (!c || !out || npairs <= 0 || !pairs) return SWH_ERR_INVALID_ARG;
+    if (!scoring_ok(sc)) return SWH_ERR_SCORING;
+    HCK(c, hipSetDevice(c->device));
+    const bool tb = !(o && o->score_only);
+    std::vector<Job> jobs;
+    if (device) {
+        for (int64_t k = 0; k < npairs; ++k)
+            jobs.push_back(make_job(pairs[k].seq1, pairs[k].len1, pairs[k].seq2, pairs[k].len2, tb));
+        return tune_jobs(c, jobs, sc, o, sample, out);
+    }
+    // host pairs: the sample is uploaded once
+    const int64_t ns = std::max<int64_t>(1, std::min(sample > 0 ? sample : 2048, npairs));
+    int64_t total = 0;
+    for (int64_t k = 0; k < ns; ++k) {
+        const swh_pair& q = pairs[k * npairs / ns];
+        if (q.len1 < 0 || q.len2 < 0 || (q.len1 && !q.seq1) || (q.len2 && !q.seq2)) return SWH_ERR_INVALID_ARG;
+        total += q.len1 + q.len2;
+    }
+    DevBuf buf;
+    HCK(c, ensure(buf, (size_t)total + 64));
+    std::vector<uint8_t> host((size_t)total + 1);
+    int64_t off = 0;
+    for (int64_t k = 0; k < ns; ++k) {
+        const swh_pair& q = pairs[k * npairs / ns];
+        if (q.len1) memcpy(host.data() + off, q.seq1, (size_t)q.len1);
+        if (q.len2) memcpy(host.data() + off + q.len1, q.seq2, (size_t)q.len2);
+        const uint8_t* d = (const uint8_t*)buf.p + off;
+        jobs.push_back(make_job(d, q.len1, d + q.len1, q.len2, tb));
+        off += q.len1 + q.len2;
+    }
+    int rc = hipMemcpy(buf.p, host.data(), (size_t)total, hipMemcpyHostToDevice) == hipSuccess ? SWH_OK : SWH_ERR_HIP;
+    if (!rc) rc = tune_jobs(c, jobs, sc, o, ns, out);
+    (void)hipFree(buf.p);
+    return rc;
+}
 
 int swh_batch_align_device(swh_ctx* c, const swh_pair* pairs, int64_t npairs, const swh_scoring* sc,
                            const swh_opts* o, swh_result* out, int32_t* statuses) {

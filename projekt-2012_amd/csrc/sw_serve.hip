@@ -122,6 +122,9 @@ struct SJob {
     std::vector<Region> regs;
     std::vector<int64_t> Sr, Gr;
     int64_t D = 0, d = 0;
+    int L = 0;                      // traceback: checkpoint blocks recomputed ahead per direction
+    std::vector<WalkRegion> wtab;   // the round's (L+1)^2 blocks, rows == 0 where none
+    std::vector<int> wtab_reg;      // region of each block (-1 where none)
     int32_t best = 0;
     int64_t bi = -1, bj = -1;
     WalkState st{};
@@ -182,7 +185,11 @@ void layout(swh_server* s, SJob& j) {
     const int64_t W = tile_cols(s->ctx, bw, s->opts.tile_cols);
     const int64_t tstride = ceil_div(W + 63, 64) * 64, G = ceil_div(bw, W);
     const int64_t dirs = ceil_div(bh, kStripeRows) * G * tstride * 64 * 2;
-    const int64_t tb = al(dirs) + al(16 * bw) + al(16 * bh) + al(8 * ceil_div(bh, kStripeRows));
+    const int64_t blk = al(dirs) + al(16 * bw) + al(16 * bh) + al(8 * ceil_div(bh, kStripeRows));
+    // lookahead: up to 3 blocks in each direction when the job has several, within 1 GiB
+    j.L = (j.pl.nsr > 0 || j.pl.nsc > 0) ? 3 : 0;
+    while (j.L > 0 && (j.L + 1) * (j.L + 1) * blk > (INT64_C(1) << 30)) --j.L;
+    const int64_t tb = (j.L + 1) * (j.L + 1) * blk;
     j.work_bytes = std::max({stage1, search, tb});
     j.bytes = persist + j.work_bytes;
 }
@@ -225,8 +232,8 @@ void start_sweep(SJob& j, int mode, int64_t W, int64_t G_fix, int64_t tstride) {
     };
     int64_t G = G_fix;
     for (size_t q = 0; q < j.regs.size(); ++q) G = std::max(G, ceil_div(j.regs[q].c1 - j.regs[q].c0 + 1, W));
-    if (mode == kDirs) j.regs[0].dirs = (uint16_t*)take(ceil_div(j.regs[0].r1 - j.regs[0].r0 + 1, kStripeRows) * G *
-                                                         tstride * 64 * 2);
+    if (mode == kDirs)
+        for (Region& r : j.regs) r.dirs = (uint16_t*)take(ceil_div(r.r1 - r.r0 + 1, kStripeRows) * G * tstride * 64 * 2);
     for (size_t q = 0; q < j.regs.size(); ++q) {
         Region& r = j.regs[q];
         const int64_t rows = r.r1 - r.r0 + 1, cols = r.c1 - r.c0 + 1;
@@ -285,20 +292,36 @@ void setup_search(swh_server* s, SJob& j) {
     start_sweep(j, kSearch, tile_cols(s->ctx, bw, s->opts.tile_cols), 0, 0);
 }
 
-// The block of the walk's current cell, cut at the cell (no lookahead: one block a round).
+// A traceback round: the block of the walk's current cell (cut at the cell, region 0) and
+// the full blocks up to L above and to the left, each from its own checkpoints.
 void setup_dirs(swh_server* s, SJob& j) {
     const int64_t bw = std::min(j.pl.Kc, j.n);
     const int64_t W = tile_cols(s->ctx, bw, s->opts.tile_cols);
     const int64_t tstride = ceil_div(W + 63, 64) * 64, G = ceil_div(bw, W);
-    Region r = base_region(j);
-    r.r0 = (j.st.i / j.pl.K) * j.pl.K;
-    r.r1 = j.st.i;
-    r.c0 = (j.st.j / j.pl.Kc) * j.pl.Kc;
-    r.c1 = j.st.j;
-    r.top = r.r0 > 0 ? j.srow + (r.r0 / j.pl.K - 1) * j.n + r.c0 : nullptr;
-    r.left = r.c0 > 0 ? j.scol + (r.c0 / j.pl.Kc - 1) * j.m : nullptr;
-    j.regs.assign(1, r);
+    const int64_t brow = j.st.i / j.pl.K, bcol = j.st.j / j.pl.Kc, L = j.L;
+    j.regs.clear();
+    j.wtab.assign((size_t)((L + 1) * (L + 1)), WalkRegion{});
+    j.wtab_reg.assign(j.wtab.size(), -1);
+    for (int64_t da = 0; da <= L; ++da) {
+        for (int64_t db = 0; db <= L; ++db) {
+            if (brow - da < 0 || bcol - db < 0) continue;
+            Region r = base_region(j);
+            r.r0 = (brow - da) * j.pl.K;
+            r.r1 = da == 0 ? j.st.i : std::min(j.m, r.r0 + j.pl.K) - 1;
+            r.c0 = (bcol - db) * j.pl.Kc;
+            r.c1 = db == 0 ? j.st.j : std::min(j.n, r.c0 + j.pl.Kc) - 1;
+            r.top = r.r0 > 0 ? j.srow + (r.r0 / j.pl.K - 1) * j.n + r.c0 : nullptr;
+            r.left = r.c0 > 0 ? j.scol + (r.c0 / j.pl.Kc - 1) * j.m : nullptr;
+            j.wtab_reg[(size_t)(da * (L + 1) + db)] = (int)j.regs.size();
+            j.regs.push_back(r);
+        }
+    }
     start_sweep(j, kDirs, W, G, tstride);
+    for (size_t k = 0; k < j.wtab.size(); ++k) {
+        if (j.wtab_reg[k] < 0) continue;
+        const Region& r = j.regs[(size_t)j.wtab_reg[k]];
+        j.wtab[k] = WalkRegion{r.r0, r.c0, r.r1 - r.r0 + 1, r.c1 - r.c0 + 1, r.dirs};
+    }
 }
 
 // Marks a job finished; its pool block is released when the loop publishes it.
@@ -401,10 +424,7 @@ int tick(swh_server* s) {
     std::vector<WalkArgs> wa;
     std::vector<WalkRegion> wr;
     for (SJob* j : s->flight)
-        if (j->phase == kWalk) {
-            const Region& r = j->regs[0];
-            wr.push_back(WalkRegion{r.r0, r.c0, r.r1 - r.r0 + 1, r.c1 - r.c0 + 1, r.dirs});
-        }
+        if (j->phase == kWalk) wr.insert(wr.end(), j->wtab.begin(), j->wtab.end());
     if (!wr.empty()) {
         HCK(c, ensure(s->wregs, sizeof(WalkRegion) * wr.size()));
         size_t x = 0;
@@ -414,8 +434,9 @@ int tick(swh_server* s) {
             WalkArgs a{};
             a.A = j->dA;
             a.B = j->dB;
-            a.regs = (const WalkRegion*)s->wregs.p + x++;
-            a.L = 0;
+            a.regs = (const WalkRegion*)s->wregs.p + x;
+            x += j->wtab.size();
+            a.L = j->L;
             a.brow = j->st.i / j->pl.K;
             a.bcol = j->st.j / j->pl.Kc;
             a.tstride = r.tstride;

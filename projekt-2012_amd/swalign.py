@@ -32,7 +32,7 @@ class Opts(C.Structure):
                 ("hbm_budget_bytes", C.c_int64), ("score_only", C.c_int32),
                 ("precision", C.c_int32), ("want_xpoints", C.c_int32),
                 ("tile_cols", C.c_int32), ("tb_lookahead", C.c_int32),
-                ("reserved", C.c_int32 * 3)]
+                ("waves_per_simd", C.c_int32), ("autotune", C.c_int32), ("reserved", C.c_int32 * 1)]
 
 
 class XPoint(C.Structure):
@@ -76,6 +76,8 @@ EXPORTS = [
                                   C.POINTER(Opts), C.POINTER(Result), C.POINTER(C.c_int32)]),
     ("swh_batch_align_device", C.c_int, [C.c_void_p, C.POINTER(Pair), C.c_int64, C.POINTER(Scoring),
                                          C.POINTER(Opts), C.POINTER(Result), C.POINTER(C.c_int32)]),
+    ("swh_batch_tune", C.c_int, [C.c_void_p, C.POINTER(Pair), C.c_int64, C.c_int32, C.POINTER(Scoring),
+                                 C.POINTER(Opts), C.c_int64, C.c_void_p]),
     ("swh_server_create", C.c_int, [C.c_int, C.POINTER(Scoring), C.POINTER(Opts), C.c_int64, C.c_int32,
                                     C.POINTER(C.c_void_p)]),
     ("swh_server_submit", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
@@ -138,9 +140,12 @@ def _result_dict(r):
     return d
 
 
-def make_opts(K=0, Kc=0, score_only=False, xpoints=False, budget=0, precision=0, tile_cols=0, lookahead=0):
+def make_opts(K=0, Kc=0, score_only=False, xpoints=False, budget=0, precision=0, tile_cols=0, lookahead=0,
+              waves_per_simd=0, autotune=False):
     o = Opts()
     o.tile_cols = tile_cols
+    o.waves_per_simd = waves_per_simd
+    o.autotune = 1 if autotune else 0
     o.tb_lookahead = lookahead
     o.checkpoint_rows = K
     o.checkpoint_cols = Kc
@@ -254,6 +259,19 @@ class Context:
             arr[k] = Pair(C.c_void_p(p1), l1, C.c_void_p(p2), l2)
         return self._batch_call("swh_batch_align_device", arr, len(dev_pairs), sc, kw)
 
+    def batch_tune(self, dev_pairs, sc, sample=2048, **kw):
+        """swh_batch_tune over device-resident pairs (ptr1, len1, ptr2, len2): the fastest
+        tile width, waves per SIMD and checkpoint spacing on a sample of the batch."""
+        arr = (Pair * len(dev_pairs))()
+        for k, (p1, l1, p2, l2) in enumerate(dev_pairs):
+            arr[k] = Pair(C.c_void_p(p1), l1, C.c_void_p(p2), l2)
+        t = BatchTuning()
+        o = make_opts(**kw)
+        rc = lib().swh_batch_tune(self._h, arr, len(dev_pairs), 1, C.byref(sc), C.byref(o), sample, C.byref(t))
+        if rc != SWH_OK:
+            raise SWError(rc, "swh_batch_tune")
+        return {k: getattr(t, k) for k, _ in BatchTuning._fields_ if not k.startswith("_")}
+
     def _batch_call(self, fn, arr, npairs, sc, kw):
         import time
         res = (Result * npairs)()
@@ -296,6 +314,11 @@ def synth_mutate(seed, src, p_sub, p_indel, flank_frac):
         return C.string_at(out, ln.value)
     finally:
         lib().swh_free(out)
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("tile_cols", C.c_int32), ("waves_per_simd", C.c_int32), ("checkpoint_rows", C.c_int64),
+                ("ms_best", C.c_double), ("ms_default", C.c_double), ("candidates", C.c_int32), ("_pad", C.c_int32)]
 
 
 class ServeStats(C.Structure):

@@ -22,7 +22,7 @@ def test_header_declares_the_boundary():
     names = declared_functions()
     for must in ("swh_ctx_create", "swh_align_pair", "swh_align_pair_device", "swh_forward", "swh_batch_align",
                  "swh_batch_align_device", "swh_server_create", "swh_server_submit", "swh_server_poll",
-                 "swh_server_stats", "swh_server_destroy",
+                 "swh_server_stats", "swh_server_destroy", "swh_batch_tune",
                  "swh_result_free", "swh_strerror"):
         assert must in names
 

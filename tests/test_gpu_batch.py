@@ -91,3 +91,28 @@ def test_batch_score_only(sw, orc, ctx):
         assert r["score"] == sc
         if sc > 0:
             assert r["end"] == (i, j) and r["cigar"] is None
+
+
+def test_batch_autotune(sw, orc, ctx):
+    """The tuner reports a candidate from its grid, and a batch run with autotune gives the
+    oracle's alignments like any other setting."""
+    pairs = _mixed_pairs(sw, 23, 120, [40, 400, 1500, 3000])
+    dev = sw.DeviceBuffer(b"".join(a + b for a, b in pairs))
+    base, off, dp = dev.address, 0, []
+    for a, b in pairs:
+        dp.append((base + off, len(a), base + off + len(a), len(b)))
+        off += len(a) + len(b)
+    t = ctx.batch_tune(dp, sw.scoring(*SC), sample=64)
+    assert t["tile_cols"] in (256, 384, 512, 768, 1024)
+    assert t["waves_per_simd"] in (1, 2)
+    assert t["checkpoint_rows"] in (512, 1024, 2048)
+    assert t["candidates"] == 30
+    assert 0 < t["ms_best"] <= t["ms_default"] * 1.0001
+    # crosspoints sit on checkpoint rows, so they depend on the spacing the tuner picks
+    res, st = ctx.batch_device(dp, sw.scoring(*SC), autotune=True)
+    _check(orc, pairs, res, st, xpoints=False)
+    K = t["checkpoint_rows"]
+    res2, st2 = ctx.batch_device(dp, sw.scoring(*SC), xpoints=True, tile_cols=t["tile_cols"],
+                                 waves_per_simd=t["waves_per_simd"], K=K, Kc=K)
+    _check(orc, pairs, res2, st2, K=K)
+    dev.free()
