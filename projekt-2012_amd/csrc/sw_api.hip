@@ -152,6 +152,11 @@ int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions,
         return Sr[(size_t)x] + Gr[(size_t)x] > Sr[(size_t)y] + Gr[(size_t)y];
     });
     std::vector<TileSpan> spans;
+    {
+        int64_t total = 0;
+        for (int64_t q = 0; q < nr; ++q) total += Sr[(size_t)q] + Gr[(size_t)q] - 1;
+        spans.reserve((size_t)total);
+    }
     std::vector<int64_t> span_at((size_t)D + 1, 0), tiles_at((size_t)D, 0);
     for (int64_t d = 0; d < D; ++d) {
         span_at[(size_t)d] = (int64_t)spans.size();
@@ -184,6 +189,38 @@ int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions,
     if (launches) *launches = D;
     // the host vectors above fed asynchronous copies: keep them alive until those finish
     HCK(c, hipStreamSynchronize(c->stream));
+    return SWH_OK;
+}
+
+int code_shift1_for(const uint32_t mask[8], const swh_scoring* sc) {
+    if ((int64_t)sc->match - sc->mismatch > 255) return 0;
+    for (int sh = 0; sh <= 5; ++sh) {
+        unsigned used = 0;
+        bool ok = true;
+        for (int x = 0; x < 256 && ok; ++x) {
+            if (!(mask[x >> 5] >> (x & 31) & 1u)) continue;
+            const unsigned code = 1u << ((x >> sh) & 7);
+            ok = !(used & code);
+            used |= code;
+        }
+        if (ok) return sh + 1;
+    }
+    return 0;
+}
+
+int code_shift1_device(swh_ctx* c, const std::vector<ByteRange>& ranges, const swh_scoring* sc, int* shift1) {
+    *shift1 = 0;
+    if (c->codes_off || (int64_t)sc->match - sc->mismatch > 255 || ranges.empty()) return SWH_OK;
+    HCK(c, ensure(c->ranges, sizeof(ByteRange) * ranges.size()));
+    HCK(c, ensure(c->alpha, sizeof(uint32_t) * 8));
+    HCK(c, hipMemcpyAsync(c->ranges.p, ranges.data(), sizeof(ByteRange) * ranges.size(), hipMemcpyHostToDevice,
+                          c->stream));
+    HCK(c, hipMemsetAsync(c->alpha.p, 0, sizeof(uint32_t) * 8, c->stream));
+    HCK(c, launch_alphabet((const ByteRange*)c->ranges.p, (int64_t)ranges.size(), (uint32_t*)c->alpha.p, c->stream));
+    uint32_t mask[8] = {};
+    HCK(c, hipMemcpyAsync(mask, c->alpha.p, sizeof mask, hipMemcpyDeviceToHost, c->stream));
+    HCK(c, hipStreamSynchronize(c->stream));
+    *shift1 = code_shift1_for(mask, sc);
     return SWH_OK;
 }
 
@@ -262,6 +299,8 @@ constexpr int64_t kBatchTileCols = 512;
 // Runs stage 1, the end search and (for jobs that want it) the traceback of every job,
 // all jobs side by side in each sweep.  Fills out[k] (result_init'ed by the caller) and
 // jobs[k].status.  Stage times are those of the whole batch.
+constexpr int64_t kAlphabetRange = INT64_C(1) << 20;
+
 int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const EngineOpts& eo, swh_result* out) {
     const int64_t nj = (int64_t)jobs.size();
     int64_t rm_elems = 0, sr_elems = 0, scl_elems = 0, ops_elems = 0, xps_elems = 0;
@@ -339,6 +378,16 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
     } else {
         wps = eo.tile_cols > 0 || packed ? 1 : choose_wps(c, max_m, max_n);
         W = tile_cols(c, max_n, eo.tile_cols, packed ? 2 : wps);
+    }
+    {  // symbol codes for the int32 sweeps, from the bytes of the pairs (ranges of <= 1 MiB)
+        std::vector<ByteRange> br;
+        for (int64_t k : live) {
+            const Job& j = jobs[k];
+            for (int64_t o = 0; o < j.m; o += kAlphabetRange) br.push_back(ByteRange{j.A + o, std::min(kAlphabetRange, j.m - o)});
+            for (int64_t o = 0; o < j.n; o += kAlphabetRange) br.push_back(ByteRange{j.B + o, std::min(kAlphabetRange, j.n - o)});
+        }
+        const int rc = code_shift1_device(c, br, sc, &p.code_shift1);
+        if (rc) return rc;
     }
     int64_t launches = 0;
     int rc = run_sweep(c, kStage1, p, regions, W, 0, c->ev[0], c->ev[1], wps, &launches, 0, packed);
@@ -591,9 +640,16 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
     HCK(c, hipEventRecord(c->ev[5], c->stream));
 
     // ---- results: ops (reversed) -> run-length CIGAR, crosspoints start -> end ---------
-    std::vector<uint8_t> ops_h((size_t)ops_elems);
+    if ((size_t)ops_elems > c->host_ops_cap) {  // pinned, grow-only: the copy runs at full link speed
+        if (c->host_ops) (void)hipHostFree(c->host_ops);
+        c->host_ops = nullptr;
+        c->host_ops_cap = 0;
+        HCK(c, hipHostMalloc((void**)&c->host_ops, (size_t)ops_elems, 0));
+        c->host_ops_cap = (size_t)ops_elems;
+    }
+    const uint8_t* ops_h = c->host_ops;
     std::vector<XPoint> xps_h((size_t)xps_elems);
-    HCK(c, hipMemcpyAsync(ops_h.data(), c->ops.p, (size_t)ops_elems, hipMemcpyDeviceToHost, c->stream));
+    HCK(c, hipMemcpyAsync(c->host_ops, c->ops.p, (size_t)ops_elems, hipMemcpyDeviceToHost, c->stream));
     if (eo.want_xp)
         HCK(c, hipMemcpyAsync(xps_h.data(), c->xps.p, sizeof(XPoint) * xps_elems, hipMemcpyDeviceToHost, c->stream));
     HCK(c, hipStreamSynchronize(c->stream));
@@ -608,7 +664,7 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
             if (!j.traceback || j.status != SWH_OK) continue;
             r.ms_traceback = ms_tb;
             r.ms_total = r.ms_forward + r.ms_end_search + ms_tb;
-            if (fill_alignment(r, states[(size_t)k], ops_h.data() + j.ops_off, xps_h.data() + j.xps_off, j.xcap,
+            if (fill_alignment(r, states[(size_t)k], ops_h + j.ops_off, xps_h.data() + j.xps_off, j.xcap,
                                j.best, eo.want_xp) != SWH_OK) {
                 fail[(size_t)tid] = 1;
                 return;
@@ -734,6 +790,9 @@ int swh_ctx_create(int device, swh_ctx** out) {
     // with more tiles than others, and a launch lasts as long as its busiest SIMD).
     const char* env = getenv("SWALIGN_SWEEP_WAVES_PER_SIMD");
     if (env && (atoi(env) == 1 || atoi(env) == 2)) c->wps_fixed = atoi(env);
+    // SWALIGN_SYMBOL_CODES=0 keeps the byte-compare score in every sweep
+    const char* codes = getenv("SWALIGN_SYMBOL_CODES");
+    c->codes_off = codes && atoi(codes) == 0;
     const size_t per_cu = (size_t)c->props.maxSharedMemoryPerMultiProcessor, stat = sweep_static_lds();
     for (int w = 1; w <= 2; ++w) {
         const size_t want = per_cu / (size_t)(w + 1) + 1024;
@@ -752,7 +811,7 @@ int swh_ctx_destroy(swh_ctx* c) {
     (void)hipSetDevice(c->device);
     DevBuf* bufs[] = {&c->seqA, &c->seqB, &c->rbuf, &c->cb, &c->cor, &c->rowmax, &c->srow, &c->scol,
                       &c->dirs, &c->regions, &c->key, &c->found, &c->walk, &c->ops, &c->xps, &c->wregs,
-                      &c->argjobs, &c->wargs, &c->seqs, &c->spans};
+                      &c->argjobs, &c->wargs, &c->seqs, &c->spans, &c->alpha, &c->ranges};
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
@@ -760,6 +819,7 @@ int swh_ctx_destroy(swh_ctx* c) {
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->staging) (void)hipHostFree(c->staging);
+    if (c->host_ops) (void)hipHostFree(c->host_ops);
     delete c;
     return SWH_OK;
 }

@@ -21,12 +21,15 @@ struct swh_ctx {
     hipStream_t stream = nullptr;
     hipDeviceProp_t props{};
     DevBuf seqA, seqB, rbuf, cb, cor, rowmax, srow, scol, dirs, regions, key, found, walk, ops, xps, wregs,
-        argjobs, wargs, seqs, spans;
+        argjobs, wargs, seqs, spans, alpha, ranges;
     uint8_t* staging = nullptr;  // pinned host staging of batch inputs
     size_t staging_cap = 0;
+    uint8_t* host_ops = nullptr;  // pinned host copy of the traceback ops
+    size_t host_ops_cap = 0;
     hipEvent_t ev[8] = {};
     size_t lds_pad[3] = {0, 0, 0};  // dynamic LDS per sweep workgroup for w workgroups per CU
     int wps_fixed = 0;              // sweep waves per SIMD forced by the environment, 0 = model
+    bool codes_off = false;         // symbol-code scores disabled by the environment
     hipError_t last = hipSuccess;
 };
 
@@ -58,6 +61,14 @@ int run_sweep(swh_ctx* c, int mode, swk::SweepParams p, std::vector<swk::Region>
 
 // True if a pair of m x n under scoring sc cannot leave int16 in the packed stage 1.
 bool fits_int16(int64_t m, int64_t n, const swh_scoring* sc);
+
+// Symbol codes for the sweeps' scores (SweepParams::code_shift1): the smallest shift s in
+// [0, 5] under which (byte >> s) & 7 is one-to-one on the bytes present (mask: bit x of
+// word x >> 5), plus one; 0 if there is none or match - mismatch exceeds a byte.
+int code_shift1_for(const uint32_t mask[8], const swh_scoring* sc);
+// Same, for the bytes of device ranges (one presence kernel and a read-back).
+int code_shift1_device(swh_ctx* c, const std::vector<swk::ByteRange>& ranges, const swh_scoring* sc,
+                       int* shift1);
 
 struct Plan {
     int64_t K, Kc, nsr, nsc;

@@ -74,7 +74,7 @@ struct SweepParams {
     const Region* regions;
     const TileSpan* spans;  // this launch's spans, ordered by first_tile
     int32_t nspans;
-    int32_t _pad;
+    int32_t code_shift1;    // 0: compare bytes; s + 1: symbol code (byte >> s) & 7 selects a row profile byte
     int64_t ntiles;         // tiles of this launch (grid = ceil(ntiles / kSweepWaves) workgroups)
 };
 
@@ -133,6 +133,14 @@ struct ArgmaxJob {
     unsigned long long* key;
 };
 hipError_t launch_argmax_rows(const ArgmaxJob* jobs, int njobs, int blocks_per_job, hipStream_t s);
+// A byte range in device memory.
+struct ByteRange {
+    const uint8_t* p;
+    int64_t n;
+};
+// ORs into mask[8] (zeroed by the caller) bit x for every byte x present in the ranges.
+hipError_t launch_alphabet(const ByteRange* ranges, int64_t nranges, uint32_t* mask, hipStream_t s);
+
 hipError_t launch_walk(const WalkArgs* jobs, int njobs, hipStream_t s);  // one workgroup per job
 
 }  // namespace swk

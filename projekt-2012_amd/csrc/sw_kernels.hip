@@ -17,6 +17,13 @@
 //     E  = max(E + ext, Ho_left)               F = max(F_up + ext, Ho_up)
 //     H  = max(0, Hdiag + s, E, F)             Ho = H + O
 // with Hdiag + s computed as Ho_diag + (s - O).
+//
+// Substitution score.  In general s = (a == b) ? match : mismatch, one compare and one
+// select per cell.  When the pair's bytes map one-to-one onto 3-bit symbol codes
+// ((byte >> shift) & 7, chosen by the host from the bytes present), each row instead holds
+// an 8-byte profile whose byte c is (match - mismatch) if c is the row's code, else 0, and
+// each column carries a v_perm_b32 selector for its code: one v_perm_b32 yields the bonus
+// and Hdiag + s = Ho_diag + (mismatch - O) + bonus is one three-operand add.
 #include "sw_internal.h"
 
 namespace swk {
@@ -29,13 +36,15 @@ namespace {
 #define SWK_DO_PRAGMA(x) _Pragma(#x)
 #define SWK_UNROLL(n) SWK_DO_PRAGMA(unroll n)
 
-constexpr int kDppWaveShl1 = 0x130;  // lane l <- lane l+1, lane 63 keeps `old`
+constexpr int kDppWaveShl1 = 0x130;  // lane l <- lane l+1
 constexpr int kDppWaveShr1 = 0x138;  // lane l <- lane l-1, lane 0 keeps `old`
 
 __device__ __forceinline__ int shr1(int old, int src) {
     return __builtin_amdgcn_update_dpp(old, src, kDppWaveShr1, 0xF, 0xF, false);
 }
-__device__ __forceinline__ int shl1(int v) { return __builtin_amdgcn_update_dpp(v, v, kDppWaveShl1, 0xF, 0xF, false); }
+// Queue shift: lane 63 receives 0 (bound_ctrl), which is never read, so the shift needs
+// no copy of the queue to preserve an `old` value and shr1 can consume the queue in place.
+__device__ __forceinline__ int shl1(int v) { return __builtin_amdgcn_mov_dpp(v, kDppWaveShl1, 0xF, 0xF, true); }
 
 // Copies a wave-uniform value into a VGPR the compiler cannot see through (a DPP
 // row_mirror of a uniform value is the value itself), so that per-step selects keep
@@ -65,9 +74,20 @@ struct TileCtx {
     uint16_t* dirs;     // kDirs: [step][lane] codes of this tile
 };
 
+// Column symbol carried down the lanes: the byte itself, or with symbol codes the
+// v_perm_b32 selector of its code (byte 0 picks profile byte `code`, bytes 1-3 read 0).
+// Pad columns get a value that matches nothing (0x100 / all-zero selector).
+constexpr int kPadByte = 0x100;
+constexpr int kPadSel = 0x0c0c0c0c;
+template <bool PROF>
+__device__ __forceinline__ int col_symbol(const SweepParams& p, uint32_t byte) {
+    if (!PROF) return (int)byte;
+    return (int)(((byte >> (p.code_shift1 - 1)) & 7u) | 0x0c0c0c00u);
+}
+
 // Loads the inputs of chunk cc (steps [64cc, 64cc+64)) one column per lane: the top
-// boundary {H+O, F+E} and the column byte of column 64cc + lane (0x100 pads never equal
-// a byte).
+// boundary {H+O, F+E} and the column symbol of column 64cc + lane.
+template <bool PROF>
 __device__ __forceinline__ void load_chunk_inputs(const SweepParams& p, const TileCtx& T, int64_t cc,
                                                   int2& in, int& b1) {
     const int lane = wave_lane();
@@ -79,7 +99,27 @@ __device__ __forceinline__ void load_chunk_inputs(const SweepParams& p, const Ti
         F = v.y;
     }
     in = make_int2(H + p.sc.O, F + p.sc.E);
-    b1 = (col < T.w) ? (int)T.B[T.C0 + col] : 0x100;
+    b1 = (col < T.w) ? col_symbol<PROF>(p, T.B[T.C0 + col]) : (PROF ? kPadSel : kPadByte);
+}
+
+// Row symbols of a lane: lo[k] is row k's byte, or with symbol codes lo/hi[k] hold row k's
+// 8-byte profile (bytes 0-3 / 4-7).  Rows past the region match nothing.
+template <int R>
+struct RowSyms {
+    uint32_t lo[R], hi[R];
+};
+template <bool PROF>
+__device__ __forceinline__ void row_symbol(const SweepParams& p, bool in, uint32_t byte, uint32_t& lo,
+                                           uint32_t& hi) {
+    if (!PROF) {
+        lo = in ? byte : 0x200u;
+        hi = 0;
+        return;
+    }
+    const uint32_t code = (byte >> (p.code_shift1 - 1)) & 7u;
+    const uint64_t prof = in ? (uint64_t)(uint32_t)(p.sc.M - p.sc.X) << (8 * code) : 0;
+    lo = (uint32_t)prof;
+    hi = (uint32_t)(prof >> 32);
 }
 
 // 64 steps of one tile.  The chunk's inputs wait in register queues, one column per
@@ -89,9 +129,9 @@ __device__ __forceinline__ void load_chunk_inputs(const SweepParams& p, const Ti
 // column byte of the lane's current column, passed down the lanes with the diagonal.
 // EDGE: some (lane, step) pairs fall outside [0, w) and must leave the lane state
 // untouched.  DUMP: stage-1 chunk containing a checkpoint column.
-template <int R, int MODE, bool EDGE, bool DUMP>
+template <int R, int MODE, bool PROF, bool EDGE, bool DUMP>
 __device__ __forceinline__ void chunk_steps(const SweepParams& p, const TileCtx& T, int64_t c, int64_t row0,
-                                            const uint32_t (&a)[R], int (&Ho)[R], int (&E)[R], int (&rmax)[R],
+                                            const RowSyms<R>& a, int (&Ho)[R], int (&E)[R], int (&rmax)[R],
                                             int& Ho_up_prev, int& Ho_bot, int& Fe_bot, int& Qh, int& Qf, int& Qb,
                                             int& bl, int2* s_out, int& found_col) {
     const int lane = wave_lane();
@@ -121,8 +161,9 @@ __device__ __forceinline__ void chunk_steps(const SweepParams& p, const TileCtx&
         uint32_t nib = 0;
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            const int sp = (a[k] == b) ? MO : XO;
-            const int Hd = diag + sp;  // H(i-1,j-1) + s(i,j)
+            int Hd;  // H(i-1,j-1) + s(i,j)
+            if (PROF) Hd = diag + XO + (int)__builtin_amdgcn_perm(a.hi[k], a.lo[k], b);
+            else Hd = diag + ((a.lo[k] == b) ? MO : XO);
             diag = Ho[k];              // H(i,j-1) + O: diagonal of the next row
             const int Ee = E[k] + Ev;
             const int En = max(Ee, Ho[k]);
@@ -176,7 +217,7 @@ struct SweepLds {
 // Sweeps one tile.  left/left_r0: {H,E} of column C0-1 at absolute row r is
 // left[r - left_r0] (nullptr = zero column); corner = H(R0-1, C0-1); right (nullable)
 // receives the tile's last column {H,E} at index row - right_r0.
-template <int R, int MODE>
+template <int R, int MODE, bool PROF>
 __device__ void sweep_tile(const SweepParams& p, const TileCtx& T, const int2* left, int64_t left_r0, int corner,
                            int2* right, int64_t right_r0, bool first_segment, int32_t* found, SweepLds& lds) {
     int2* s_out = lds.out;
@@ -185,13 +226,13 @@ __device__ void sweep_tile(const SweepParams& p, const TileCtx& T, const int2* l
     const int64_t w = T.w;
     const int64_t row0 = T.R0 + (int64_t)lane * R;
 
-    uint32_t a[R];
+    RowSyms<R> a;
     int Ho[R], E[R], rmax[R];
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         const int64_t row = row0 + k;
         const bool in = row <= T.r1;
-        a[k] = in ? (uint32_t)T.A[row] : 0x200u;
+        row_symbol<PROF>(p, in, in ? (uint32_t)T.A[row] : 0u, a.lo[k], a.hi[k]);
         if (left && in) {
             const int2 v = left[row - left_r0];
             Ho[k] = v.x + O;
@@ -214,11 +255,11 @@ __device__ void sweep_tile(const SweepParams& p, const TileCtx& T, const int2* l
 
     int2 in_reg;
     int b_reg;
-    load_chunk_inputs(p, T, 0, in_reg, b_reg);
-    int Qh = in_reg.x, Qf = in_reg.y, Qb = b_reg, bl = 0x100;
+    load_chunk_inputs<PROF>(p, T, 0, in_reg, b_reg);
+    int Qh = in_reg.x, Qf = in_reg.y, Qb = b_reg, bl = PROF ? kPadSel : kPadByte;
 
     for (int64_t c = 0; c < nchunks; ++c) {
-        if (c + 1 < nchunks) load_chunk_inputs(p, T, c + 1, in_reg, b_reg);
+        if (c + 1 < nchunks) load_chunk_inputs<PROF>(p, T, c + 1, in_reg, b_reg);
         const bool interior = c >= 1 && c <= last_interior;
         bool dump = false;
         if (MODE == kStage1 && T.scol) {
@@ -228,13 +269,13 @@ __device__ void sweep_tile(const SweepParams& p, const TileCtx& T, const int2* l
         }
         if (!dump) {
             if (interior)
-                chunk_steps<R, MODE, false, false>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
+                chunk_steps<R, MODE, PROF, false, false>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
                                                    Qh, Qf, Qb, bl, s_out, found_col);
             else
-                chunk_steps<R, MODE, true, false>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
+                chunk_steps<R, MODE, PROF, true, false>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
                                                   Qh, Qf, Qb, bl, s_out, found_col);
         } else {
-            chunk_steps<R, MODE, true, true>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot, Qh, Qf,
+            chunk_steps<R, MODE, PROF, true, true>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot, Qh, Qf,
                                              Qb, bl, s_out, found_col);
         }
         // Bottom row of the tile for columns 64c-63+lane.  The LDS hand-off from lane 63 to
@@ -350,13 +391,13 @@ __device__ bool make_tile(const SweepParams& p, int64_t x, TileSetup& ts) {
 
 // One wave = one tile (s, g) with s + g = d of its span; tile index x = wave number in the
 // grid.  Waves of a workgroup are independent.
-template <int R, int MODE>
+template <int R, int MODE, bool PROF>
 __device__ void sweep_main(const SweepParams& p, SweepLds& lds) {
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wave
     const int64_t x = (int64_t)blockIdx.x * kSweepWaves + wid;
     TileSetup ts;
     if (!make_tile<R, MODE>(p, x, ts)) return;
-    sweep_tile<R, MODE>(p, ts.T, ts.left, ts.left_r0, ts.corner, ts.right, ts.right_r0, ts.first_segment, ts.found,
+    sweep_tile<R, MODE, PROF>(p, ts.T, ts.left, ts.left_r0, ts.corner, ts.right, ts.right_r0, ts.first_segment, ts.found,
                         lds);
 }
 
@@ -601,10 +642,10 @@ __global__ __launch_bounds__(64 * kSweepWaves) void k_sweep_pk(SweepParams p) {
     sweep_main_pk<R>(p, lds[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)]);
 }
 
-template <int R, int MODE>
+template <int R, int MODE, bool PROF>
 __global__ __launch_bounds__(64 * kSweepWaves) void k_sweep(SweepParams p) {
     __shared__ SweepLds lds[kSweepWaves];
-    sweep_main<R, MODE>(p, lds[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)]);
+    sweep_main<R, MODE, PROF>(p, lds[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)]);
 }
 
 // Row argmax of job blockIdx.y over blockIdx.x's share of its rows: the largest score,
@@ -621,6 +662,26 @@ __device__ void argmax_rows_main(const ArgmaxJob& j) {
         best = o > best ? o : best;
     }
     if ((threadIdx.x & 63) == 0 && best) atomicMax(j.key, best);
+}
+
+// Bytes present in a set of byte ranges: bit x of mask[x >> 5] is set if byte x occurs.
+// Each workgroup gathers its ranges' bytes in LDS (an atomic only for a byte not yet seen)
+// and ors its words into mask once.
+__global__ __launch_bounds__(256) void k_alphabet(const ByteRange* ranges, int64_t nranges, uint32_t* mask) {
+    __shared__ uint32_t seen[8];
+    if (threadIdx.x < 8) seen[threadIdx.x] = 0;
+    __syncthreads();
+    for (int64_t q = blockIdx.x; q < nranges; q += gridDim.x) {
+        const ByteRange r = ranges[q];
+        for (int64_t i = threadIdx.x; i < r.n; i += blockDim.x) {
+            const uint32_t x = r.p[i];
+            const uint32_t bit = 1u << (x & 31u);
+            if (!(__hip_atomic_load(&seen[x >> 5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) & bit))
+                atomicOr(&seen[x >> 5], bit);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 8 && seen[threadIdx.x]) atomicOr(&mask[threadIdx.x], seen[threadIdx.x]);
 }
 
 __global__ __launch_bounds__(256) void k_argmax_rows(const ArgmaxJob* jobs) {
@@ -822,10 +883,14 @@ __global__ __launch_bounds__(64) void k_walk(const WalkArgs* jobs) {
 hipError_t launch_sweep(int mode, const SweepParams& p, size_t lds_pad, hipStream_t s) {
     constexpr int R = kRowsPerLane;
     const dim3 grid((unsigned)((p.ntiles + kSweepWaves - 1) / kSweepWaves)), block(64 * kSweepWaves);
-    switch (mode) {
-        case kStage1: hipLaunchKernelGGL((k_sweep<R, kStage1>), grid, block, lds_pad, s, p); break;
-        case kDirs: hipLaunchKernelGGL((k_sweep<R, kDirs>), grid, block, lds_pad, s, p); break;
-        case kSearch: hipLaunchKernelGGL((k_sweep<R, kSearch>), grid, block, lds_pad, s, p); break;
+    const bool prof = p.code_shift1 != 0;
+    switch (mode * 2 + (prof ? 1 : 0)) {
+        case kStage1 * 2: hipLaunchKernelGGL((k_sweep<R, kStage1, false>), grid, block, lds_pad, s, p); break;
+        case kDirs * 2: hipLaunchKernelGGL((k_sweep<R, kDirs, false>), grid, block, lds_pad, s, p); break;
+        case kSearch * 2: hipLaunchKernelGGL((k_sweep<R, kSearch, false>), grid, block, lds_pad, s, p); break;
+        case kStage1 * 2 + 1: hipLaunchKernelGGL((k_sweep<R, kStage1, true>), grid, block, lds_pad, s, p); break;
+        case kDirs * 2 + 1: hipLaunchKernelGGL((k_sweep<R, kDirs, true>), grid, block, lds_pad, s, p); break;
+        case kSearch * 2 + 1: hipLaunchKernelGGL((k_sweep<R, kSearch, true>), grid, block, lds_pad, s, p); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -843,11 +908,12 @@ size_t sweep_static_lds() { return sizeof(SweepLds) * kSweepWaves; }
 hipError_t sweep_allow_lds(size_t dyn_bytes) {
     constexpr int R = kRowsPerLane;
     const int v = (int)dyn_bytes;
-    hipError_t e = hipFuncSetAttribute((const void*)k_sweep<R, kStage1>, hipFuncAttributeMaxDynamicSharedMemorySize, v);
-    if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void*)k_sweep<R, kDirs>, hipFuncAttributeMaxDynamicSharedMemorySize, v);
-    if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void*)k_sweep<R, kSearch>, hipFuncAttributeMaxDynamicSharedMemorySize, v);
+    const void* fns[] = {(const void*)k_sweep<R, kStage1, false>, (const void*)k_sweep<R, kDirs, false>,
+                         (const void*)k_sweep<R, kSearch, false>, (const void*)k_sweep<R, kStage1, true>,
+                         (const void*)k_sweep<R, kDirs, true>,    (const void*)k_sweep<R, kSearch, true>};
+    hipError_t e = hipSuccess;
+    for (const void* f : fns)
+        if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, v);
     if (e == hipSuccess)
         e = hipFuncSetAttribute((const void*)k_sweep_pk<R>, hipFuncAttributeMaxDynamicSharedMemorySize, v);
     return e;
@@ -857,6 +923,13 @@ hipError_t launch_argmax_rows(const ArgmaxJob* jobs, int njobs, int blocks_per_j
     if (njobs <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_argmax_rows, dim3((unsigned)(blocks_per_job > 1 ? blocks_per_job : 1), (unsigned)njobs), dim3(256), 0, s,
                        jobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_alphabet(const ByteRange* ranges, int64_t nranges, uint32_t* mask, hipStream_t s) {
+    if (nranges <= 0) return hipSuccess;
+    const unsigned grid = (unsigned)(nranges < 4096 ? nranges : 4096);
+    hipLaunchKernelGGL(k_alphabet, dim3(grid), dim3(256), 0, s, ranges, nranges, mask);
     return hipGetLastError();
 }
 

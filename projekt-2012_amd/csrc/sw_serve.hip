@@ -97,6 +97,7 @@ constexpr int kTickRing = 4;
 struct SJob {
     int64_t id = 0;
     std::vector<uint8_t> a, b;  // host copies, oriented (a = rows)
+    uint32_t alpha[8] = {};     // bytes present in a and b (bit x of word x >> 5)
     bool swapped = false;
     int64_t m = 0, n = 0;
     Plan pl{};
@@ -384,6 +385,12 @@ int tick(swh_server* s) {
     }
     SweepParams p{};
     p.sc = Score{s->sc.match, s->sc.mismatch, s->sc.gap_open, s->sc.gap_extend};
+    if (!c->codes_off) {  // symbol codes when one mapping separates the bytes of every job in flight
+        uint32_t mask[8] = {};
+        for (SJob* j : s->flight)
+            for (int w = 0; w < 8; ++w) mask[w] |= j->alpha[w];
+        p.code_shift1 = code_shift1_for(mask, &s->sc);
+    }
     for (int mo = 0; mo < 3; ++mo) {
         if (regs[mo].empty() || spans[mo].empty()) continue;
         HCK(c, ensure(slot.regs[mo], sizeof(Region) * regs[mo].size()));
@@ -683,6 +690,13 @@ int swh_server_submit(swh_server* s, const uint8_t* seq1, int64_t len1, const ui
     j->n = j->swapped ? len1 : len2;
     j->a.assign(A, A + j->m);
     j->b.assign(B, B + j->n);
+    {
+        bool seen[256] = {};
+        for (uint8_t x : j->a) seen[x] = true;
+        for (uint8_t x : j->b) seen[x] = true;
+        for (int x = 0; x < 256; ++x)
+            if (seen[x]) j->alpha[x >> 5] |= 1u << (x & 31);
+    }
     // checkpoints every max(m, n)/128 rows (1024 to 8192, or the options: short jobs keep
     // their traceback blocks small, long ones need fewer rounds), doubled until they take
     // at most an eighth of the pool

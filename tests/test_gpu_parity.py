@@ -156,3 +156,42 @@ def test_int16_bound_selects_precision(sw, ctx, L, precision):
     r = ctx.align(s, s, sw.scoring(1, -3, -5, -2), score_only=True)
     assert r["precision"] == precision
     assert r["score"] == L and r["end"] == (L - 1, L - 1)
+
+
+def _alphabet_pair(name, L, seed):
+    rng = random.Random(seed)
+    al = {"acgt": b"ACGT", "acgtn": b"ACGTN", "mixed_case": b"ACGTacgt", "ac": b"AC",
+          "bytes": bytes(range(1, 256))}[name]
+    a = bytes(rng.choice(al) for _ in range(L))
+    b = bytearray(a[L // 10: L - L // 12])
+    for i in range(len(b)):
+        if rng.random() < 0.1:
+            b[i] = rng.choice(al)
+    return a, bytes(b)
+
+
+@pytest.mark.parametrize("name", ["acgt", "acgtn", "mixed_case", "ac", "bytes"])
+@pytest.mark.parametrize("score", [(1, -3, -5, -2), (200, -100, -50, -10)])
+def test_symbol_codes_and_byte_compare(sw, orc, ctx, name, score):
+    """Substitution scores through per-row symbol profiles (alphabets that 3-bit codes
+    separate, match - mismatch <= 255) and through byte compares (mixed case, all bytes,
+    a wide score range) agree with the oracle, and the byte-compare path forced by
+    SWALIGN_SYMBOL_CODES=0 gives the same results."""
+    a, b = _alphabet_pair(name, 2600, len(name) + score[0])
+    sc = sw.scoring(*score)
+    got = ctx.align(a, b, sc, K=512, Kc=256, xpoints=True, tile_cols=96)
+    A, B = orient(a, b)
+    ref = orc.align_linear(A, B, orc.scoring(*score), 512, 256)
+    _cmp(got, ref)
+    assert got["xpoints"] == [tuple(x) for x in ref["xpoints"]]
+    os.environ["SWALIGN_SYMBOL_CODES"] = "0"
+    try:
+        plain = sw.Context(0)
+    finally:
+        del os.environ["SWALIGN_SYMBOL_CODES"]
+    try:
+        r2 = plain.align(a, b, sc, K=512, Kc=256, xpoints=True, tile_cols=96)
+    finally:
+        plain.close()
+    for k in ("score", "end", "start", "cigar", "xpoints"):
+        assert r2[k] == got[k], k
