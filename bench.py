@@ -61,12 +61,12 @@ def _counter_avg(fname, counter, kernel_part):
     return None
 
 
-def measured_traffic(workload):
+def measured_traffic(workload, codes=1):
     """HBM bytes per stage-1 launch from the committed FETCH_SIZE / WRITE_SIZE profiles of
     this workload, each scaled by what the counter reads for a 256 MiB copy."""
     if workload != 1:
         return None, None
-    k = "k_sweep<4, 0>"
+    k = "k_sweep<4, 0, %s>" % ("true" if codes else "false")
     fetch = _counter_avg("k_sweep_stage1_1Mbp_pair_fetch_size.txt", "FETCH_SIZE", k)
     write = _counter_avg("k_sweep_stage1_1Mbp_pair_write_size.txt", "WRITE_SIZE", k)
     cf = _counter_avg("copy_256MiB_fetch_size.txt", "FETCH_SIZE", "copyBuffer")
@@ -86,12 +86,12 @@ def algorithmic_bytes(cells, n_cols_per_tile, K, Kc):
     return cells * per_cell
 
 
-def roofline(cells, fwd_ms, launches, workload=None, W=None, K=None, Kc=None, precision=32):
+def roofline(cells, fwd_ms, launches, workload=None, W=None, K=None, Kc=None, precision=32, codes=1):
     """Stage-1 sweep against the integer-VALU peak of the precision it ran in (int32, or
     packed int16 pairs at twice the operation rate)."""
     achieved = cells * OPS_PER_CELL / (fwd_ms / 1000.0) / 1e12
     peak = PEAK_INT16X2_TOPS if precision == 16 else PEAK_INT32_TOPS
-    traffic, source = measured_traffic(workload)
+    traffic, source = measured_traffic(workload, codes)
     extra = {}
     if W and K and Kc:
         extra["algorithmic_bytes_per_launch"] = int(algorithmic_bytes(cells, W, K, Kc) / max(launches, 1))
@@ -100,7 +100,8 @@ def roofline(cells, fwd_ms, launches, workload=None, W=None, K=None, Kc=None, pr
     return {
         "bound": "valu",
         "kernel": ("k_sweep_pk<4> (forward sweep, packed 16-bit)" if precision == 16
-                   else "k_sweep<4, kStage1> (forward sweep with checkpoints)"),
+                   else "k_sweep<4, kStage1, %s> (forward sweep with checkpoints, %s scores)"
+                   % (("true", "symbol-code") if codes else ("false", "byte-compare"))),
         "achieved": round(achieved, 3),
         "peak": round(peak, 2),
         "unit": "TOP/s",
@@ -151,14 +152,14 @@ def run_batch(args, torch, sw):
     # plain path: one swh_align_pair_device per pair, on a prefix of the batch
     nplain, t_plain, cells_plain = 0, 0.0, 0
     t0 = time.perf_counter()
-    for (p1, l1, p2, l2), (a, b) in zip(dp, pairs):
+    for (p1, l1, p2, l2), (a, b) in zip(dp if not args.no_plain else [], pairs):
         ctx.align_device(p1, l1, p2, l2, sc)
         nplain += 1
         cells_plain += len(a) * len(b)
         if time.perf_counter() - t0 > 10.0:
             break
     t_plain = time.perf_counter() - t0
-    plain_gcups = cells_plain / t_plain / 1e9
+    plain_gcups = cells_plain / t_plain / 1e9 if cells_plain else None
     gcups = cells / sec / 1e9
     line = {
         "metric": METRIC, "value": round(gcups, 2), "unit": "GCUPS", "n_gpus": 1, "steps": args.steps,
@@ -171,8 +172,8 @@ def run_batch(args, torch, sw):
             "stage_ms": {"forward": round(r0["ms_forward"], 3), "end_search": round(r0["ms_end_search"], 3),
                          "traceback": round(r0["ms_traceback"], 3)},
             "forward_gcups": round(cells / (r0["ms_forward"] / 1000.0) / 1e9, 2),
-            "plain_per_pair_gcups": round(plain_gcups, 2), "plain_sample_pairs": nplain,
-            "batch_vs_plain": round(gcups / plain_gcups, 2),
+            "plain_per_pair_gcups": plain_gcups and round(plain_gcups, 2), "plain_sample_pairs": nplain,
+            "batch_vs_plain": plain_gcups and round(gcups / plain_gcups, 2),
             "autotune": tuning and {k: (round(v, 3) if isinstance(v, float) else v) for k, v in tuning.items()},
         },
         "roofline": roofline(cells, r0["ms_forward"], r0["forward_launches"], precision=r0["precision"]),
@@ -317,6 +318,7 @@ def main():
     ap.add_argument("--workload", type=int, default=1, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="workload 3: default batch settings")
+    ap.add_argument("--no-plain", action="store_true", help="workload 3: skip the pair-after-pair comparison")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only copy BYTES device-to-device once (calibrates the FETCH_SIZE / WRITE_SIZE "
                          "counters against a transfer of known size) and exit")
@@ -402,7 +404,7 @@ def main():
             "device": info["name"], "cus": info["cus"], "clock_mhz": round(clock_ghz * 1000),
         },
         "roofline": roofline(cells, fwd_ms, r0["forward_launches"], args.workload, precision=r0["precision"],
-                             W=-(-n // (4 * int(info["cus"] or 256))), K=r0["checkpoint_rows"] or m,
+                             codes=r0["score_codes"], W=-(-n // (4 * int(info["cus"] or 256))), K=r0["checkpoint_rows"] or m,
                              Kc=r0["checkpoint_cols"] or n),
     }
     if not args.no_cpu_baseline:

@@ -69,7 +69,11 @@ typedef struct {
                                /* each direction per round; 0 = auto (3), -1 = none            */
     int32_t waves_per_simd;    /* sweep waves resident per SIMD (1 or 2), 0 = auto             */
     int32_t autotune;          /* batch: 1 = time candidate settings on a sample first          */
-    int32_t reserved[1];
+    int32_t host_checkpoints;  /* 1: checkpoint rows and columns live in pinned host memory    */
+                               /* mapped into the device (the reference's temp/row_R side      */
+                               /* channel, src/Builders.cpp:27-104, kept in memory): HBM holds */
+                               /* none of them and hbm_budget_bytes caps that host memory      */
+                               /* instead (0 = half the available host memory)                 */
 } swh_opts;
 
 /* Path node on a checkpoint row (the analogue of the reference's crosspoints,
@@ -98,6 +102,10 @@ typedef struct {
     double ms_total;          /* device work of the call                          */
     int32_t precision_used;   /* 32 or 16                                        */
     int32_t forward_launches; /* kernel launches of the stage-1 sweep             */
+    int32_t score_codes;      /* 1: the int32 sweeps took substitution scores from
+                                 per-row symbol-code profiles; 0: from byte compares
+                                 (always 0 from the request server)               */
+    int32_t reserved0;
 } swh_result;
 
 typedef struct swh_ctx swh_ctx;
@@ -164,7 +172,7 @@ int swh_batch_align_device(swh_ctx* ctx, const swh_pair* pairs, int64_t npairs, 
                            const swh_opts* opts, swh_result* out, int32_t* statuses);
 
 /* Batch autotuner: times every combination of tile width {256, 384, 512, 768, 1024},
- * sweep waves per SIMD {1, 2} and checkpoint spacing {512, 1024, 2048} on an evenly spaced
+ * sweep waves per SIMD {1, 2} and checkpoint spacing {256, 512, 1024, 2048} on an evenly spaced
  * sample of `sample` pairs (device-resident pairs if `device` != 0) and reports the
  * fastest, to be passed back in swh_opts (tile_cols, waves_per_simd, checkpoint_rows and
  * checkpoint_cols).  swh_opts.autotune = 1 makes the batch entry points do this first. */

@@ -20,6 +20,8 @@
 #include <thread>
 #include <vector>
 
+#include <unistd.h>
+
 #include "sw_host.h"
 
 using namespace swk;
@@ -42,6 +44,23 @@ hipError_t ensure(DevBuf& b, size_t bytes) {
     hipError_t e = hipMalloc(&b.p, bytes);
     if (e == hipSuccess) b.cap = bytes;
     else (void)hipGetLastError();
+    return e;
+}
+
+hipError_t ensure_host(HostBuf& b, size_t bytes) {
+    if (bytes == 0) bytes = 64;
+    if (b.h && b.cap >= bytes) return hipSuccess;
+    if (b.h) (void)hipHostFree(b.h);
+    b = HostBuf{};
+    hipError_t e = hipHostMalloc(&b.h, bytes, hipHostMallocMapped);
+    if (e == hipSuccess) e = hipHostGetDevicePointer(&b.d, b.h, 0);
+    if (e == hipSuccess) {
+        b.cap = bytes;
+    } else {
+        (void)hipGetLastError();
+        if (b.h) (void)hipHostFree(b.h);
+        b = HostBuf{};
+    }
     return e;
 }
 
@@ -263,6 +282,10 @@ int choose_plan(int64_t m, int64_t n, const swh_opts* o, int64_t budget, Plan* p
 
 int64_t default_budget(const swh_opts* o) {
     if (o && o->hbm_budget_bytes > 0) return o->hbm_budget_bytes;
+    if (o && o->host_checkpoints) {  // half the available host memory
+        const long pages = sysconf(_SC_AVPHYS_PAGES), psz = sysconf(_SC_PAGESIZE);
+        return pages > 0 && psz > 0 ? (int64_t)pages * psz / 2 : (INT64_C(8) << 30);
+    }
     size_t freeb = 0, totalb = 0;
     (void)hipMemGetInfo(&freeb, &totalb);
     return (int64_t)(freeb / 2);
@@ -290,6 +313,7 @@ struct EngineOpts {
     int lookahead = 3;       // traceback blocks ahead per direction (0 = none)
     bool want_xp = false;
     bool batch = false;      // many pairs: fixed tile width, two waves per SIMD
+    bool host_ckpt = false;  // checkpoints in mapped pinned host memory
 };
 
 // Columns per tile of a batch sweep: wide enough that the 63 fill steps of a tile cost
@@ -299,7 +323,7 @@ constexpr int64_t kBatchTileCols = 512;
 // Runs stage 1, the end search and (for jobs that want it) the traceback of every job,
 // all jobs side by side in each sweep.  Fills out[k] (result_init'ed by the caller) and
 // jobs[k].status.  Stage times are those of the whole batch.
-constexpr int64_t kAlphabetRange = INT64_C(1) << 20;
+constexpr int64_t kAlphabetRange = INT64_C(1) << 16;  // bytes per workgroup of k_alphabet
 
 int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const EngineOpts& eo, swh_result* out) {
     const int64_t nj = (int64_t)jobs.size();
@@ -321,13 +345,22 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
         max_n = std::max(max_n, j.n);
     }
     HCK(c, ensure(c->rowmax, sizeof(int32_t) * std::max<int64_t>(rm_elems, 1)));
-    HCK(c, ensure(c->srow, sizeof(int2) * std::max<int64_t>(sr_elems, 1)));
-    HCK(c, ensure(c->scol, sizeof(int2) * std::max<int64_t>(scl_elems, 1)));
+    int2* srow;
+    int2* scol;
+    if (eo.host_ckpt) {
+        HCK(c, ensure_host(c->hsrow, sizeof(int2) * std::max<int64_t>(sr_elems, 1)));
+        HCK(c, ensure_host(c->hscol, sizeof(int2) * std::max<int64_t>(scl_elems, 1)));
+        srow = (int2*)c->hsrow.d;
+        scol = (int2*)c->hscol.d;
+    } else {
+        HCK(c, ensure(c->srow, sizeof(int2) * std::max<int64_t>(sr_elems, 1)));
+        HCK(c, ensure(c->scol, sizeof(int2) * std::max<int64_t>(scl_elems, 1)));
+        srow = (int2*)c->srow.p;
+        scol = (int2*)c->scol.p;
+    }
     HCK(c, ensure(c->key, sizeof(unsigned long long) * std::max<int64_t>(nj, 1)));
     HCK(c, ensure(c->found, sizeof(int32_t) * std::max<int64_t>(nj, 1)));
     int32_t* rowmax = (int32_t*)c->rowmax.p;
-    int2* srow = (int2*)c->srow.p;
-    int2* scol = (int2*)c->scol.p;
     const Score S{sc->match, sc->mismatch, sc->gap_open, sc->gap_extend};
     SweepParams p{};
     p.sc = S;
@@ -379,7 +412,7 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
         wps = eo.tile_cols > 0 || packed ? 1 : choose_wps(c, max_m, max_n);
         W = tile_cols(c, max_n, eo.tile_cols, packed ? 2 : wps);
     }
-    {  // symbol codes for the int32 sweeps, from the bytes of the pairs (ranges of <= 1 MiB)
+    {  // symbol codes for the int32 sweeps, from the bytes of the pairs
         std::vector<ByteRange> br;
         for (int64_t k : live) {
             const Job& j = jobs[k];
@@ -392,7 +425,10 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
     int64_t launches = 0;
     int rc = run_sweep(c, kStage1, p, regions, W, 0, c->ev[0], c->ev[1], wps, &launches, 0, packed);
     if (rc) return rc;
-    for (int64_t k : live) out[k].precision_used = packed ? 16 : 32;
+    for (int64_t k : live) {
+        out[k].precision_used = packed ? 16 : 32;
+        out[k].score_codes = p.code_shift1 != 0;
+    }
 
     // ---- argmax of the row maxima, per pair ---------------------------------------
     std::vector<ArgmaxJob> aj;  // source of an asynchronous copy: lives until the sync below
@@ -728,6 +764,7 @@ EngineOpts engine_opts(const swh_opts* o, bool batch) {
     eo.wps = o && (o->waves_per_simd == 1 || o->waves_per_simd == 2) ? o->waves_per_simd : 0;
     eo.want_xp = o && o->want_xpoints;
     eo.batch = batch;
+    eo.host_ckpt = o && o->host_checkpoints;
     eo.lookahead = batch ? 0 : 3;
     if (o && o->tb_lookahead != 0) eo.lookahead = std::max(0, o->tb_lookahead);
     return eo;
@@ -820,6 +857,8 @@ int swh_ctx_destroy(swh_ctx* c) {
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->staging) (void)hipHostFree(c->staging);
     if (c->host_ops) (void)hipHostFree(c->host_ops);
+    for (HostBuf* b : {&c->hsrow, &c->hscol})
+        if (b->h) (void)hipHostFree(b->h);
     delete c;
     return SWH_OK;
 }
@@ -966,7 +1005,7 @@ int tune_jobs(swh_ctx* c, const std::vector<Job>& jobs, const swh_scoring* sc, c
     for (int64_t k = 0; k < sample; ++k) base.push_back(jobs[(size_t)(k * n / sample)]);
     const int Ws[] = {256, 384, 512, 768, 1024};
     const int Ps[] = {1, 2};
-    const int64_t Ks[] = {512, 1024, 2048};
+    const int64_t Ks[] = {256, 512, 1024, 2048};
     std::vector<swh_result> res((size_t)sample);
     double best = 1e300;
     *out = swh_batch_tuning{};

@@ -16,6 +16,13 @@ struct DevBuf {
     size_t cap = 0;
 };
 
+// A grow-only pinned host allocation mapped into the device's address space.
+struct HostBuf {
+    void* h = nullptr;  // host address
+    void* d = nullptr;  // device address of the same memory
+    size_t cap = 0;
+};
+
 struct swh_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -26,6 +33,7 @@ struct swh_ctx {
     size_t staging_cap = 0;
     uint8_t* host_ops = nullptr;  // pinned host copy of the traceback ops
     size_t host_ops_cap = 0;
+    HostBuf hsrow, hscol;         // host-resident checkpoint rows / columns (swh_opts.host_checkpoints)
     hipEvent_t ev[8] = {};
     size_t lds_pad[3] = {0, 0, 0};  // dynamic LDS per sweep workgroup for w workgroups per CU
     int wps_fixed = 0;              // sweep waves per SIMD forced by the environment, 0 = model
@@ -44,6 +52,7 @@ int map_hip(swh_ctx* c, hipError_t e);
     } while (0)
 
 hipError_t ensure(DevBuf& b, size_t bytes);
+hipError_t ensure_host(HostBuf& b, size_t bytes);
 int64_t ceil_div(int64_t a, int64_t b);
 int64_t next_pow2(int64_t v);
 bool scoring_ok(const swh_scoring* s);

@@ -694,10 +694,13 @@ __global__ __launch_bounds__(256) void k_argmax_rows(const ArgmaxJob* jobs) {
 // a window of kWalkSteps consecutive steps of one tile is one contiguous, coalesced read:
 // it covers the tile's whole stripe of 64R rows along a band of anti-diagonals, which the
 // path (moving up and left, so to lower steps) leaves only by its top, its left edge or
-// the window's first step.  Windows are staged in LDS with the sequence bytes they need;
-// lane 0 walks.  The walk stops at the start of the alignment or at a cell outside every
+// the window's first step.  Windows are staged in LDS with the sequence bytes they need,
+// by all kWalkThreads threads with every load in flight at once (the codes were written
+// by the previous launch and are read from HBM); wave 0 walks.  The walk stops at the start of the alignment or at a cell outside every
 // region of the round.
 constexpr int kWalkSteps = 256;
+constexpr int kWalkThreads = 256;
+constexpr int kWalkLoads = kWalkSteps * 64 * 2 / 16 / kWalkThreads;  // uint4 loads per thread per window
 // Ops of one window: the path stays in the window's 64R rows and kWalkSteps + 63 columns.
 constexpr int kWalkOps = 64 * kRowsPerLane + kWalkSteps + 64;
 
@@ -734,8 +737,9 @@ template <int R>
 __device__ void walk_main(const WalkArgs& a, WalkLds<R>& lds) {
     constexpr int HR = 64 * R;
     WalkState& sst = lds.sst;
-    const int lane = threadIdx.x;
-    if (lane == 0) sst = *a.st;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    if (tid == 0) sst = *a.st;
     __syncthreads();
     while (true) {
         WalkState st = sst;
@@ -755,21 +759,32 @@ __device__ void walk_main(const WalkArgs& a, WalkLds<R>& lds) {
         const int t0 = t1 >= kWalkSteps ? t1 - kWalkSteps + 1 : 0;
         const uint16_t* tile = rgn.dirs + (s * a.G + sg) * a.tstride * 64;
         {
-            // steps [t0, t1] are contiguous: (t1 - t0 + 1) * 64 codes
+            // steps [t0, t1] are contiguous: (t1 - t0 + 1) * 64 codes, at most
+            // kWalkLoads * kWalkThreads uint4; all loads are issued before the first store
             const uint4* src = (const uint4*)(tile + (int64_t)t0 * 64);
             uint4* dst = (uint4*)lds.win;
             const int n16 = (t1 - t0 + 1) * 64 * 2 / 16;
-            for (int k = lane; k < n16; k += 64) dst[k] = src[k];
+            uint4 v[kWalkLoads];
+#pragma unroll
+            for (int q = 0; q < kWalkLoads; ++q) {
+                const int k = tid + q * kWalkThreads;
+                if (k < n16) v[q] = src[k];
+            }
+#pragma unroll
+            for (int q = 0; q < kWalkLoads; ++q) {
+                const int k = tid + q * kWalkThreads;
+                if (k < n16) dst[k] = v[q];
+            }
         }
         const int64_t rows_here = min((int64_t)HR, rgn.r0 + rgn.rows - R0);
-        for (int k = lane; k < HR; k += 64) lds.a[k] = k < rows_here ? a.A[R0 + k] : 0;
+        for (int k = tid; k < HR; k += kWalkThreads) lds.a[k] = k < rows_here ? a.A[R0 + k] : 0;
         // columns of the band: col = t - l for t in [t0, t1], l in [0, 64): [t0 - 63, t1]
-        for (int k = lane; k < kWalkSteps + 64; k += 64) {
+        for (int k = tid; k < kWalkSteps + 64; k += kWalkThreads) {
             const int cj = t0 - 63 + k;
             lds.b[k] = (cj >= 0 && cj <= t1) ? a.B[C0 + cj] : 0;
         }
         __syncthreads();
-        {
+        if (tid < 64) {
             // The whole wave walks in lockstep on wave-uniform values: each step's code (and
             // whether the pair on the diagonal matches) is read once from LDS and broadcast
             // with readfirstlane, so the loop's control flow is scalar.  Lane 0 alone writes
@@ -856,8 +871,8 @@ __device__ void walk_main(const WalkArgs& a, WalkLds<R>& lds) {
         {
             const int nw = lds.nw;
             uint8_t* dst = a.ops + lds.ops_at;
-            for (int k = lane; k < nw; k += 64) dst[k] = lds.ops[k];
-            if (lane == 0 && lds.nxw && a.srow && sst.nxp < a.xcap) {
+            for (int k = tid; k < nw; k += kWalkThreads) dst[k] = lds.ops[k];
+            if (tid == 0 && lds.nxw && a.srow && sst.nxp < a.xcap) {
                 XPoint xp = lds.xp;
                 const int2 v = a.srow[((xp.row + 1) / a.K - 1) * a.n + xp.col];
                 xp.score = xp.layer == 2 ? v.y : v.x;
@@ -868,11 +883,11 @@ __device__ void walk_main(const WalkArgs& a, WalkLds<R>& lds) {
         __syncthreads();
     }
     __syncthreads();
-    if (lane == 0) *a.st = sst;
+    if (tid == 0) *a.st = sst;
 }
 
 template <int R>
-__global__ __launch_bounds__(64) void k_walk(const WalkArgs* jobs) {
+__global__ __launch_bounds__(kWalkThreads) void k_walk(const WalkArgs* jobs) {
     __shared__ WalkLds<R> lds;
     const WalkArgs a = jobs[blockIdx.x];
     walk_main<R>(a, lds);
@@ -935,7 +950,7 @@ hipError_t launch_alphabet(const ByteRange* ranges, int64_t nranges, uint32_t* m
 
 hipError_t launch_walk(const WalkArgs* jobs, int njobs, hipStream_t s) {
     if (njobs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_walk<kRowsPerLane>, dim3((unsigned)njobs), dim3(64), 0, s, jobs);
+    hipLaunchKernelGGL(k_walk<kRowsPerLane>, dim3((unsigned)njobs), dim3(kWalkThreads), 0, s, jobs);
     return hipGetLastError();
 }
 

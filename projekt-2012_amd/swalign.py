@@ -32,7 +32,7 @@ class Opts(C.Structure):
                 ("hbm_budget_bytes", C.c_int64), ("score_only", C.c_int32),
                 ("precision", C.c_int32), ("want_xpoints", C.c_int32),
                 ("tile_cols", C.c_int32), ("tb_lookahead", C.c_int32),
-                ("waves_per_simd", C.c_int32), ("autotune", C.c_int32), ("reserved", C.c_int32 * 1)]
+                ("waves_per_simd", C.c_int32), ("autotune", C.c_int32), ("host_checkpoints", C.c_int32)]
 
 
 class XPoint(C.Structure):
@@ -49,7 +49,8 @@ class Result(C.Structure):
                 ("checkpoint_rows", C.c_int64), ("checkpoint_cols", C.c_int64),
                 ("ms_forward", C.c_double), ("ms_end_search", C.c_double),
                 ("ms_traceback", C.c_double), ("ms_total", C.c_double),
-                ("precision_used", C.c_int32), ("forward_launches", C.c_int32)]
+                ("precision_used", C.c_int32), ("forward_launches", C.c_int32),
+                ("score_codes", C.c_int32), ("reserved0", C.c_int32)]
 
 
 class Pair(C.Structure):
@@ -131,7 +132,7 @@ def _result_dict(r):
              checkpoint_rows=r.checkpoint_rows, checkpoint_cols=r.checkpoint_cols,
              ms_forward=r.ms_forward, ms_end_search=r.ms_end_search, ms_traceback=r.ms_traceback,
              ms_total=r.ms_total, precision=r.precision_used,
-             forward_launches=r.forward_launches)
+             forward_launches=r.forward_launches, score_codes=r.score_codes)
     if r.xpoints and r.n_xpoints:
         d["xpoints"] = [(r.xpoints[k].row, r.xpoints[k].col, r.xpoints[k].score, r.xpoints[k].layer)
                         for k in range(r.n_xpoints)]
@@ -141,8 +142,9 @@ def _result_dict(r):
 
 
 def make_opts(K=0, Kc=0, score_only=False, xpoints=False, budget=0, precision=0, tile_cols=0, lookahead=0,
-              waves_per_simd=0, autotune=False):
+              waves_per_simd=0, autotune=False, host_checkpoints=False):
     o = Opts()
+    o.host_checkpoints = 1 if host_checkpoints else 0
     o.tile_cols = tile_cols
     o.waves_per_simd = waves_per_simd
     o.autotune = 1 if autotune else 0

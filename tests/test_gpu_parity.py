@@ -179,7 +179,10 @@ def test_symbol_codes_and_byte_compare(sw, orc, ctx, name, score):
     SWALIGN_SYMBOL_CODES=0 gives the same results."""
     a, b = _alphabet_pair(name, 2600, len(name) + score[0])
     sc = sw.scoring(*score)
-    got = ctx.align(a, b, sc, K=512, Kc=256, xpoints=True, tile_cols=96)
+    got = ctx.align(a, b, sc, K=512, Kc=256, xpoints=True, tile_cols=96, precision=32)
+    # codes separate ACGT, ACGTN and AC; not upper and lower case, nor all bytes; the
+    # profile bytes hold match - mismatch, which must fit a byte
+    assert got["score_codes"] == int(name in ("acgt", "acgtn", "ac") and score[0] - score[1] <= 255)
     A, B = orient(a, b)
     ref = orc.align_linear(A, B, orc.scoring(*score), 512, 256)
     _cmp(got, ref)
@@ -190,8 +193,32 @@ def test_symbol_codes_and_byte_compare(sw, orc, ctx, name, score):
     finally:
         del os.environ["SWALIGN_SYMBOL_CODES"]
     try:
-        r2 = plain.align(a, b, sc, K=512, Kc=256, xpoints=True, tile_cols=96)
+        r2 = plain.align(a, b, sc, K=512, Kc=256, xpoints=True, tile_cols=96, precision=32)
     finally:
         plain.close()
+    assert r2["score_codes"] == 0
     for k in ("score", "end", "start", "cigar", "xpoints"):
         assert r2[k] == got[k], k
+
+
+@pytest.mark.parametrize("batch", [False, True])
+def test_host_checkpoints_same_result(sw, orc, ctx, batch):
+    """Checkpoint rows and columns in mapped pinned host memory (swh_opts.host_checkpoints)
+    give the alignment, crosspoints included, that HBM-resident checkpoints give."""
+    sc = sw.scoring(1, -3, -5, -2)
+    if not batch:
+        a, b = sw.synth_pair(77, 4100, 3900, 2500, 0.1, 0.02)
+        r0 = ctx.align(a, b, sc, K=512, Kc=256, xpoints=True, tile_cols=160)
+        r1 = ctx.align(a, b, sc, K=512, Kc=256, xpoints=True, tile_cols=160, host_checkpoints=True)
+        for k in ("score", "end", "start", "cigar", "xpoints", "checkpoint_rows", "checkpoint_cols"):
+            assert r0[k] == r1[k], k
+        A, B = orient(a, b)
+        _cmp(r1, orc.align_linear(A, B, orc.scoring(1, -3, -5, -2), 512, 256))
+    else:
+        pairs = [sw.synth_pair(300 + k, 200 + 97 * k, 180 + 88 * k, 100 + 40 * k, 0.1, 0.02) for k in range(24)]
+        q0, s0 = ctx.batch(pairs, sc, K=256, Kc=256, xpoints=True)
+        q1, s1 = ctx.batch(pairs, sc, K=256, Kc=256, xpoints=True, host_checkpoints=True)
+        assert s0 == s1 == [0] * len(pairs)
+        for x, y in zip(q0, q1):
+            for k in ("score", "end", "start", "cigar", "xpoints"):
+                assert x[k] == y[k], k
