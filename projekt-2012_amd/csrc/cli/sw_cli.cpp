@@ -10,6 +10,11 @@
 //   --visual               print the alignment in 50-column blocks
 //   --gaps PREFIX          write PREFIX.first.gaps / PREFIX.second.gaps gap lists
 //   --json                 print one JSON object with the result after the text
+//   --host-checkpoints     keep checkpoint rows and columns in pinned host memory mapped
+//                          into the device instead of HBM
+//   --dump-rows DIR        write every checkpoint row as DIR/row_R in the reference's
+//                          temp/row_R record format (src/Builders.cpp:27-104): the {int32 H,
+//                          int32 F} pairs of the first R rows' last row, in column order
 //   --device N             HIP device (default 0)
 // Exit status: 255 for input errors, 254 for device errors (the reference's -1 / -2).
 #include <algorithm>
@@ -46,7 +51,8 @@ int main(int argc, char** argv) {
     printf("### Welcome to SWalign v%s\n\n", kVersion);
 
     std::vector<const char*> pos;
-    bool score_only = false, no_cigar = false, visual = false, json = false;
+    bool score_only = false, no_cigar = false, visual = false, json = false, host_ck = false;
+    const char* dump_dir = nullptr;
     int64_t K = 0, Kc = 0;
     int device = 0;
     const char* gaps_prefix = nullptr;
@@ -57,6 +63,8 @@ int main(int argc, char** argv) {
             else if (!strcmp(a, "--no-cigar")) no_cigar = true;
             else if (!strcmp(a, "--visual")) visual = true;
             else if (!strcmp(a, "--json")) json = true;
+            else if (!strcmp(a, "--host-checkpoints")) host_ck = true;
+            else if (!strcmp(a, "--dump-rows") && k + 1 < argc) dump_dir = argv[++k];
             else if (!strcmp(a, "--checkpoint-rows") && k + 1 < argc) K = atoll(argv[++k]);
             else if (!strcmp(a, "--checkpoint-cols") && k + 1 < argc) Kc = atoll(argv[++k]);
             else if (!strcmp(a, "--gaps") && k + 1 < argc) gaps_prefix = argv[++k];
@@ -125,6 +133,7 @@ int main(int argc, char** argv) {
     opts.checkpoint_cols = Kc;
     opts.score_only = score_only ? 1 : 0;
     opts.want_xpoints = 1;
+    opts.host_checkpoints = host_ck ? 1 : 0;
     swh_result res{};
     const double t_before = seconds_since(t_start);
     rc = swh_align_pair(ctx, first.seq.data(), (int64_t)first.seq.size(), second.seq.data(),
@@ -171,6 +180,26 @@ int main(int argc, char** argv) {
             if (!write_gap_list(p1.c_str(), ga) || !write_gap_list(p2.c_str(), gb))
                 exit_with_msg("Error writing gap lists.", -1);
         }
+    }
+    if (dump_dir && !score_only && res.checkpoint_rows > 0 && res.m / res.checkpoint_rows > 0) {
+        // the checkpoint rows of the same spacing, recomputed by a forward pass and written as
+        // the reference's stage 1 left them in temp/ (file row_R: matrix row R-1, n records)
+        const int64_t K = res.checkpoint_rows, Kc = res.checkpoint_cols > 0 ? res.checkpoint_cols : K;
+        const int64_t nrows = res.m / K, n = res.n;
+        std::vector<int32_t> rows((size_t)(nrows * n * 2));
+        int32_t score = 0;
+        int64_t i = 0, j = 0;
+        rc = swh_forward(ctx, A, res.m, B, n, &sc, K, Kc, &score, &i, &j, nullptr, rows.data(), nullptr);
+        if (rc != SWH_OK) exit_with_msg(swh_strerror(rc), -2);
+        for (int64_t q = 0; q < nrows; ++q) {
+            const std::string path = std::string(dump_dir) + "/row_" + std::to_string((q + 1) * K);
+            FILE* f = fopen(path.c_str(), "wb");
+            if (!f || fwrite(rows.data() + q * n * 2, sizeof(int32_t) * 2, (size_t)n, f) != (size_t)n)
+                exit_with_msg("Error writing checkpoint rows.", -1);
+            fclose(f);
+        }
+        printf("\t>Checkpoint rows written: %lld (every %lld rows) to %s\n\n", (long long)nrows, (long long)K,
+               dump_dir);
     }
     if (json) {
         printf("{\"score\": %d, \"end\": [%lld, %lld], \"start\": [%lld, %lld], \"swapped\": %d, "

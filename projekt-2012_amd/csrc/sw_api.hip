@@ -345,6 +345,8 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
         max_n = std::max(max_n, j.n);
     }
     HCK(c, ensure(c->rowmax, sizeof(int32_t) * std::max<int64_t>(rm_elems, 1)));
+    HCK(c, ensure(c->rowseg, sizeof(int32_t) * std::max<int64_t>(rm_elems, 1)));
+    HCK(c, ensure(c->segs, sizeof(int32_t) * std::max<int64_t>(nj, 1)));
     int2* srow;
     int2* scol;
     if (eo.host_ckpt) {
@@ -361,6 +363,8 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
     HCK(c, ensure(c->key, sizeof(unsigned long long) * std::max<int64_t>(nj, 1)));
     HCK(c, ensure(c->found, sizeof(int32_t) * std::max<int64_t>(nj, 1)));
     int32_t* rowmax = (int32_t*)c->rowmax.p;
+    int32_t* rowseg = (int32_t*)c->rowseg.p;  // first stage-1 segment holding each row maximum
+    int32_t* segs = (int32_t*)c->segs.p;
     const Score S{sc->match, sc->mismatch, sc->gap_open, sc->gap_extend};
     SweepParams p{};
     p.sc = S;
@@ -391,6 +395,7 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
         r.c0 = 0;
         r.c1 = j.n - 1;
         r.rowmax = rowmax + j.rowmax_off;
+        r.rowseg = rowseg + j.rowmax_off;
         r.srow = j.pl.nsr ? srow + j.srow_off : nullptr;
         r.K = j.pl.K;
         r.scol = j.pl.nsc ? scol + j.scol_off : nullptr;
@@ -433,15 +438,21 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
     // ---- argmax of the row maxima, per pair ---------------------------------------
     std::vector<ArgmaxJob> aj;  // source of an asynchronous copy: lives until the sync below
     {
-        for (int64_t k : live) aj.push_back(ArgmaxJob{rowmax + jobs[k].rowmax_off, jobs[k].m, (unsigned long long*)c->key.p + k});
+        for (int64_t k : live)
+            aj.push_back(ArgmaxJob{rowmax + jobs[k].rowmax_off, jobs[k].m, (unsigned long long*)c->key.p + k,
+                                   rowseg + jobs[k].rowmax_off, segs + k});
         HCK(c, ensure(c->argjobs, sizeof(ArgmaxJob) * aj.size()));
         HCK(c, hipMemcpyAsync(c->argjobs.p, aj.data(), sizeof(ArgmaxJob) * aj.size(), hipMemcpyHostToDevice, c->stream));
         HCK(c, hipMemsetAsync(c->key.p, 0, sizeof(unsigned long long) * nj, c->stream));
+        HCK(c, hipMemsetAsync(segs, 0xFF, sizeof(int32_t) * nj, c->stream));  // -1: search every block
         const int bpj = (int)std::min<int64_t>(1024, std::max<int64_t>(1, ceil_div(max_m, 16384)));
         HCK(c, launch_argmax_rows((const ArgmaxJob*)c->argjobs.p, (int)aj.size(), bpj, c->stream));
+        HCK(c, launch_end_segment((const ArgmaxJob*)c->argjobs.p, (int)aj.size(), c->stream));
     }
     std::vector<unsigned long long> keys((size_t)nj, 0);
+    std::vector<int32_t> endseg((size_t)nj, -1);
     HCK(c, hipMemcpyAsync(keys.data(), c->key.p, sizeof(unsigned long long) * nj, hipMemcpyDeviceToHost, c->stream));
+    HCK(c, hipMemcpyAsync(endseg.data(), segs, sizeof(int32_t) * nj, hipMemcpyDeviceToHost, c->stream));
     HCK(c, hipStreamSynchronize(c->stream));
     const float ms_fwd = elapsed(c->ev[0], c->ev[1]);
     std::vector<int64_t> scored;
@@ -458,7 +469,9 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
     if (scored.empty()) return SWH_OK;
 
     // ---- end-cell search: per pair, the band above the end row, one region per
-    // checkpoint-column block -------------------------------------------------------
+    // checkpoint-column block.  Only the blocks over the first stage-1 segment (tile
+    // columns [g W, (g+1) W)) holding the end row's maximum are searched: no column left
+    // of it holds the score in that row, and the first one that does lies inside it. ------
     {
         regions.clear();
         int64_t wmax = 0;
@@ -468,7 +481,13 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
             const int64_t nq = ceil_div(j.n, std::min(j.pl.Kc, j.n));
             const int64_t bw = std::min(j.pl.Kc, j.n);
             wmax = std::max(wmax, bw);
-            for (int64_t q = 0; q < nq; ++q) {
+            int64_t q0 = 0, q1 = nq - 1;
+            const int64_t g = endseg[(size_t)k];
+            if (g >= 0 && g * W < j.n) {
+                q0 = (g * W) / bw;
+                q1 = (std::min(j.n, (g + 1) * W) - 1) / bw;
+            }
+            for (int64_t q = q0; q <= q1; ++q) {
                 Region r{};
                 r.A = j.A;
                 r.B = j.B;
@@ -848,7 +867,8 @@ int swh_ctx_destroy(swh_ctx* c) {
     (void)hipSetDevice(c->device);
     DevBuf* bufs[] = {&c->seqA, &c->seqB, &c->rbuf, &c->cb, &c->cor, &c->rowmax, &c->srow, &c->scol,
                       &c->dirs, &c->regions, &c->key, &c->found, &c->walk, &c->ops, &c->xps, &c->wregs,
-                      &c->argjobs, &c->wargs, &c->seqs, &c->spans, &c->alpha, &c->ranges};
+                      &c->argjobs, &c->wargs, &c->seqs, &c->spans, &c->alpha, &c->ranges, &c->rowseg,
+                      &c->segs};
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);

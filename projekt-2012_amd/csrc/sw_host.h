@@ -28,7 +28,7 @@ struct swh_ctx {
     hipStream_t stream = nullptr;
     hipDeviceProp_t props{};
     DevBuf seqA, seqB, rbuf, cb, cor, rowmax, srow, scol, dirs, regions, key, found, walk, ops, xps, wregs,
-        argjobs, wargs, seqs, spans, alpha, ranges;
+        argjobs, wargs, seqs, spans, alpha, ranges, rowseg, segs;
     uint8_t* staging = nullptr;  // pinned host staging of batch inputs
     size_t staging_cap = 0;
     uint8_t* host_ops = nullptr;  // pinned host copy of the traceback ops

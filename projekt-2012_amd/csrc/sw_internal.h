@@ -46,6 +46,7 @@ struct Region {
     int32_t* cor;      // 2 x stripes: H(R0-1, C0-1) of the next segment, kept in slot (g+1)&1
     // kStage1 outputs (nullable checkpoints)
     int32_t* rowmax;   // m row maxima
+    int32_t* rowseg;   // nullable: m first segments (tile columns) holding each row maximum
     int2* srow;        // (m/K) x n {H,F}: rows K-1, 2K-1, ...
     int64_t K;         // multiple of kStripeRows
     int2* scol;        // (n/Kc) x m {H,E}: columns Kc-1, 2Kc-1, ...
@@ -131,8 +132,12 @@ struct ArgmaxJob {
     const int32_t* rowmax;
     int64_t m;
     unsigned long long* key;
+    const int32_t* rowseg;  // nullable: row -> first segment holding the row maximum
+    int32_t* seg;           // receives rowseg[end row] (with rowseg)
 };
 hipError_t launch_argmax_rows(const ArgmaxJob* jobs, int njobs, int blocks_per_job, hipStream_t s);
+// After launch_argmax_rows on the same stream: seg = rowseg[end row] for jobs with rowseg.
+hipError_t launch_end_segment(const ArgmaxJob* jobs, int njobs, hipStream_t s);
 // A byte range in device memory.
 struct ByteRange {
     const uint8_t* p;

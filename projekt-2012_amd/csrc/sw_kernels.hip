@@ -66,6 +66,8 @@ struct TileCtx {
     int2* scol;         // stage 1: checkpoint columns (nullable), every Kc columns
     int64_t Kc;
     int32_t* rowmax;    // stage 1: row maxima
+    int32_t* rowseg;    // stage 1 (nullable): first segment holding each row maximum
+    int32_t seg;        // segment of the tile within its region
     int64_t target_row; // kSearch
     int32_t target;
     const int2* top;    // {H,F} of row R0-1 at columns C0.. (index col-C0); nullptr = zero row
@@ -302,10 +304,15 @@ __device__ void sweep_tile(const SweepParams& p, const TileCtx& T, const int2* l
             if (row0 + k <= T.r1) right[row0 + k - right_r0] = make_int2(Ho[k] - O, E[k]);
     }
     if (MODE == kStage1) {
+        // segments of a stripe run in increasing order (one launch each), so a strictly
+        // larger value keeps the first segment that holds the row maximum
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             const int64_t row = row0 + k;
-            if (row <= T.r1) T.rowmax[row] = first_segment ? rmax[k] : max(T.rowmax[row], rmax[k]);
+            if (row <= T.r1 && (first_segment || rmax[k] > T.rowmax[row])) {
+                T.rowmax[row] = rmax[k];
+                if (T.rowseg) T.rowseg[row] = T.seg;
+            }
         }
     }
     if (MODE == kSearch) {
@@ -355,6 +362,8 @@ __device__ bool make_tile(const SweepParams& p, int64_t x, TileSetup& ts) {
     T.scol = Rg.scol;
     T.Kc = Rg.Kc;
     T.rowmax = Rg.rowmax;
+    T.rowseg = Rg.rowseg;
+    T.seg = (int32_t)g;
     T.target_row = Rg.target_row;
     T.target = Rg.target;
     T.R0 = Rg.r0 + s * HR;
@@ -609,7 +618,10 @@ __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], Sw
             if (row > T[t].r1) continue;
             if (ts[t].right) ts[t].right[row - ts[t].right_r0] = make_int2(Ho[k][t] - O, E[k][t]);
             const int r = rmax[k][t];
-            T[t].rowmax[row] = ts[t].first_segment ? r : max(T[t].rowmax[row], r);
+            if (ts[t].first_segment || r > T[t].rowmax[row]) {  // first segment holding the maximum
+                T[t].rowmax[row] = r;
+                if (T[t].rowseg) T[t].rowseg[row] = T[t].seg;
+            }
         }
     }
 }
@@ -630,6 +642,7 @@ __device__ void sweep_main_pk(const SweepParams& p, SweepLds& lds) {
         ts[1].T.bot = nullptr;
         ts[1].T.srow = nullptr;
         ts[1].T.scol = nullptr;
+        ts[1].T.rowseg = nullptr;
         ts[1].left = nullptr;
         ts[1].right = nullptr;
     }
@@ -687,6 +700,18 @@ __global__ __launch_bounds__(256) void k_alphabet(const ByteRange* ranges, int64
 __global__ __launch_bounds__(256) void k_argmax_rows(const ArgmaxJob* jobs) {
     const ArgmaxJob j = jobs[blockIdx.y];
     argmax_rows_main(j);
+}
+
+// seg = rowseg[end row] of each job whose maximum is positive (one thread per job).
+__global__ __launch_bounds__(256) void k_end_segment(const ArgmaxJob* jobs, int njobs) {
+    const int x = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (x >= njobs) return;
+    const ArgmaxJob j = jobs[x];
+    if (!j.rowseg || !j.seg) return;
+    const unsigned long long key = *j.key;
+    if (!key) return;
+    const int64_t row = (int64_t)(0xFFFFFFFFu - (uint32_t)(key & 0xFFFFFFFFu));
+    if (row >= 0 && row < j.m) *j.seg = j.rowseg[row];
 }
 
 // Greedy backward walk over the regions of one traceback round (see WalkArgs).  The codes
@@ -805,6 +830,58 @@ __device__ void walk_main(const WalkArgs& a, WalkLds<R>& lds) {
                 }
                 const int l = ri / R, t = cj + l;
                 if (t < t0) break;  // left the window
+                if (layer == 0) {
+                    // Diagonal run: lane k reads the code of cell (ri - k, cj - k); the moves
+                    // up to the first lane whose cell is not a diagonal move (or lies outside
+                    // the tile or the window) are taken at once, one op per lane.
+                    const int rik = ri - lane, cjk = cj - lane;
+                    const int lk = rik >= 0 ? rik / R : 0, tk = cjk + lk;
+                    const bool in = rik >= 0 && cjk >= 0 && tk >= t0;
+                    uint32_t ck = 0;
+                    bool eqk = false;
+                    if (in) {
+                        ck = (lds.win[(tk - t0) * 64 + lk] >> (4 * (rik % R))) & 3u;
+                        eqk = lds.a[rik] == lds.b[cjk - bofs];
+                    }
+                    const uint64_t stop = __ballot(!(in && ck == 1u));
+                    const int run = stop ? (int)__builtin_ctzll(stop) : 64;
+                    if (run > 0) {
+                        if (lane < run) lds.ops[nw + lane] = eqk ? (uint8_t)'=' : (uint8_t)'X';
+                        // checkpoint-row bookkeeping of the rows i, i-1, ..., i-run+1, as the
+                        // single steps below would do it (xrow <= i holds after row i)
+                        const int64_t i = R0 + ri, ilast = i - (run - 1);
+                        if (i != last_row) {
+                            if (xrow > i) xrow -= a.K;
+                            if (i == xrow && nxw == 0) {
+                                if (lane == 0) {
+                                    lds.xp.row = i;
+                                    lds.xp.col = C0 + cj;
+                                    lds.xp.layer = 0;
+                                }
+                                nxw = 1;
+                            }
+                        }
+                        if (run > 1) {
+                            if (xrow > i - 1) xrow -= a.K;
+                            if (xrow >= ilast && nxw == 0) {
+                                if (lane == 0) {
+                                    lds.xp.row = xrow;
+                                    lds.xp.col = C0 + cj - (i - xrow);
+                                    lds.xp.layer = 0;
+                                }
+                                nxw = 1;
+                            }
+                            if (xrow > ilast) xrow -= a.K;
+                        }
+                        last_row = ilast;
+                        si = ilast;
+                        sj = C0 + cj - (run - 1);
+                        nw += run;
+                        ri -= run;
+                        cj -= run;
+                        continue;
+                    }
+                }
                 const uint32_t word = lds.win[(t - t0) * 64 + l];
                 const uint32_t eq = lds.a[ri] == lds.b[cj - bofs] ? 16u : 0u;
                 const uint32_t v = __builtin_amdgcn_readfirstlane(((word >> (4 * (ri % R))) & 15u) | eq);
@@ -938,6 +1015,12 @@ hipError_t launch_argmax_rows(const ArgmaxJob* jobs, int njobs, int blocks_per_j
     if (njobs <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_argmax_rows, dim3((unsigned)(blocks_per_job > 1 ? blocks_per_job : 1), (unsigned)njobs), dim3(256), 0, s,
                        jobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_end_segment(const ArgmaxJob* jobs, int njobs, hipStream_t s) {
+    if (njobs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_end_segment, dim3((unsigned)((njobs + 255) / 256)), dim3(256), 0, s, jobs, njobs);
     return hipGetLastError();
 }
 

@@ -69,3 +69,24 @@ def test_invalid_scoring_after_device_setup():
     p = run(os.path.join(FA, "seq0.fna"), os.path.join(FA, "seq1.fna"), "1", "3", "-5", "-2")
     assert p.returncode == 255
     assert "ERROR\nOne or more scoring values were not usable!" in p.stdout
+
+
+@pytest.mark.gpu
+def test_dump_rows_reference_format(tmp_path, orc):
+    """--dump-rows writes every checkpoint row as row_R, the {int32 H, int32 F} records of the
+    first R rows' last row in column order (the reference's temp/row_R files); with
+    --host-checkpoints the alignment is unchanged."""
+    import numpy as np
+    from conftest import fasta_bytes, orient
+    f1, f2 = os.path.join(FA, "NC_005046.fna"), os.path.join(FA, "NC_012557.fna")
+    p = run(f1, f2, "5", "-5", "-5", "-5", "--checkpoint-rows", "256", "--checkpoint-cols", "256",
+            "--host-checkpoints", "--dump-rows", str(tmp_path))
+    assert p.returncode == 0, p.stdout + p.stderr
+    assert "\t>Alignment score: 1580 at [1389, 1351]" in p.stdout
+    A, B = orient(fasta_bytes("NC_005046.fna"), fasta_bytes("NC_012557.fna"))
+    srow, _ = orc.checkpoints(A, B, orc.scoring(5, -5, -5, -5), 256, 256)
+    files = sorted(os.listdir(tmp_path), key=lambda x: int(x.split("_")[1]))
+    assert files == ["row_%d" % (256 * (q + 1)) for q in range(len(A) // 256)]
+    for q, name in enumerate(files):
+        got = np.fromfile(str(tmp_path / name), dtype=np.int32).reshape(-1, 2)
+        assert np.array_equal(got, np.asarray(srow[q]).reshape(-1, 2)), name
