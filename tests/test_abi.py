@@ -84,3 +84,34 @@ def test_mutation_model(sw):
     assert abs(len(out) - len(src)) < 0.02 * len(src)
     out = sw.synth_mutate(5, src[:10_000], 0.0, 0.0, 0.2)
     assert 0.6 * 10_000 < len(out) < 1.4 * 10_000
+
+
+def test_struct_layouts_match_the_header(sw, tmp_path):
+    """The ctypes mirrors in swalign.py have the size and field offsets of the C structs (a C
+    program built with gcc against include/swalign_hip.h prints them)."""
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        pytest.skip("gcc not available")
+    mirrors = {"swh_scoring": sw.Scoring, "swh_opts": sw.Opts, "swh_xpoint": sw.XPoint, "swh_result": sw.Result,
+               "swh_pair": sw.Pair, "swh_batch_tuning": sw.BatchTuning, "swh_serve_stats": sw.ServeStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "swalign_hip.h"', "int main(void) {"]
+    for cname, py in mirrors.items():
+        lines.append('printf("%s size %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f, _ in py._fields_:
+            if f.startswith("_") or f.startswith("reserved"):
+                continue
+            lines.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (cname, f, cname, f))
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                         check=True).stdout.split("\n") if l)
+    for cname, py in mirrors.items():
+        assert int(out[cname + " size"]) == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            if f.startswith("_") or f.startswith("reserved"):
+                continue
+            assert int(out["%s.%s" % (cname, f)]) == getattr(py, f).offset, (cname, f)
