@@ -66,7 +66,8 @@ typedef struct {
     int32_t tile_cols;         /* columns per sweep tile (any width >= 1);                     */
                                /* 0 = auto (about four tiles per CU across the row, >= 256)   */
     int32_t tb_lookahead;      /* traceback: checkpoint blocks recomputed ahead of the walk in  */
-                               /* each direction per round; 0 = auto (3), -1 = none            */
+                               /* each direction per round (beyond 3, only blocks within 2 of  */
+                               /* the diagonal); 0 = auto (7; batches 0), -1 = none            */
     int32_t waves_per_simd;    /* sweep waves resident per SIMD (1 or 2), 0 = auto             */
     int32_t autotune;          /* batch: 1 = time candidate settings on a sample first          */
     int32_t host_checkpoints;  /* 1: checkpoint rows and columns live in pinned host memory    */

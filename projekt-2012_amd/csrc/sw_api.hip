@@ -310,7 +310,7 @@ struct EngineOpts {
     int64_t tile_cols = 0;   // 0 = auto
     int precision = 0;       // 0 = packed 16-bit stage 1 where it cannot overflow, 32 = int32 only
     int wps = 0;             // sweep waves per SIMD, 0 = auto
-    int lookahead = 3;       // traceback blocks ahead per direction (0 = none)
+    int lookahead = 7;       // traceback blocks ahead per direction (0 = none)
     bool want_xp = false;
     bool batch = false;      // many pairs: fixed tile width, two waves per SIMD
     bool host_ckpt = false;  // checkpoints in mapped pinned host memory
@@ -578,11 +578,11 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
         int64_t L = std::max(0, eo.lookahead);
         int64_t need = 0;
         for (int64_t k : active) need += blk_dirs(jobs[k]);
-        while (L > 0 && (L + 1) * (L + 1) * need > dirs_cap) --L;
+        while (L > 0 && lookahead_blocks(L) * need > dirs_cap) --L;
         std::vector<int64_t> round;
         int64_t used = 0;
         for (int64_t k : active) {
-            const int64_t b = (L + 1) * (L + 1) * blk_dirs(jobs[k]);
+            const int64_t b = lookahead_blocks(L) * blk_dirs(jobs[k]);
             if (!round.empty() && used + b > dirs_cap) break;
             round.push_back(k);
             used += b;
@@ -602,7 +602,7 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
                     const int64_t wi = x * (L + 1) * (L + 1) + da * (L + 1) + db;
                     WalkRegion& w = wregs[(size_t)wi];
                     w = WalkRegion{};
-                    if (brow - da < 0 || bcol - db < 0) continue;
+                    if (brow - da < 0 || bcol - db < 0 || !lookahead_block(L, da, db)) continue;
                     Region r{};
                     r.A = j.A;
                     r.B = j.B;
@@ -784,7 +784,7 @@ EngineOpts engine_opts(const swh_opts* o, bool batch) {
     eo.want_xp = o && o->want_xpoints;
     eo.batch = batch;
     eo.host_ckpt = o && o->host_checkpoints;
-    eo.lookahead = batch ? 0 : 3;
+    eo.lookahead = batch ? 0 : 7;
     if (o && o->tb_lookahead != 0) eo.lookahead = std::max(0, o->tb_lookahead);
     return eo;
 }

@@ -79,6 +79,20 @@ int code_shift1_for(const uint32_t mask[8], const swh_scoring* sc);
 int code_shift1_device(swh_ctx* c, const std::vector<swk::ByteRange>& ranges, const swh_scoring* sc,
                        int* shift1);
 
+// Traceback lookahead: block (da, db) above / left of the current cell's checkpoint block is
+// recomputed in a round with lookahead L if da, db <= L and, beyond L = 3, |da - db| <= 2 (a
+// band along the diagonal, where paths run; a path that leaves it is picked up next round).
+inline bool lookahead_block(int64_t L, int64_t da, int64_t db) {
+    return da <= L && db <= L && (L <= 3 || (da > db ? da - db : db - da) <= 2);
+}
+// Blocks per round with lookahead L.
+inline int64_t lookahead_blocks(int64_t L) {
+    int64_t n = 0;
+    for (int64_t da = 0; da <= L; ++da)
+        for (int64_t db = 0; db <= L; ++db) n += lookahead_block(L, da, db);
+    return n;
+}
+
 struct Plan {
     int64_t K, Kc, nsr, nsc;
 };

@@ -187,10 +187,11 @@ void layout(swh_server* s, SJob& j) {
     const int64_t tstride = ceil_div(W + 63, 64) * 64, G = ceil_div(bw, W);
     const int64_t dirs = ceil_div(bh, kStripeRows) * G * tstride * 64 * 2;
     const int64_t blk = al(dirs) + al(16 * bw) + al(16 * bh) + al(8 * ceil_div(bh, kStripeRows));
-    // lookahead: up to 3 blocks in each direction when the job has several, within 1 GiB
-    j.L = (j.pl.nsr > 0 || j.pl.nsc > 0) ? 3 : 0;
-    while (j.L > 0 && (j.L + 1) * (j.L + 1) * blk > (INT64_C(1) << 30)) --j.L;
-    const int64_t tb = (j.L + 1) * (j.L + 1) * blk;
+    // lookahead: up to 7 blocks in each direction (a band along the diagonal beyond 3) when
+    // the job has several, within 1 GiB
+    j.L = (j.pl.nsr > 0 || j.pl.nsc > 0) ? 7 : 0;
+    while (j.L > 0 && lookahead_blocks(j.L) * blk > (INT64_C(1) << 30)) --j.L;
+    const int64_t tb = lookahead_blocks(j.L) * blk;
     j.work_bytes = std::max({stage1, search, tb});
     j.bytes = persist + j.work_bytes;
 }
@@ -305,7 +306,7 @@ void setup_dirs(swh_server* s, SJob& j) {
     j.wtab_reg.assign(j.wtab.size(), -1);
     for (int64_t da = 0; da <= L; ++da) {
         for (int64_t db = 0; db <= L; ++db) {
-            if (brow - da < 0 || bcol - db < 0) continue;
+            if (brow - da < 0 || bcol - db < 0 || !lookahead_block(L, da, db)) continue;
             Region r = base_region(j);
             r.r0 = (brow - da) * j.pl.K;
             r.r1 = da == 0 ? j.st.i : std::min(j.m, r.r0 + j.pl.K) - 1;

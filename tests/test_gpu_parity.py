@@ -120,7 +120,7 @@ def test_known_answers_large(sw, orc, ctx, name):
     assert s == got["score"] and end == got["end"]
 
 
-@pytest.mark.parametrize("W,L", [(64, 0), (128, -1), (448, 1), (37, 2), (1, 3)])
+@pytest.mark.parametrize("W,L", [(64, 0), (128, -1), (448, 1), (37, 2), (1, 3), (96, 5), (160, 7)])
 def test_tile_width_shapes(sw, orc, ctx, W, L):
     """Tiles narrower than the matrix: hand-offs of bottom rows, right columns and corners
     between tiles (ragged last tile, ragged last stripe); traceback rounds with L blocks
@@ -222,3 +222,16 @@ def test_host_checkpoints_same_result(sw, orc, ctx, batch):
         for x, y in zip(q0, q1):
             for k in ("score", "end", "start", "cigar", "xpoints"):
                 assert x[k] == y[k], k
+
+
+def test_lookahead_band_with_oblong_blocks(sw, orc, ctx):
+    """Blocks of 256 rows by 64 columns put a diagonal path four block columns per block row,
+    so it keeps leaving the lookahead band (|da - db| <= 2 beyond L = 3) and is picked up by
+    the next round; the alignment and crosspoints are unchanged."""
+    (a, b), sc = sw.synth_pair(4242, 3000, 2800, 2000, 0.1, 0.02), sw.scoring(1, -3, -5, -2)
+    A, B = orient(a, b)
+    ref = orc.align_linear(A, B, orc.scoring(1, -3, -5, -2), 256, 64)
+    for L in (7, 4, 3):
+        got = ctx.align(a, b, sc, K=256, Kc=64, xpoints=True, lookahead=L)
+        _cmp(got, ref)
+        assert got["xpoints"] == [tuple(x) for x in ref["xpoints"]], L
