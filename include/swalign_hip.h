@@ -61,7 +61,9 @@ typedef struct {
     int64_t checkpoint_cols;   /* Kc: checkpoint column spacing (power of two >= 64), 0 = K */
     int64_t hbm_budget_bytes;  /* cap on checkpoint memory, 0 = half of free HBM            */
     int32_t score_only;        /* 1: score + end cell only (no traceback)                   */
-    int32_t precision;         /* 0 = auto, 32 = int32; batch path: 16 = packed int16 first  */
+    int32_t precision;         /* stage 1: 16 = packed int16 (values relative to each tile's  */
+                               /* base) where a tile's range fits int16, else int32; 32 = int32;*/
+                               /* 0 = 16 for batches, 32 for single pairs (faster there)       */
     int32_t want_xpoints;      /* 1: return the path nodes on checkpoint rows                */
     int32_t tile_cols;         /* columns per sweep tile (any width >= 1);                     */
                                /* 0 = auto (about four tiles per CU across the row, >= 256)   */

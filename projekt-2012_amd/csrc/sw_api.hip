@@ -243,10 +243,13 @@ int code_shift1_device(swh_ctx* c, const std::vector<ByteRange>& ranges, const s
     return SWH_OK;
 }
 
-bool fits_int16(int64_t m, int64_t n, const swh_scoring* sc) {
+int64_t int16_tile_cols(const swh_scoring* sc) {
     const int64_t M = sc->match, X = -(int64_t)sc->mismatch, O = -(int64_t)sc->gap_open, E = -(int64_t)sc->gap_extend;
-    if (M > 1000 || X > 1000 || O > 1000 || E > 1000) return false;
-    return M * std::min(m, n) + M + O + X + E <= 32000;
+    if (M > 1000 || X > 1000 || O > 1000 || E > 1000) return 0;
+    // H moves by at most M + O between neighbouring cells: a tile of 256 rows by W columns and
+    // its inputs lie within (256 + W + 2) (M + O) of the tile's base, E and F up to O + E below
+    const int64_t w = (32000 - (M + O + X + E)) / (M + O) - (kStripeRows + 2);
+    return w > 0 ? w : 0;
 }
 
 int64_t plan_mem(int64_t m, int64_t n, int64_t K, int64_t Kc) {
@@ -405,17 +408,34 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
     }
     if (live.empty()) return SWH_OK;
     // packed 16-bit stage 1 when no pair of the sweep can leave int16 (two tiles per wave)
-    bool packed = eo.precision != 32;
-    for (int64_t k : live) packed = packed && fits_int16(jobs[k].m, jobs[k].n, sc);
+    // packed 16-bit stage 1 when every tile's values fit int16 around the tile's base: by
+    // default for batches; single pairs only on request (precision 16), since on MI355X the
+    // int32 sweep of one pair is as fast or faster (one wave per SIMD, wider tiles, shorter
+    // wavefront ramp: 715 vs 772 ms at 1 Mbp, 35.5 vs 45.8 s at 10 Mbp)
+    const int64_t w16 = int16_tile_cols(sc);
+    bool packed = (eo.precision == 16 || (eo.precision == 0 && eo.batch)) && w16 > 0;
     int wps = 1;
     int64_t W = 0;
     if (eo.batch) {
         wps = eo.wps ? eo.wps : (c->wps_fixed ? c->wps_fixed : (c->lds_pad[2] ? 2 : 1));
         if (wps == 2 && !c->lds_pad[2]) wps = 1;
         W = eo.tile_cols > 0 ? eo.tile_cols : kBatchTileCols;
+        packed = packed && W <= w16;
     } else {
-        wps = eo.tile_cols > 0 || packed ? 1 : choose_wps(c, max_m, max_n);
-        W = tile_cols(c, max_n, eo.tile_cols, packed ? 2 : wps);
+        if (packed) {  // two tiles per wave, one wave per SIMD
+            W = tile_cols(c, max_n, eo.tile_cols, 2);
+            if (W > w16 && eo.tile_cols > 0) {
+                packed = false;  // the requested width does not fit int16
+            } else if (W > w16) {
+                W = w16;  // narrower tiles: more of them per launch, two waves per SIMD if needed
+                const int64_t slots = (int64_t)kSweepWaves * std::max(1, c->props.multiProcessorCount);
+                if (ceil_div(max_n, W) > 2 * slots && c->lds_pad[2]) wps = 2;
+            }
+        }
+        if (!packed) {
+            wps = eo.tile_cols > 0 ? 1 : choose_wps(c, max_m, max_n);
+            W = tile_cols(c, max_n, eo.tile_cols, wps);
+        }
     }
     {  // symbol codes for the int32 sweeps, from the bytes of the pairs
         std::vector<ByteRange> br;

@@ -68,8 +68,9 @@ int run_sweep(swh_ctx* c, int mode, swk::SweepParams p, std::vector<swk::Region>
               int64_t G_fix = 0, hipEvent_t ev_before = nullptr, hipEvent_t ev_after = nullptr, int wps = 1,
               int64_t* launches = nullptr, int64_t tstride = 0, bool packed = false);
 
-// True if a pair of m x n under scoring sc cannot leave int16 in the packed stage 1.
-bool fits_int16(int64_t m, int64_t n, const swh_scoring* sc);
+// Widest stage-1 tile (columns) whose values stay in int16 relative to the tile's base under
+// scoring sc, for the packed stage 1; 0 if none.
+int64_t int16_tile_cols(const swh_scoring* sc);
 
 // Symbol codes for the sweeps' scores (SweepParams::code_shift1): the smallest shift s in
 // [0, 5] under which (byte >> s) & 7 is one-to-one on the bytes present (mask: bit x of

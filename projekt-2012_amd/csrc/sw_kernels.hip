@@ -413,9 +413,14 @@ __device__ void sweep_main(const SweepParams& p, SweepLds& lds) {
 // ---- packed 16-bit stage 1: two tiles per wave --------------------------------------
 // The low 16 bits of every value belong to tile 0 of the wave and the high 16 bits to tile
 // 1 (any two tiles of the launch: tiles of one anti-diagonal are independent), so one
-// packed operation (v_pk_*) serves two cells.  Used only when no value can leave int16:
-// every H is at most match * min(m, n), every E, F at least gap_open, and the host checks
-// that these bounds, plus the scoring values, fit.  Inputs and outputs in memory stay int32.
+// packed operation (v_pk_*) serves two cells.  Values are held relative to a base per
+// tile, the H of the cell above and left of the tile's first cell: H changes by at most
+// match - gap_open between neighbouring cells (in either direction), so every H, E, F of a
+// tile and of its inputs lies within (64R + W + 2) * (match - gap_open) + |gap_open| +
+// |gap_extend| of the base, and the host runs this kernel only when that fits int16 (the
+// absolute scores may be far larger).  The zero floor of H becomes max(-base, -32768): when
+// the base exceeds 32768 no cell of the tile is near zero.  Inputs and outputs in memory
+// stay int32 and absolute.
 typedef short s16x2 __attribute__((vector_size(4)));
 
 __device__ __forceinline__ int pk_bits(s16x2 v) { return __builtin_bit_cast(int, v); }
@@ -438,9 +443,10 @@ template <int R, bool EDGE, bool DUMP>
 __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileCtx (&T)[2], int64_t c,
                                                const int64_t (&row0)[2], const int (&a)[R], s16x2 (&Ho)[R],
                                                s16x2 (&E)[R], s16x2 (&rmax)[R], s16x2& Ho_up_prev, s16x2& Ho_bot,
-                                               s16x2& Fe_bot, int& Qh, int& Qf, int& Qb, int& bl, int2* s_out) {
+                                               s16x2& Fe_bot, int& Qh, int& Qf, int& Qb, int& bl, int2* s_out,
+                                               const s16x2 zero, const int (&base)[2]) {
     const int lane = wave_lane();
-    const s16x2 vO = pk2(p.sc.O, p.sc.O), vE = pk2(p.sc.E, p.sc.E), zero = pk2(0, 0), one = pk2(1, 1);
+    const s16x2 vO = pk2(p.sc.O, p.sc.O), vE = pk2(p.sc.E, p.sc.E), one = pk2(1, 1);
     const s16x2 vMO = pk2(p.sc.M - p.sc.O, p.sc.M - p.sc.O);
     const s16x2 vD = pk2(p.sc.M - p.sc.X, p.sc.M - p.sc.X);  // match - mismatch
     SWK_UNROLL(UNROLL_STEPS)
@@ -493,7 +499,8 @@ __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileC
                     int2* dst = T[t].scol + ((X + 1) / T[t].Kc - 1) * T[t].m;
 #pragma unroll
                     for (int k = 0; k < R; ++k)
-                        if (row0[t] + k <= T[t].r1) dst[row0[t] + k] = make_int2(Ho[k][t] - p.sc.O, E[k][t]);
+                        if (row0[t] + k <= T[t].r1)
+                            dst[row0[t] + k] = make_int2(Ho[k][t] - p.sc.O + base[t], E[k][t] + base[t]);
                 }
             }
         }
@@ -509,6 +516,9 @@ __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], Sw
     const int O = p.sc.O, Ev = p.sc.E;
     TileCtx T[2] = {ts[0].T, ts[1].T};
     const int64_t row0[2] = {T[0].R0 + (int64_t)lane * R, T[1].R0 + (int64_t)lane * R};
+    const int base[2] = {ts[0].corner, ts[1].corner};  // every value is held as value - base
+    const int floor16[2] = {max(-base[0], -32768), max(-base[1], -32768)};
+    const s16x2 zero = pk2(floor16[0], floor16[1]);  // H's zero floor, relative
     int a[R];
     s16x2 Ho[R], E[R], rmax[R];
 #pragma unroll
@@ -521,24 +531,25 @@ __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], Sw
             av[t] = in ? (int)T[t].A[row] : 0x200;
             if (ts[t].left && in) {
                 const int2 v = ts[t].left[row - ts[t].left_r0];
-                hv[t] = v.x + O;
-                ev[t] = v.y;
+                hv[t] = v.x + O - base[t];
+                ev[t] = v.y - base[t];
             } else {
-                hv[t] = O;
-                ev[t] = 0;
+                hv[t] = O - base[t];
+                ev[t] = -base[t];
             }
         }
         a[k] = (av[0] & 0xFFFF) | (av[1] << 16);
         Ho[k] = pk2(hv[0], hv[1]);
         E[k] = pk2(ev[0], ev[1]);
-        rmax[k] = pk2(0, 0);
+        rmax[k] = zero;
     }
     int up[2];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
-        up[t] = ts[t].corner + O;
+        up[t] = ts[t].corner + O - base[t];
         if (lane > 0)
-            up[t] = (ts[t].left && row0[t] - 1 <= T[t].r1) ? ts[t].left[row0[t] - 1 - ts[t].left_r0].x + O : O;
+            up[t] = ((ts[t].left && row0[t] - 1 <= T[t].r1) ? ts[t].left[row0[t] - 1 - ts[t].left_r0].x : 0) + O -
+                    base[t];
     }
     s16x2 Ho_up_prev = pk2(up[0], up[1]);
     s16x2 Ho_bot = pk2(O, O), Fe_bot = pk2(Ev, Ev);
@@ -558,8 +569,8 @@ __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], Sw
                 H = v.x;
                 F = v.y;
             }
-            h[t] = H + O;
-            f[t] = F + Ev;
+            h[t] = H + O - base[t];
+            f[t] = F + Ev - base[t];
             b[t] = (col < T[t].w) ? (int)T[t].B[T[t].C0 + col] : 0x100;
         }
         qh = pk_bits(pk2(h[0], h[1]));
@@ -583,13 +594,13 @@ __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], Sw
         if (!dump) {
             if (interior)
                 chunk_steps_pk<R, false, false>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot, Qh, Qf, Qb,
-                                                bl, s_out);
+                                                bl, s_out, zero, base);
             else
                 chunk_steps_pk<R, true, false>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot, Qh, Qf, Qb,
-                                               bl, s_out);
+                                               bl, s_out, zero, base);
         } else {
             chunk_steps_pk<R, true, true>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot, Qh, Qf, Qb, bl,
-                                          s_out);
+                                          s_out, zero, base);
         }
         __builtin_amdgcn_wave_barrier();
         {
@@ -599,7 +610,7 @@ __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], Sw
 #pragma unroll
             for (int t = 0; t < 2; ++t) {
                 if (col >= 0 && col < T[t].w) {
-                    const int2 v = make_int2(oh[t] - O, of[t] - Ev);
+                    const int2 v = make_int2(oh[t] - O + base[t], of[t] - Ev + base[t]);
                     if (T[t].bot) T[t].bot[col] = v;
                     if (T[t].srow) T[t].srow[col] = v;
                 }
@@ -616,8 +627,8 @@ __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], Sw
         for (int k = 0; k < R; ++k) {
             const int64_t row = row0[t] + k;
             if (row > T[t].r1) continue;
-            if (ts[t].right) ts[t].right[row - ts[t].right_r0] = make_int2(Ho[k][t] - O, E[k][t]);
-            const int r = rmax[k][t];
+            if (ts[t].right) ts[t].right[row - ts[t].right_r0] = make_int2(Ho[k][t] - O + base[t], E[k][t] + base[t]);
+            const int r = rmax[k][t] + base[t];
             if (ts[t].first_segment || r > T[t].rowmax[row]) {  // first segment holding the maximum
                 T[t].rowmax[row] = r;
                 if (T[t].rowseg) T[t].rowseg[row] = T[t].seg;

@@ -138,8 +138,8 @@ def test_packed16_equals_int32(sw, orc, ctx, m, n, W):
     """Stage 1 in packed 16-bit arithmetic (two tiles per wave) gives exactly the int32
     result: score, end, start, CIGAR, crosspoints."""
     (a, b), sc = sw.synth_pair(m + n + W, m, n, min(m, n) // 2, 0.1, 0.02), sw.scoring(1, -3, -5, -2)
-    r16 = ctx.align(a, b, sc, K=512, Kc=256, xpoints=True, tile_cols=W)
-    r32 = ctx.align(a, b, sc, K=512, Kc=256, xpoints=True, tile_cols=W, precision=32)
+    r16 = ctx.align(a, b, sc, K=512, Kc=256, xpoints=True, tile_cols=W, precision=16)
+    r32 = ctx.align(a, b, sc, K=512, Kc=256, xpoints=True, tile_cols=W)
     assert r16["precision"] == 16 and r32["precision"] == 32
     for k in ("score", "end", "start", "cigar", "xpoints"):
         assert r16[k] == r32[k], k
@@ -147,15 +147,32 @@ def test_packed16_equals_int32(sw, orc, ctx, m, n, W):
     _cmp(r16, orc.align_linear(A, B, orc.scoring(1, -3, -5, -2), 512, 256))
 
 
-@pytest.mark.parametrize("L,precision", [(31000, 16), (33000, 32)])
-def test_int16_bound_selects_precision(sw, ctx, L, precision):
-    """Identical sequences score L*match: up to the int16 bound the packed path runs and is
-    exact at its largest values; beyond it the int32 path is chosen."""
+@pytest.mark.parametrize("L,score,precision", [(40000, (1, -3, -5, -2), 16), (3000, (1000, -1000, -1000, -1000), 32)])
+def test_int16_tile_bound_selects_precision(sw, ctx, L, score, precision):
+    """Identical sequences score L*match.  The packed path holds each tile's values relative
+    to the tile's base, so it runs whatever the absolute score (40000 > 32767 here) and is
+    exact; scorings whose per-cell changes are too large for a tile to fit int16 run int32."""
     rng = random.Random(L)
     s = bytes(rng.choice(b"ACGT") for _ in range(L))
-    r = ctx.align(s, s, sw.scoring(1, -3, -5, -2), score_only=True)
+    r = ctx.align(s, s, sw.scoring(*score), score_only=True, precision=16)
     assert r["precision"] == precision
-    assert r["score"] == L and r["end"] == (L - 1, L - 1)
+    assert r["score"] == L * score[0] and r["end"] == (L - 1, L - 1)
+
+
+@pytest.mark.parametrize("score,W", [((30, -30, -50, -20), 0), ((40, -20, -40, -10), 100), ((2, -3, -5, -1), 0)])
+def test_packed16_beyond_int16_scores(sw, orc, ctx, score, W):
+    """Scores far above 32767 through the packed stage 1 (values relative to each tile's
+    base): identical to the int32 path and to the oracle, crosspoints included."""
+    (a, b) = sw.synth_pair(score[0] + W, 4000, 3800, 3600, 0.05, 0.01)
+    sc = sw.scoring(*score)
+    r16 = ctx.align(a, b, sc, K=512, Kc=256, xpoints=True, tile_cols=W, precision=16)
+    r32 = ctx.align(a, b, sc, K=512, Kc=256, xpoints=True, tile_cols=W, precision=32)
+    assert r16["precision"] == 16 and r32["precision"] == 32
+    assert score[1] == -3 or r16["score"] > 32767
+    for k in ("score", "end", "start", "cigar", "xpoints"):
+        assert r16[k] == r32[k], k
+    A, B = orient(a, b)
+    _cmp(r16, orc.align_linear(A, B, orc.scoring(*score), 512, 256))
 
 
 def _alphabet_pair(name, L, seed):
