@@ -227,9 +227,28 @@ int code_shift1_for(const uint32_t mask[8], const swh_scoring* sc) {
     return 0;
 }
 
-int code_shift1_device(swh_ctx* c, const std::vector<ByteRange>& ranges, const swh_scoring* sc, int* shift1) {
+int code2_shift1_for(const uint32_t mask[8], const swh_scoring* sc) {
+    const int64_t mo = (int64_t)sc->match - sc->gap_open, xo = (int64_t)sc->mismatch - sc->gap_open;
+    if (mo > 255 || xo < 0 || xo > 255) return 0;
+    for (int sh = 0; sh <= 6; ++sh) {
+        unsigned used = 0;
+        bool ok = true;
+        for (int x = 0; x < 256 && ok; ++x) {
+            if (!(mask[x >> 5] >> (x & 31) & 1u)) continue;
+            const unsigned code = 1u << ((x >> sh) & 3);
+            ok = !(used & code);
+            used |= code;
+        }
+        if (ok) return sh + 1;
+    }
+    return 0;
+}
+
+int code_shift1_device(swh_ctx* c, const std::vector<ByteRange>& ranges, const swh_scoring* sc, int* shift1,
+                       int* shift2) {
     *shift1 = 0;
-    if (c->codes_off || (int64_t)sc->match - sc->mismatch > 255 || ranges.empty()) return SWH_OK;
+    if (shift2) *shift2 = 0;
+    if (c->codes_off || ranges.empty()) return SWH_OK;
     HCK(c, ensure(c->ranges, sizeof(ByteRange) * ranges.size()));
     HCK(c, ensure(c->alpha, sizeof(uint32_t) * 8));
     HCK(c, hipMemcpyAsync(c->ranges.p, ranges.data(), sizeof(ByteRange) * ranges.size(), hipMemcpyHostToDevice,
@@ -240,6 +259,7 @@ int code_shift1_device(swh_ctx* c, const std::vector<ByteRange>& ranges, const s
     HCK(c, hipMemcpyAsync(mask, c->alpha.p, sizeof mask, hipMemcpyDeviceToHost, c->stream));
     HCK(c, hipStreamSynchronize(c->stream));
     *shift1 = code_shift1_for(mask, sc);
+    if (shift2) *shift2 = code2_shift1_for(mask, sc);
     return SWH_OK;
 }
 
@@ -444,7 +464,7 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
             for (int64_t o = 0; o < j.m; o += kAlphabetRange) br.push_back(ByteRange{j.A + o, std::min(kAlphabetRange, j.m - o)});
             for (int64_t o = 0; o < j.n; o += kAlphabetRange) br.push_back(ByteRange{j.B + o, std::min(kAlphabetRange, j.n - o)});
         }
-        const int rc = code_shift1_device(c, br, sc, &p.code_shift1);
+        const int rc = code_shift1_device(c, br, sc, &p.code_shift1, &p.code2_shift1);
         if (rc) return rc;
     }
     int64_t launches = 0;

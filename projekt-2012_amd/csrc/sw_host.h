@@ -78,7 +78,11 @@ int64_t int16_tile_cols(const swh_scoring* sc);
 int code_shift1_for(const uint32_t mask[8], const swh_scoring* sc);
 // Same, for the bytes of device ranges (one presence kernel and a read-back).
 int code_shift1_device(swh_ctx* c, const std::vector<swk::ByteRange>& ranges, const swh_scoring* sc,
-                       int* shift1);
+                       int* shift1, int* shift2 = nullptr);
+// Packed stage 1 (SweepParams::code2_shift1): the smallest shift s in [0, 6] under which
+// (byte >> s) & 3 is one-to-one on the bytes present, plus one; 0 if there is none or a
+// profile byte (match - gap_open, mismatch - gap_open) leaves [0, 255].
+int code2_shift1_for(const uint32_t mask[8], const swh_scoring* sc);
 
 // Traceback lookahead: block (da, db) above / left of the current cell's checkpoint block is
 // recomputed in a round with lookahead L if da, db <= L and, beyond L = 3, |da - db| <= 2 (a

@@ -77,6 +77,8 @@ struct SweepParams {
     int32_t nspans;
     int32_t code_shift1;    // 0: compare bytes; s + 1: symbol code (byte >> s) & 7 selects a row profile byte
     int64_t ntiles;         // tiles of this launch (grid = ceil(ntiles / kSweepWaves) workgroups)
+    int32_t code2_shift1;   // packed stage 1: 0 = compare bytes; s + 1: 2-bit codes (byte >> s) & 3
+    int32_t _pad2;
 };
 
 struct WalkState {

@@ -439,9 +439,10 @@ __device__ __forceinline__ s16x2 pk_sel(int mask, s16x2 a, s16x2 b) {
 }
 
 // 64 steps of two tiles (see chunk_steps).  EDGE: per-half validity masks.
-template <int R, bool EDGE, bool DUMP>
+template <int R, bool PROF, bool EDGE, bool DUMP>
 __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileCtx (&T)[2], int64_t c,
-                                               const int64_t (&row0)[2], const int (&a)[R], s16x2 (&Ho)[R],
+                                               const int64_t (&row0)[2], const int (&a)[R], const int (&a2)[R],
+                                               s16x2 (&Ho)[R],
                                                s16x2 (&E)[R], s16x2 (&rmax)[R], s16x2& Ho_up_prev, s16x2& Ho_bot,
                                                s16x2& Fe_bot, int& Qh, int& Qf, int& Qb, int& bl, int2* s_out,
                                                const s16x2 zero, const int (&base)[2]) {
@@ -466,9 +467,16 @@ __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileC
         Ho_up_prev = EDGE ? pk_sel(vmask, Ho_up, Ho_up_prev) : Ho_up;
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-            // substitution: the halves of a ^ b are zero exactly where the codes agree
-            const s16x2 ne = pk_min(pk_from(a[k] ^ bl), one);
-            const s16x2 Hd = diag + vMO - ne * vD;  // H(i-1,j-1) + s(i,j)
+            s16x2 Hd;  // H(i-1,j-1) + s(i,j)
+            if (PROF) {
+                // a[k]: the two rows' profiles (byte c of each half = s - O for code c); bl:
+                // the selector of the two columns' codes
+                Hd = diag + pk_from((int)__builtin_amdgcn_perm((uint32_t)a2[k], (uint32_t)a[k], (uint32_t)bl));
+            } else {
+                // substitution: the halves of a ^ b are zero exactly where the bytes agree
+                const s16x2 ne = pk_min(pk_from(a[k] ^ bl), one);
+                Hd = diag + vMO - ne * vD;
+            }
             diag = Ho[k];
             const s16x2 Ee = E[k] + vE;
             const s16x2 En = pk_max(Ee, Ho[k]);
@@ -509,13 +517,14 @@ __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileC
 
 // Stage 1 over two tiles at once (has[1] false: the second half runs on empty input and
 // stores nothing).
-template <int R>
+template <int R, bool PROF>
 __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], SweepLds& lds) {
     int2* s_out = lds.out;
     const int lane = wave_lane();
     const int O = p.sc.O, Ev = p.sc.E;
     TileCtx T[2] = {ts[0].T, ts[1].T};
     const int64_t row0[2] = {T[0].R0 + (int64_t)lane * R, T[1].R0 + (int64_t)lane * R};
+    int a2[R];  // with symbol codes: tile 1's row profiles
     const int base[2] = {ts[0].corner, ts[1].corner};  // every value is held as value - base
     const int floor16[2] = {max(-base[0], -32768), max(-base[1], -32768)};
     const s16x2 zero = pk2(floor16[0], floor16[1]);  // H's zero floor, relative
@@ -538,7 +547,23 @@ __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], Sw
                 ev[t] = -base[t];
             }
         }
-        a[k] = (av[0] & 0xFFFF) | (av[1] << 16);
+        if (PROF) {
+            // row profiles: tile 0's in a[k] (byte c = s(row, code c) - O), tile 1's in a2[k]
+            const int sh = p.code2_shift1 - 1;
+            const uint32_t mo = (uint32_t)(p.sc.M - O), xo = (uint32_t)(p.sc.X - O);
+            uint32_t pr[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const uint32_t code = ((uint32_t)av[t] >> sh) & 3u;
+                pr[t] = (xo * 0x01010101u) & ~(0xFFu << (8 * code));
+                pr[t] |= mo << (8 * code);
+            }
+            a[k] = (int)pr[0];
+            a2[k] = (int)pr[1];
+        } else {
+            a[k] = (av[0] & 0xFFFF) | (av[1] << 16);
+            a2[k] = 0;
+        }
         Ho[k] = pk2(hv[0], hv[1]);
         E[k] = pk2(ev[0], ev[1]);
         rmax[k] = zero;
@@ -571,13 +596,17 @@ __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], Sw
             }
             h[t] = H + O - base[t];
             f[t] = F + Ev - base[t];
-            b[t] = (col < T[t].w) ? (int)T[t].B[T[t].C0 + col] : 0x100;
+            if (PROF)  // v_perm_b32 selector bytes: tile 0 reads profile byte code, tile 1 byte 4 + code
+                b[t] = (col < T[t].w) ? (int)((((uint32_t)T[t].B[T[t].C0 + col] >> (p.code2_shift1 - 1)) & 3u) + 4 * t)
+                                      : 0x0c;
+            else
+                b[t] = (col < T[t].w) ? (int)T[t].B[T[t].C0 + col] : 0x100;
         }
         qh = pk_bits(pk2(h[0], h[1]));
         qf = pk_bits(pk2(f[0], f[1]));
-        qb = (b[0] & 0xFFFF) | (b[1] << 16);
+        qb = PROF ? (b[0] | 0x0c00 | (b[1] << 16) | 0x0c000000) : ((b[0] & 0xFFFF) | (b[1] << 16));
     };
-    int Qh, Qf, Qb, nh, nf, nb, bl = 0x01000100;
+    int Qh, Qf, Qb, nh, nf, nb, bl = PROF ? 0x0c0c0c0c : 0x01000100;
     load(0, Qh, Qf, Qb);
     for (int64_t c = 0; c < nchunks; ++c) {
         if (c + 1 < nchunks) load(c + 1, nh, nf, nb);
@@ -593,14 +622,14 @@ __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], Sw
         }
         if (!dump) {
             if (interior)
-                chunk_steps_pk<R, false, false>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot, Qh, Qf, Qb,
-                                                bl, s_out, zero, base);
+                chunk_steps_pk<R, PROF, false, false>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
+                                                      Qh, Qf, Qb, bl, s_out, zero, base);
             else
-                chunk_steps_pk<R, true, false>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot, Qh, Qf, Qb,
-                                               bl, s_out, zero, base);
+                chunk_steps_pk<R, PROF, true, false>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
+                                                     Qh, Qf, Qb, bl, s_out, zero, base);
         } else {
-            chunk_steps_pk<R, true, true>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot, Qh, Qf, Qb, bl,
-                                          s_out, zero, base);
+            chunk_steps_pk<R, PROF, true, true>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot, Qh,
+                                                Qf, Qb, bl, s_out, zero, base);
         }
         __builtin_amdgcn_wave_barrier();
         {
@@ -638,7 +667,7 @@ __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], Sw
 }
 
 // Packed stage 1: wave x takes tiles 2x and 2x+1 of the launch.
-template <int R>
+template <int R, bool PROF>
 __device__ void sweep_main_pk(const SweepParams& p, SweepLds& lds) {
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t x = (int64_t)blockIdx.x * kSweepWaves + wid;
@@ -657,13 +686,13 @@ __device__ void sweep_main_pk(const SweepParams& p, SweepLds& lds) {
         ts[1].left = nullptr;
         ts[1].right = nullptr;
     }
-    sweep_tile_pk<R>(p, ts, lds);
+    sweep_tile_pk<R, PROF>(p, ts, lds);
 }
 
-template <int R>
+template <int R, bool PROF>
 __global__ __launch_bounds__(64 * kSweepWaves) void k_sweep_pk(SweepParams p) {
     __shared__ SweepLds lds[kSweepWaves];
-    sweep_main_pk<R>(p, lds[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)]);
+    sweep_main_pk<R, PROF>(p, lds[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)]);
 }
 
 template <int R, int MODE, bool PROF>
@@ -1002,7 +1031,8 @@ hipError_t launch_sweep(int mode, const SweepParams& p, size_t lds_pad, hipStrea
 hipError_t launch_sweep_pk(const SweepParams& p, size_t lds_pad, hipStream_t s) {
     const int64_t pairs = (p.ntiles + 1) / 2;
     const dim3 grid((unsigned)((pairs + kSweepWaves - 1) / kSweepWaves)), block(64 * kSweepWaves);
-    hipLaunchKernelGGL((k_sweep_pk<kRowsPerLane>), grid, block, lds_pad, s, p);
+    if (p.code2_shift1) hipLaunchKernelGGL((k_sweep_pk<kRowsPerLane, true>), grid, block, lds_pad, s, p);
+    else hipLaunchKernelGGL((k_sweep_pk<kRowsPerLane, false>), grid, block, lds_pad, s, p);
     return hipGetLastError();
 }
 
@@ -1018,7 +1048,9 @@ hipError_t sweep_allow_lds(size_t dyn_bytes) {
     for (const void* f : fns)
         if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, v);
     if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void*)k_sweep_pk<R>, hipFuncAttributeMaxDynamicSharedMemorySize, v);
+        e = hipFuncSetAttribute((const void*)k_sweep_pk<R, false>, hipFuncAttributeMaxDynamicSharedMemorySize, v);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)k_sweep_pk<R, true>, hipFuncAttributeMaxDynamicSharedMemorySize, v);
     return e;
 }
 

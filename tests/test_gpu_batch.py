@@ -116,3 +116,30 @@ def test_batch_autotune(sw, orc, ctx):
                                  waves_per_simd=t["waves_per_simd"], K=K, Kc=K)
     _check(orc, pairs, res2, st2, K=K)
     dev.free()
+
+
+@pytest.mark.parametrize("alphabet", [b"ACGT", b"ACGTN", b"AC", b"ACGTacgt"])
+@pytest.mark.parametrize("score", [(1, -3, -5, -2), (2, -3, -5, -1), (3, -9, -5, -2)])
+def test_batch_packed_scores_by_alphabet(sw, orc, ctx, alphabet, score):
+    """The packed stage 1 takes substitution scores from 2-bit symbol-code profiles (one
+    v_perm_b32 per cell pair) when the batch's bytes fit 4 codes and the profile bytes
+    (match - gap_open, mismatch - gap_open) fit [0, 255], and from byte compares otherwise
+    (ACGTN, upper and lower case, mismatch below gap_open): the oracle's alignments either way."""
+    rng = random.Random(len(alphabet) * 100 + score[0])
+    pairs = []
+    for k in range(12):
+        L = 300 + 220 * k
+        a = bytes(rng.choice(alphabet) for _ in range(L))
+        b = bytearray(a[L // 10: L - L // 12])
+        for i in range(len(b)):
+            if rng.random() < 0.1:
+                b[i] = rng.choice(alphabet)
+        pairs.append((a, bytes(b)))
+    res, st = ctx.batch(pairs, sw.scoring(*score), K=256, Kc=256, tile_cols=128)
+    assert all(x == 0 for x in st)
+    for (a, b), r in zip(pairs, res):
+        assert r["precision"] == 16
+        A, B = orient(a, b)
+        ref = orc.align_linear(A, B, orc.scoring(*score), 256, 256)
+        for k in ("score", "end", "start", "cigar"):
+            assert r[k] == ref[k], (k, len(a), len(b))
