@@ -348,8 +348,22 @@ constexpr int64_t kBatchTileCols = 512;
 // jobs[k].status.  Stage times are those of the whole batch.
 constexpr int64_t kAlphabetRange = INT64_C(1) << 16;  // bytes per workgroup of k_alphabet
 
+// Host wall time between the stages of run_jobs, printed to stderr when the context traces.
+struct StageClock {
+    bool on;
+    std::chrono::steady_clock::time_point last = std::chrono::steady_clock::now();
+    void mark(const char* stage, int64_t n) {
+        if (!on) return;
+        const auto now = std::chrono::steady_clock::now();
+        fprintf(stderr, "swalign: %-10s %9.3f ms (%lld)\n", stage,
+                std::chrono::duration<double, std::milli>(now - last).count(), (long long)n);
+        last = now;
+    }
+};
+
 int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const EngineOpts& eo, swh_result* out) {
     const int64_t nj = (int64_t)jobs.size();
+    StageClock clk{c->trace};
     int64_t rm_elems = 0, sr_elems = 0, scl_elems = 0, ops_elems = 0, xps_elems = 0;
     int64_t max_m = 0, max_n = 0;
     for (Job& j : jobs) {
@@ -467,9 +481,11 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
         const int rc = code_shift1_device(c, br, sc, &p.code_shift1, &p.code2_shift1);
         if (rc) return rc;
     }
+    clk.mark("setup", (int64_t)live.size());
     int64_t launches = 0;
     int rc = run_sweep(c, kStage1, p, regions, W, 0, c->ev[0], c->ev[1], wps, &launches, 0, packed);
     if (rc) return rc;
+    clk.mark("stage1", launches);
     for (int64_t k : live) {
         out[k].precision_used = packed ? 16 : 32;
         out[k].score_codes = p.code_shift1 != 0;
@@ -495,6 +511,7 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
     HCK(c, hipMemcpyAsync(endseg.data(), segs, sizeof(int32_t) * nj, hipMemcpyDeviceToHost, c->stream));
     HCK(c, hipStreamSynchronize(c->stream));
     const float ms_fwd = elapsed(c->ev[0], c->ev[1]);
+    clk.mark("argmax", (int64_t)aj.size());
     std::vector<int64_t> scored;
     for (int64_t k : live) {
         Job& j = jobs[k];
@@ -557,6 +574,7 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
         HCK(c, hipEventRecord(c->ev[3], c->stream));
         HCK(c, hipStreamSynchronize(c->stream));
         const float ms_search = elapsed(c->ev[1], c->ev[3]);
+        clk.mark("search", (int64_t)regions.size());
         for (int64_t k : scored) {
             Job& j = jobs[k];
             if (found[k] == INT32_MAX || found[k] < 0) {
@@ -733,6 +751,7 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
         active.swap(next);
     }
     HCK(c, hipEventRecord(c->ev[5], c->stream));
+    clk.mark("traceback", guard);
 
     // ---- results: ops (reversed) -> run-length CIGAR, crosspoints start -> end ---------
     if ((size_t)ops_elems > c->host_ops_cap) {  // pinned, grow-only: the copy runs at full link speed
@@ -749,6 +768,7 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
         HCK(c, hipMemcpyAsync(xps_h.data(), c->xps.p, sizeof(XPoint) * xps_elems, hipMemcpyDeviceToHost, c->stream));
     HCK(c, hipStreamSynchronize(c->stream));
     const float ms_tb = elapsed(c->ev[4], c->ev[5]);
+    clk.mark("readback", ops_elems);
     // CIGARs are built on the host threads, pairs split between them
     std::vector<int> fail(16, 0);
     auto build = [&](int tid, int nth) {
@@ -777,6 +797,7 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
     }
     for (int f : fail)
         if (f) return SWH_ERR_OUT_OF_MEMORY;
+    clk.mark("cigars", (int64_t)scored.size());
     return SWH_OK;
 }
 
@@ -786,22 +807,25 @@ int fill_alignment(swh_result& r, const WalkState& st, const uint8_t* ops, const
                    int32_t best, bool want_xp) {
     r.start_row = st.start_i;
     r.start_col = st.start_j;
-    std::string rle;
-    rle.reserve((size_t)st.nops / 4 + 16);
-    char num[32];
+    // a run of n ops takes at most (digits of n) + 1 <= 2n characters
+    char* out = (char*)malloc((size_t)(2 * st.nops + 1));
+    if (!out) return SWH_ERR_OUT_OF_MEMORY;
+    size_t len = 0;
     for (int64_t b = st.nops - 1; b >= 0;) {  // ops were written end -> start
         int64_t e = b;
         while (e >= 0 && ops[e] == ops[b]) --e;
-        const int len = snprintf(num, sizeof num, "%lld%c", (long long)(b - e), ops[b]);
-        rle.append(num, (size_t)len);
+        char digits[24];
+        int nd = 0;
+        for (uint64_t v = (uint64_t)(b - e); v; v /= 10) digits[nd++] = (char)('0' + v % 10);
+        while (nd) out[len++] = digits[--nd];
+        out[len++] = (char)ops[b];
         b = e;
     }
+    out[len] = 0;
     r.n_ops = st.nops;
-    r.cigar = (char*)malloc(rle.size() + 1);
-    if (!r.cigar) return SWH_ERR_OUT_OF_MEMORY;
-    memcpy(r.cigar, rle.data(), rle.size());
-    r.cigar[rle.size()] = 0;
-    r.cigar_len = (int64_t)rle.size();
+    char* shrunk = (char*)realloc(out, len + 1);
+    r.cigar = shrunk ? shrunk : out;
+    r.cigar_len = (int64_t)len;
     if (want_xp) {
         const int64_t nx = std::min<int64_t>(st.nxp, xcap);
         r.xpoints = (swh_xpoint*)malloc(sizeof(swh_xpoint) * (size_t)(nx + 1));
@@ -889,6 +913,8 @@ int swh_ctx_create(int device, swh_ctx** out) {
     // SWALIGN_SYMBOL_CODES=0 keeps the byte-compare score in every sweep
     const char* codes = getenv("SWALIGN_SYMBOL_CODES");
     c->codes_off = codes && atoi(codes) == 0;
+    const char* trace = getenv("SWALIGN_TRACE");
+    c->trace = trace && atoi(trace) != 0;
     const size_t per_cu = (size_t)c->props.maxSharedMemoryPerMultiProcessor, stat = sweep_static_lds();
     for (int w = 1; w <= 2; ++w) {
         const size_t want = per_cu / (size_t)(w + 1) + 1024;

@@ -38,6 +38,7 @@ struct swh_ctx {
     size_t lds_pad[3] = {0, 0, 0};  // dynamic LDS per sweep workgroup for w workgroups per CU
     int wps_fixed = 0;              // sweep waves per SIMD forced by the environment, 0 = model
     bool codes_off = false;         // symbol-code scores disabled by the environment
+    bool trace = false;             // SWALIGN_TRACE=1: host wall time of each stage to stderr
     hipError_t last = hipSuccess;
 };
 
