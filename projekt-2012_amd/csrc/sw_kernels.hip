@@ -5,9 +5,9 @@
 // to right.  Lane l holds rows [l*R, l*R+R) of the tile and, at step t, updates column
 // t-l: the tile advances along anti-diagonals, so lane l needs the bottom-row {H,F} of
 // lane l-1 from the previous step, which it receives with one DPP wave_shr:1 per value.
-// Lane 0 receives the tile's top boundary (the bottom row of the tile above) from LDS;
-// lane 63's bottom row is staged in LDS and written once per 64 steps with a coalesced
-// store.  Tiles are scheduled by anti-diagonal of the tile grid, one kernel launch per
+// Lane 0 takes the tile's top boundary (the bottom row of the tile above) from register
+// queues filled by coalesced loads once per 64 steps; lane 63's bottom row is staged in
+// LDS and written once per 64 steps with a coalesced store.  Tiles are scheduled by anti-diagonal of the tile grid, one kernel launch per
 // diagonal, so a tile's inputs were all written by earlier launches: no wave waits on
 // another wave.
 //
