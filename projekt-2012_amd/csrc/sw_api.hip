@@ -201,7 +201,8 @@ int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions,
         p.spans = (const TileSpan*)c->spans.p + span_at[(size_t)d];
         p.nspans = (int32_t)(span_at[(size_t)d + 1] - span_at[(size_t)d]);
         p.ntiles = tiles_at[(size_t)d];
-        if (packed && mode == kStage1) HCK(c, launch_sweep_pk(p, c->lds_pad[wps == 2 ? 2 : 1], c->stream));
+        if (packed && (mode == kStage1 || mode == kDirs))
+            HCK(c, launch_sweep_pk(mode, p, c->lds_pad[wps == 2 ? 2 : 1], c->stream));
         else HCK(c, launch_sweep(mode, p, c->lds_pad[wps == 2 ? 2 : 1], c->stream));
     }
     if (ev_after) HCK(c, hipEventRecord(ev_after, c->stream));
@@ -616,6 +617,8 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
     const int64_t Wtb = eo.batch ? W : tile_cols(c, wmax, eo.tile_cols);
     const int64_t tstride = ceil_div(Wtb + 63, 64) * 64;
     const int64_t Gblk = ceil_div(wmax, Wtb);
+    const bool tb_packed = eo.batch && packed && Wtb <= w16;
+    const int tb_wps = eo.batch && c->lds_pad[2] ? 2 : 1;
     auto blk_dirs = [&](const Job& j) {
         return ceil_div(std::min(j.m, j.pl.K), kStripeRows) * Gblk * tstride * 64 * (int64_t)sizeof(uint16_t);
     };
@@ -690,7 +693,9 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
             regions[q].dirs = d;
             wregs[(size_t)slots[q].first].dirs = d;
         }
-        rc = run_sweep(c, kDirs, p, regions, Wtb, Gblk, nullptr, nullptr, 1, nullptr, tstride);
+        // batches: packed 16-bit codes sweep (two tiles per wave) when stage 1 ran packed and the
+        // tiles fit, two waves per SIMD; a single pair's rounds are latency-bound: int32, one wave
+        rc = run_sweep(c, kDirs, p, regions, Wtb, Gblk, nullptr, nullptr, tb_wps, nullptr, tstride, tb_packed);
         if (rc) return rc;
         // device arrays of the round; allocated before their addresses go into the walk args
         HCK(c, ensure(c->wregs, sizeof(WalkRegion) * wregs.size()));

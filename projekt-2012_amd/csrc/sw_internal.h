@@ -125,7 +125,7 @@ struct WalkArgs {
 hipError_t launch_sweep(int mode, const SweepParams& p, size_t lds_pad, hipStream_t s);
 // Stage 1 in packed 16-bit arithmetic, two tiles per wave (the caller guarantees no value
 // leaves int16).
-hipError_t launch_sweep_pk(const SweepParams& p, size_t lds_pad, hipStream_t s);
+hipError_t launch_sweep_pk(int mode, const SweepParams& p, size_t lds_pad, hipStream_t s);  // kStage1 or kDirs
 size_t sweep_static_lds();                     // static LDS of a sweep workgroup
 hipError_t sweep_allow_lds(size_t dyn_bytes);  // permits dyn_bytes of dynamic LDS
 // Row argmax of many pairs: job k reduces rowmax[off[k] .. off[k] + len[k]) into key[k]

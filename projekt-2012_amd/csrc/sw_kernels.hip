@@ -422,6 +422,12 @@ __device__ void sweep_main(const SweepParams& p, SweepLds& lds) {
 // the base exceeds 32768 no cell of the tile is near zero.  Inputs and outputs in memory
 // stay int32 and absolute.
 typedef short s16x2 __attribute__((vector_size(4)));
+typedef unsigned short u16x2 __attribute__((vector_size(4)));
+// Per half: 1 if the (non-negative, below 65536) difference d is not zero, else 0.
+__device__ __forceinline__ uint32_t differs(s16x2 d) {
+    const u16x2 one = {1, 1};
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(u16x2, d), one));
+}
 
 __device__ __forceinline__ int pk_bits(s16x2 v) { return __builtin_bit_cast(int, v); }
 __device__ __forceinline__ s16x2 pk_from(int v) { return __builtin_bit_cast(s16x2, v); }
@@ -439,7 +445,7 @@ __device__ __forceinline__ s16x2 pk_sel(int mask, s16x2 a, s16x2 b) {
 }
 
 // 64 steps of two tiles (see chunk_steps).  EDGE: per-half validity masks.
-template <int R, bool PROF, bool EDGE, bool DUMP>
+template <int R, int MODE, bool PROF, bool EDGE, bool DUMP>
 __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileCtx (&T)[2], int64_t c,
                                                const int64_t (&row0)[2], const int (&a)[R], const int (&a2)[R],
                                                s16x2 (&Ho)[R],
@@ -450,6 +456,11 @@ __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileC
     const s16x2 vO = pk2(p.sc.O, p.sc.O), vE = pk2(p.sc.E, p.sc.E), one = pk2(1, 1);
     const s16x2 vMO = pk2(p.sc.M - p.sc.O, p.sc.M - p.sc.O);
     const s16x2 vD = pk2(p.sc.M - p.sc.X, p.sc.M - p.sc.X);  // match - mismatch
+    uint16_t* dcol[2] = {nullptr, nullptr};
+    if (MODE == kDirs) {
+#pragma unroll
+        for (int t = 0; t < 2; ++t) dcol[t] = T[t].dirs ? T[t].dirs + (64 * c) * 64 + lane : nullptr;
+    }
     SWK_UNROLL(UNROLL_STEPS)
     for (int s = 0; s < 64; ++s) {
         s16x2 Ho_up = pk_from(shr1(Qh, pk_bits(Ho_bot)));
@@ -465,6 +476,7 @@ __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileC
         }
         s16x2 diag = Ho_up_prev;
         Ho_up_prev = EDGE ? pk_sel(vmask, Ho_up, Ho_up_prev) : Ho_up;
+        uint32_t nib = 0;  // kDirs: both tiles' codes, tile 0 in the low 16 bits
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             s16x2 Hd;  // H(i-1,j-1) + s(i,j)
@@ -482,15 +494,27 @@ __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileC
             const s16x2 En = pk_max(Ee, Ho[k]);
             const s16x2 F = pk_max(Fe_up, Ho_up);
             const s16x2 H = pk_max(pk_max(Hd, zero), pk_max(En, F));
+            if (MODE == kDirs) {
+                // the 4-bit code of each half, as k_sweep's: h = 0 (H at the zero floor), 1
+                // (diagonal), 2 (F), 3 (E); bit 2: E extends; bit 3: F extends.  Differences
+                // H - x are >= 0, so min(., 1) is the half's "differs" flag; the sign bit of
+                // Ho - Ee is "E extends".  Halves never carry into each other (values <= 15).
+                const uint32_t nz = differs(H - zero), nd = differs(H - Hd), nf = differs(H - F);
+                const uint32_t a1 = nz & nd;
+                const uint32_t h = nz + a1 + (a1 & nf);
+                const uint32_t ee = ((uint32_t)pk_bits(Ho[k] - Ee) >> 15) & 0x00010001u;
+                const uint32_t fe = ((uint32_t)pk_bits(Ho_up - Fe_up) >> 15) & 0x00010001u;
+                nib |= (h | (ee << 2) | (fe << 3)) << (4 * k);
+            }
             const s16x2 Hon = H + vO;
             if (EDGE) {
                 E[k] = pk_sel(vmask, En, E[k]);
                 Ho[k] = pk_sel(vmask, Hon, Ho[k]);
-                rmax[k] = pk_sel(vmask, pk_max(rmax[k], H), rmax[k]);
+                if (MODE == kStage1) rmax[k] = pk_sel(vmask, pk_max(rmax[k], H), rmax[k]);
             } else {
                 E[k] = En;
                 Ho[k] = Hon;
-                rmax[k] = pk_max(rmax[k], H);
+                if (MODE == kStage1) rmax[k] = pk_max(rmax[k], H);
             }
             Ho_up = Hon;
             Fe_up = F + vE;
@@ -498,6 +522,10 @@ __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileC
         Ho_bot = Ho_up;
         Fe_bot = Fe_up;
         if (lane == 63) s_out[s] = make_int2(pk_bits(Ho_bot), pk_bits(Fe_bot));
+        if (MODE == kDirs) {
+            if (dcol[0]) dcol[0][s * 64] = (uint16_t)nib;
+            if (dcol[1]) dcol[1][s * 64] = (uint16_t)(nib >> 16);
+        }
         if (DUMP) {
 #pragma unroll
             for (int t = 0; t < 2; ++t) {
@@ -517,7 +545,7 @@ __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileC
 
 // Stage 1 over two tiles at once (has[1] false: the second half runs on empty input and
 // stores nothing).
-template <int R, bool PROF>
+template <int R, int MODE, bool PROF>
 __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], SweepLds& lds) {
     int2* s_out = lds.out;
     const int lane = wave_lane();
@@ -614,7 +642,7 @@ __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], Sw
         bool dump = false;
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
-            if (T[t].scol && T[t].w > 0) {
+            if (MODE == kStage1 && T[t].scol && T[t].w > 0) {
                 const int64_t lo = T[t].C0 + max(64 * c - 63, (int64_t)0), hi = T[t].C0 + min(64 * c + 63, T[t].w - 1);
                 const int64_t x = ((lo + T[t].Kc) / T[t].Kc) * T[t].Kc - 1;
                 dump = dump || x <= hi;
@@ -622,13 +650,13 @@ __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], Sw
         }
         if (!dump) {
             if (interior)
-                chunk_steps_pk<R, PROF, false, false>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
+                chunk_steps_pk<R, MODE, PROF, false, false>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
                                                       Qh, Qf, Qb, bl, s_out, zero, base);
             else
-                chunk_steps_pk<R, PROF, true, false>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
+                chunk_steps_pk<R, MODE, PROF, true, false>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
                                                      Qh, Qf, Qb, bl, s_out, zero, base);
         } else {
-            chunk_steps_pk<R, PROF, true, true>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot, Qh,
+            chunk_steps_pk<R, MODE, PROF, true, true>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot, Qh,
                                                 Qf, Qb, bl, s_out, zero, base);
         }
         __builtin_amdgcn_wave_barrier();
@@ -641,7 +669,7 @@ __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], Sw
                 if (col >= 0 && col < T[t].w) {
                     const int2 v = make_int2(oh[t] - O + base[t], of[t] - Ev + base[t]);
                     if (T[t].bot) T[t].bot[col] = v;
-                    if (T[t].srow) T[t].srow[col] = v;
+                    if (MODE == kStage1 && T[t].srow) T[t].srow[col] = v;
                 }
             }
         }
@@ -657,6 +685,7 @@ __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], Sw
             const int64_t row = row0[t] + k;
             if (row > T[t].r1) continue;
             if (ts[t].right) ts[t].right[row - ts[t].right_r0] = make_int2(Ho[k][t] - O + base[t], E[k][t] + base[t]);
+            if (MODE != kStage1) continue;
             const int r = rmax[k][t] + base[t];
             if (ts[t].first_segment || r > T[t].rowmax[row]) {  // first segment holding the maximum
                 T[t].rowmax[row] = r;
@@ -667,13 +696,13 @@ __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], Sw
 }
 
 // Packed stage 1: wave x takes tiles 2x and 2x+1 of the launch.
-template <int R, bool PROF>
+template <int R, int MODE, bool PROF>
 __device__ void sweep_main_pk(const SweepParams& p, SweepLds& lds) {
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t x = (int64_t)blockIdx.x * kSweepWaves + wid;
     TileSetup ts[2];
-    if (!make_tile<R, kStage1>(p, 2 * x, ts[0])) return;
-    if (!make_tile<R, kStage1>(p, 2 * x + 1, ts[1])) {
+    if (!make_tile<R, MODE>(p, 2 * x, ts[0])) return;
+    if (!make_tile<R, MODE>(p, 2 * x + 1, ts[1])) {
         // no second tile: an empty one (no rows, no columns, nothing stored)
         ts[1] = ts[0];
         ts[1].T.w = 0;
@@ -683,16 +712,17 @@ __device__ void sweep_main_pk(const SweepParams& p, SweepLds& lds) {
         ts[1].T.srow = nullptr;
         ts[1].T.scol = nullptr;
         ts[1].T.rowseg = nullptr;
+        ts[1].T.dirs = nullptr;
         ts[1].left = nullptr;
         ts[1].right = nullptr;
     }
-    sweep_tile_pk<R, PROF>(p, ts, lds);
+    sweep_tile_pk<R, MODE, PROF>(p, ts, lds);
 }
 
-template <int R, bool PROF>
+template <int R, int MODE, bool PROF>
 __global__ __launch_bounds__(64 * kSweepWaves) void k_sweep_pk(SweepParams p) {
     __shared__ SweepLds lds[kSweepWaves];
-    sweep_main_pk<R, PROF>(p, lds[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)]);
+    sweep_main_pk<R, MODE, PROF>(p, lds[__builtin_amdgcn_readfirstlane(threadIdx.x >> 6)]);
 }
 
 template <int R, int MODE, bool PROF>
@@ -1028,11 +1058,16 @@ hipError_t launch_sweep(int mode, const SweepParams& p, size_t lds_pad, hipStrea
     return hipGetLastError();
 }
 
-hipError_t launch_sweep_pk(const SweepParams& p, size_t lds_pad, hipStream_t s) {
+hipError_t launch_sweep_pk(int mode, const SweepParams& p, size_t lds_pad, hipStream_t s) {
     const int64_t pairs = (p.ntiles + 1) / 2;
     const dim3 grid((unsigned)((pairs + kSweepWaves - 1) / kSweepWaves)), block(64 * kSweepWaves);
-    if (p.code2_shift1) hipLaunchKernelGGL((k_sweep_pk<kRowsPerLane, true>), grid, block, lds_pad, s, p);
-    else hipLaunchKernelGGL((k_sweep_pk<kRowsPerLane, false>), grid, block, lds_pad, s, p);
+    constexpr int R = kRowsPerLane;
+    const bool prof = p.code2_shift1 != 0;
+    if (mode == kStage1 && prof) hipLaunchKernelGGL((k_sweep_pk<R, kStage1, true>), grid, block, lds_pad, s, p);
+    else if (mode == kStage1) hipLaunchKernelGGL((k_sweep_pk<R, kStage1, false>), grid, block, lds_pad, s, p);
+    else if (mode == kDirs && prof) hipLaunchKernelGGL((k_sweep_pk<R, kDirs, true>), grid, block, lds_pad, s, p);
+    else if (mode == kDirs) hipLaunchKernelGGL((k_sweep_pk<R, kDirs, false>), grid, block, lds_pad, s, p);
+    else return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
@@ -1048,9 +1083,13 @@ hipError_t sweep_allow_lds(size_t dyn_bytes) {
     for (const void* f : fns)
         if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, v);
     if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void*)k_sweep_pk<R, false>, hipFuncAttributeMaxDynamicSharedMemorySize, v);
+        e = hipFuncSetAttribute((const void*)k_sweep_pk<R, kStage1, false>, hipFuncAttributeMaxDynamicSharedMemorySize, v);
     if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void*)k_sweep_pk<R, true>, hipFuncAttributeMaxDynamicSharedMemorySize, v);
+        e = hipFuncSetAttribute((const void*)k_sweep_pk<R, kStage1, true>, hipFuncAttributeMaxDynamicSharedMemorySize, v);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)k_sweep_pk<R, kDirs, false>, hipFuncAttributeMaxDynamicSharedMemorySize, v);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)k_sweep_pk<R, kDirs, true>, hipFuncAttributeMaxDynamicSharedMemorySize, v);
     return e;
 }
 
