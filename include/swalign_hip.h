@@ -178,7 +178,10 @@ int swh_batch_align_device(swh_ctx* ctx, const swh_pair* pairs, int64_t npairs, 
  * sweep waves per SIMD {1, 2} and checkpoint spacing {256, 512, 1024, 2048} on an evenly spaced
  * sample of `sample` pairs (device-resident pairs if `device` != 0) and reports the
  * fastest, to be passed back in swh_opts (tile_cols, waves_per_simd, checkpoint_rows and
- * checkpoint_cols).  swh_opts.autotune = 1 makes the batch entry points do this first. */
+ * checkpoint_cols).  Spacings whose checkpoints for the whole batch exceed hbm_budget_bytes
+ * (default half the free HBM), and candidates that run out of memory, are skipped
+ * (SWH_ERR_BUDGET if none is left).  swh_opts.autotune = 1 makes the batch entry points do
+ * this first. */
 typedef struct {
     int32_t tile_cols, waves_per_simd;
     int64_t checkpoint_rows;

@@ -1115,7 +1115,13 @@ int tune_jobs(swh_ctx* c, const std::vector<Job>& jobs, const swh_scoring* sc, c
         if (rc) return rc;
     }
     int cand = 0;
+    const int64_t budget = default_budget(o);
     for (int64_t K : Ks) {
+        // a spacing whose checkpoints for the whole batch exceed the budget is not a candidate
+        int64_t need = 0;
+        if (!(o && o->score_only))
+            for (const Job& j : jobs) need += plan_mem(j.m, j.n, K, K);
+        if (need > budget) continue;
         for (int P : Ps) {
             if (P == 2 && !c->lds_pad[2]) continue;
             for (int W : Ws) {
@@ -1137,6 +1143,7 @@ int tune_jobs(swh_ctx* c, const std::vector<Job>& jobs, const swh_scoring* sc, c
                 const double ms =
                     std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
                 for (auto& r : res) swh_result_free(&r);
+                if (rc == SWH_ERR_OUT_OF_MEMORY) continue;  // does not fit beside other allocations
                 if (rc) return rc;
                 ++cand;
                 if (W == kBatchTileCols && K == 1024 && P == (c->lds_pad[2] ? 2 : 1)) out->ms_default = ms;
@@ -1151,7 +1158,7 @@ int tune_jobs(swh_ctx* c, const std::vector<Job>& jobs, const swh_scoring* sc, c
         }
     }
     out->candidates = cand;
-    return SWH_OK;
+    return cand ? SWH_OK : SWH_ERR_BUDGET;
 }
 
 }  // namespace
