@@ -278,7 +278,8 @@ int64_t plan_mem(int64_t m, int64_t n, int64_t K, int64_t Kc) {
 }
 
 // Checkpoint spacing of one pair: K rows (multiple of 256) and Kc columns (power of two),
-// from the options or doubled from 8192 until the checkpoints fit `budget` bytes.
+// from the options or doubled from max(m, n)/64 (1024 to 8192) until the checkpoints fit
+// `budget` bytes.
 int choose_plan(int64_t m, int64_t n, const swh_opts* o, int64_t budget, Plan* pl) {
     int64_t K = o && o->checkpoint_rows > 0 ? ceil_div(o->checkpoint_rows, kStripeRows) * kStripeRows : 0;
     int64_t Kc = o && o->checkpoint_cols > 0 ? next_pow2(o->checkpoint_cols) : 0;
@@ -286,7 +287,9 @@ int choose_plan(int64_t m, int64_t n, const swh_opts* o, int64_t budget, Plan* p
         K = INT64_C(1) << 62;
         Kc = INT64_C(1) << 62;
     } else if (K == 0) {
-        K = 8192;
+        // max(m, n)/64 rows, 1024 to 8192: a short pair's end search and traceback blocks stay
+        // small, a long pair keeps few checkpoints
+        K = std::min<int64_t>(8192, std::max<int64_t>(1024, next_pow2(std::max(m, n) / 64)));
         if (Kc == 0) Kc = next_pow2(K);
         while (plan_mem(m, n, K, Kc) > budget && K < (INT64_C(1) << 40)) {
             K *= 2;
