@@ -143,3 +143,21 @@ def test_batch_packed_scores_by_alphabet(sw, orc, ctx, alphabet, score):
         ref = orc.align_linear(A, B, orc.scoring(*score), 256, 256)
         for k in ("score", "end", "start", "cigar"):
             assert r[k] == ref[k], (k, len(a), len(b))
+
+
+def test_batch_packed_high_scores_equal_int32(sw, ctx):
+    """Pairs whose scores pass 32767 (near-identical 30-45 kbp pairs) in a packed batch, beside
+    short pairs: stage 1 and the traceback codes run in int16 relative to each tile's base and
+    give exactly the alignments of the int32 single-pair path (itself checked against the
+    oracle elsewhere)."""
+    pairs = []
+    for k, L in enumerate((45000, 38000, 30500, 800, 120, 5000)):
+        a, b = sw.synth_pair(900 + k, L, L - L // 50, L - L // 20, 0.01, 0.002)
+        pairs.append((a, b))
+    res, st = ctx.batch(pairs, sw.scoring(1, -3, -5, -2), xpoints=True, K=1024, Kc=1024)
+    assert all(x == 0 for x in st)
+    assert res[0]["precision"] == 16 and max(r["score"] for r in res) > 32767
+    for (a, b), r in zip(pairs, res):
+        one = ctx.align(a, b, sw.scoring(1, -3, -5, -2), xpoints=True, K=1024, Kc=1024, precision=32)
+        for k in ("score", "end", "start", "cigar", "xpoints"):
+            assert r[k] == one[k], (k, len(a), len(b))
