@@ -76,7 +76,8 @@ typedef struct {
                                /* mapped into the device (the reference's temp/row_R side      */
                                /* channel, src/Builders.cpp:27-104, kept in memory): HBM holds */
                                /* none of them and hbm_budget_bytes caps that host memory      */
-                               /* instead (0 = half the available host memory)                 */
+                               /* instead (0 = half the available host memory); the request    */
+                               /* server keeps its checkpoints in its HBM pool                 */
 } swh_opts;
 
 /* Path node on a checkpoint row (the analogue of the reference's crosspoints,
