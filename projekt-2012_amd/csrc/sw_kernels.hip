@@ -7,9 +7,9 @@
 // lane l-1 from the previous step, which it receives with one DPP wave_shr:1 per value.
 // Lane 0 takes the tile's top boundary (the bottom row of the tile above) from register
 // queues filled by coalesced loads once per 64 steps; lane 63's bottom row is staged in
-// LDS and written once per 64 steps with a coalesced store.  Tiles are scheduled by anti-diagonal of the tile grid, one kernel launch per
-// diagonal, so a tile's inputs were all written by earlier launches: no wave waits on
-// another wave.
+// LDS and written once per 64 steps with a coalesced store.  Tiles are scheduled by
+// anti-diagonal of the tile grid, one kernel launch per diagonal, so a tile's inputs were
+// all written by earlier launches: no wave waits on another wave.
 //
 // Cell update (Sterbic/Projekt-2012 src/FindAlignment.cuh:50-56, zero boundaries as in
 // src/SWutils.cpp:131-136), kept in "plus open" form Ho = H + O so that one add serves
@@ -125,7 +125,7 @@ __device__ __forceinline__ void row_symbol(const SweepParams& p, bool in, uint32
 }
 
 // 64 steps of one tile.  The chunk's inputs wait in register queues, one column per
-// lane: Qh/Qf hold the top boundary, Qb the column bytes; each step lane 0 takes the head
+// lane: Qh/Qf hold the top boundary, Qb the column symbols; each step lane 0 takes the head
 // of the queues (DPP wave_shr with the queue as `old`) and the queues shift down one lane
 // (wave_shl), so the inner loop touches no memory but lane 63's output.  bl is the
 // column byte of the lane's current column, passed down the lanes with the diagonal.
@@ -338,15 +338,19 @@ struct TileSetup {
 // outputs; false if the tile does not exist.  Hands the next segment of the stripe its
 // corner (the top boundary at this tile's last column).
 template <int R, int MODE>
-__device__ bool make_tile(const SweepParams& p, int64_t x, TileSetup& ts) {
+__device__ bool make_tile(const SweepParams& p, int64_t x, TileSetup& ts, int* span_io = nullptr) {
     constexpr int64_t HR = 64 * R;
     if (x >= p.ntiles) return false;
-    int lo = 0, hi = p.nspans - 1;
+    // span holding tile x: from the caller's span (x follows a tile of it) when given, whose
+    // successor usually starts beyond x, else by binary search over the launch's spans
+    int lo = span_io && *span_io >= 0 ? *span_io : 0, hi = p.nspans - 1;
+    if (span_io && *span_io >= 0 && (lo == hi || p.spans[lo + 1].first_tile > x)) hi = lo;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
         if (p.spans[mid].first_tile <= x) lo = mid;
         else hi = mid - 1;
     }
+    if (span_io) *span_io = lo;
     const TileSpan sp = p.spans[lo];
     const int64_t s = sp.s_lo + (x - sp.first_tile);
     const int64_t g = sp.d - s;
@@ -701,8 +705,9 @@ __device__ void sweep_main_pk(const SweepParams& p, SweepLds& lds) {
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int64_t x = (int64_t)blockIdx.x * kSweepWaves + wid;
     TileSetup ts[2];
-    if (!make_tile<R, MODE>(p, 2 * x, ts[0])) return;
-    if (!make_tile<R, MODE>(p, 2 * x + 1, ts[1])) {
+    int span = -1;
+    if (!make_tile<R, MODE>(p, 2 * x, ts[0], &span)) return;
+    if (!make_tile<R, MODE>(p, 2 * x + 1, ts[1], &span)) {
         // no second tile: an empty one (no rows, no columns, nothing stored)
         ts[1] = ts[0];
         ts[1].T.w = 0;
