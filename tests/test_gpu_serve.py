@@ -77,3 +77,28 @@ def test_poll_without_wait(sw):
             time.sleep(0.001)
             st, r = srv.poll(jid, wait=False)
         assert st == 0 and r["score"] > 0
+
+
+def test_served_mixed_alphabets_equal_alone(sw, ctx):
+    """Jobs of upper and lower case bytes beside ACGT jobs: ticks whose jobs' bytes no 3-bit
+    code separates score by byte compares, the others by symbol codes; every result equals
+    the job aligned alone."""
+    rng = random.Random(11)
+    pairs = _pairs(sw, 13, 10, [200, 900, 2500])
+    for k in range(4):
+        L = 600 + 400 * k
+        a = bytes(rng.choice(b"ACGTacgt") for _ in range(L))
+        b = bytearray(a[L // 10:])
+        for i in range(len(b)):
+            if rng.random() < 0.1:
+                b[i] = rng.choice(b"ACGTacgt")
+        pairs.insert(2 * k + 1, (a, bytes(b)))
+    alone = [_key(ctx.align(a, b, sw.scoring(*SC))) for a, b in pairs]
+    with sw.Server(sw.scoring(*SC), budget=1 << 30, max_in_flight=6) as srv:
+        ids = [srv.submit(a, b) for a, b in pairs]
+        got = []
+        for jid in ids:
+            st, r = srv.poll(jid)
+            assert st == 0
+            got.append(_key(r))
+    assert got == alone
