@@ -70,7 +70,8 @@ typedef struct {
     int32_t tb_lookahead;      /* traceback: checkpoint blocks recomputed ahead of the walk in  */
                                /* each direction per round (beyond 3, only blocks within 2 of  */
                                /* the diagonal); 0 = auto (7; batches 0), -1 = none            */
-    int32_t waves_per_simd;    /* sweep waves resident per SIMD (1 or 2), 0 = auto             */
+    int32_t waves_per_simd;    /* sweep waves resident per SIMD (1 to 4), 0 = auto (batches 4; */
+                               /* single pairs 1 or 2 by a model of the wavefront)             */
     int32_t autotune;          /* batch: 1 = time candidate settings on a sample first          */
     int32_t host_checkpoints;  /* 1: checkpoint rows and columns live in pinned host memory    */
                                /* mapped into the device (the reference's temp/row_R side      */
@@ -176,7 +177,7 @@ int swh_batch_align_device(swh_ctx* ctx, const swh_pair* pairs, int64_t npairs, 
                            const swh_opts* opts, swh_result* out, int32_t* statuses);
 
 /* Batch autotuner: times every combination of tile width {256, 384, 512, 768, 1024},
- * sweep waves per SIMD {1, 2} and checkpoint spacing {256, 512, 1024, 2048} on an evenly spaced
+ * sweep waves per SIMD {2, 3, 4} and checkpoint spacing {256, 512, 1024, 2048} on an evenly spaced
  * sample of `sample` pairs (device-resident pairs if `device` != 0) and reports the
  * fastest, to be passed back in swh_opts (tile_cols, waves_per_simd, checkpoint_rows and
  * checkpoint_cols).  Spacings whose checkpoints for the whole batch exceed hbm_budget_bytes

@@ -114,6 +114,8 @@ double sweep_model(swh_ctx* c, int64_t rows, int64_t cols, int wps) {
     return t * (wps == 2 ? kTwoWaveCost : 1.0);
 }
 
+int usable_wps(const swh_ctx* c, int w) { return w >= 1 && w <= kMaxSweepWps && c->lds_pad[w] ? w : 1; }
+
 // Sweep waves per SIMD for a rows x cols sweep: whichever of one or two the model prefers.
 int choose_wps(swh_ctx* c, int64_t rows, int64_t cols) {
     if (c->wps_fixed) return c->wps_fixed;
@@ -202,8 +204,8 @@ int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions,
         p.nspans = (int32_t)(span_at[(size_t)d + 1] - span_at[(size_t)d]);
         p.ntiles = tiles_at[(size_t)d];
         if (packed && (mode == kStage1 || mode == kDirs))
-            HCK(c, launch_sweep_pk(mode, p, c->lds_pad[wps == 2 ? 2 : 1], c->stream));
-        else HCK(c, launch_sweep(mode, p, c->lds_pad[wps == 2 ? 2 : 1], c->stream));
+            HCK(c, launch_sweep_pk(mode, p, c->lds_pad[usable_wps(c, wps)], c->stream));
+        else HCK(c, launch_sweep(mode, p, c->lds_pad[usable_wps(c, wps)], c->stream));
     }
     if (ev_after) HCK(c, hipEventRecord(ev_after, c->stream));
     if (launches) *launches = D;
@@ -346,6 +348,10 @@ struct EngineOpts {
 // Columns per tile of a batch sweep: wide enough that the 63 fill steps of a tile cost
 // about a tenth, narrow enough that short pairs still cut into several tiles.
 constexpr int64_t kBatchTileCols = 512;
+// Sweep waves per SIMD of a batch: a batch has tiles to spare, and four waves per SIMD hide
+// the per-tile set-up and memory waits that one or two leave exposed (65,536-pair batch,
+// forward sweep: 1210 / 763 / 668 / 650 ms at 1 / 2 / 3 / 4 waves per SIMD).
+constexpr int kBatchWps = 4;
 
 // Runs stage 1, the end search and (for jobs that want it) the traceback of every job,
 // all jobs side by side in each sweep.  Fills out[k] (result_init'ed by the caller) and
@@ -455,8 +461,7 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
     int wps = 1;
     int64_t W = 0;
     if (eo.batch) {
-        wps = eo.wps ? eo.wps : (c->wps_fixed ? c->wps_fixed : (c->lds_pad[2] ? 2 : 1));
-        if (wps == 2 && !c->lds_pad[2]) wps = 1;
+        wps = usable_wps(c, eo.wps ? eo.wps : (c->wps_fixed ? c->wps_fixed : kBatchWps));
         W = eo.tile_cols > 0 ? eo.tile_cols : kBatchTileCols;
         packed = packed && W <= w16;
     } else {
@@ -621,7 +626,7 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
     const int64_t tstride = ceil_div(Wtb + 63, 64) * 64;
     const int64_t Gblk = ceil_div(wmax, Wtb);
     const bool tb_packed = eo.batch && packed && Wtb <= w16;
-    const int tb_wps = eo.batch && c->lds_pad[2] ? 2 : 1;
+    const int tb_wps = eo.batch ? wps : 1;  // a batch's codes sweeps run as many waves per SIMD as its stage 1
     auto blk_dirs = [&](const Job& j) {
         return ceil_div(std::min(j.m, j.pl.K), kStripeRows) * Gblk * tstride * 64 * (int64_t)sizeof(uint16_t);
     };
@@ -852,7 +857,7 @@ EngineOpts engine_opts(const swh_opts* o, bool batch) {
     EngineOpts eo;
     eo.tile_cols = o ? o->tile_cols : 0;
     eo.precision = o ? o->precision : 0;
-    eo.wps = o && (o->waves_per_simd == 1 || o->waves_per_simd == 2) ? o->waves_per_simd : 0;
+    eo.wps = o && o->waves_per_simd >= 1 && o->waves_per_simd <= kMaxSweepWps ? o->waves_per_simd : 0;
     eo.want_xp = o && o->want_xpoints;
     eo.batch = batch;
     eo.host_ckpt = o && o->host_checkpoints;
@@ -913,24 +918,24 @@ int swh_ctx_create(int device, swh_ctx** out) {
         delete c;
         return SWH_ERR_HIP;
     }
-    // Reserve LDS per sweep workgroup so that exactly w of them fit a CU (w = 1, 2): the
+    // Reserve LDS per sweep workgroup so that exactly w of them fit a CU (w = 1..4): the
     // waves of a launch then spread evenly, w per SIMD (an uneven spread leaves some SIMDs
     // with more tiles than others, and a launch lasts as long as its busiest SIMD).
     const char* env = getenv("SWALIGN_SWEEP_WAVES_PER_SIMD");
-    if (env && (atoi(env) == 1 || atoi(env) == 2)) c->wps_fixed = atoi(env);
+    if (env && atoi(env) >= 1 && atoi(env) <= kMaxSweepWps) c->wps_fixed = atoi(env);
     // SWALIGN_SYMBOL_CODES=0 keeps the byte-compare score in every sweep
     const char* codes = getenv("SWALIGN_SYMBOL_CODES");
     c->codes_off = codes && atoi(codes) == 0;
     const char* trace = getenv("SWALIGN_TRACE");
     c->trace = trace && atoi(trace) != 0;
     const size_t per_cu = (size_t)c->props.maxSharedMemoryPerMultiProcessor, stat = sweep_static_lds();
-    for (int w = 1; w <= 2; ++w) {
+    for (int w = 1; w <= kMaxSweepWps; ++w) {
         const size_t want = per_cu / (size_t)(w + 1) + 1024;
         if (want > stat && want <= (size_t)c->props.sharedMemPerBlock) c->lds_pad[w] = want - stat;
     }
     if (c->lds_pad[1] && sweep_allow_lds(c->lds_pad[1]) != hipSuccess) {
         (void)hipGetLastError();
-        c->lds_pad[1] = c->lds_pad[2] = 0;
+        for (size_t& pad : c->lds_pad) pad = 0;
     }
     *out = c;
     return SWH_OK;
@@ -1098,7 +1103,7 @@ int tune_jobs(swh_ctx* c, const std::vector<Job>& jobs, const swh_scoring* sc, c
     std::vector<Job> base;
     for (int64_t k = 0; k < sample; ++k) base.push_back(jobs[(size_t)(k * n / sample)]);
     const int Ws[] = {256, 384, 512, 768, 1024};
-    const int Ps[] = {1, 2};
+    const int Ps[] = {2, 3, 4};
     const int64_t Ks[] = {256, 512, 1024, 2048};
     std::vector<swh_result> res((size_t)sample);
     double best = 1e300;
@@ -1126,13 +1131,14 @@ int tune_jobs(swh_ctx* c, const std::vector<Job>& jobs, const swh_scoring* sc, c
             for (const Job& j : jobs) need += plan_mem(j.m, j.n, K, K);
         if (need > budget) continue;
         for (int P : Ps) {
-            if (P == 2 && !c->lds_pad[2]) continue;
+            const int Pu = usable_wps(c, P);  // 1 where the LDS reservation is unavailable
+            if (Pu != P && P != Ps[0]) continue;
             for (int W : Ws) {
                 std::vector<Job> js = base;
                 swh_opts ot = o ? *o : swh_opts{};
                 ot.autotune = 0;
                 ot.tile_cols = W;
-                ot.waves_per_simd = P;
+                ot.waves_per_simd = Pu;
                 ot.checkpoint_rows = K;
                 ot.checkpoint_cols = K;
                 for (Job& j : js) {
@@ -1149,11 +1155,11 @@ int tune_jobs(swh_ctx* c, const std::vector<Job>& jobs, const swh_scoring* sc, c
                 if (rc == SWH_ERR_OUT_OF_MEMORY) continue;  // does not fit beside other allocations
                 if (rc) return rc;
                 ++cand;
-                if (W == kBatchTileCols && K == 1024 && P == (c->lds_pad[2] ? 2 : 1)) out->ms_default = ms;
+                if (W == kBatchTileCols && K == 1024 && Pu == usable_wps(c, kBatchWps)) out->ms_default = ms;
                 if (ms < best) {
                     best = ms;
                     out->tile_cols = W;
-                    out->waves_per_simd = P;
+                    out->waves_per_simd = Pu;
                     out->checkpoint_rows = K;
                     out->ms_best = ms;
                 }

@@ -23,6 +23,10 @@ struct HostBuf {
     size_t cap = 0;
 };
 
+// Sweep waves resident per SIMD that an LDS reservation can enforce (the packed sweep's
+// register use allows four).
+constexpr int kMaxSweepWps = 4;
+
 struct swh_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -35,7 +39,7 @@ struct swh_ctx {
     size_t host_ops_cap = 0;
     HostBuf hsrow, hscol;         // host-resident checkpoint rows / columns (swh_opts.host_checkpoints)
     hipEvent_t ev[8] = {};
-    size_t lds_pad[3] = {0, 0, 0};  // dynamic LDS per sweep workgroup for w workgroups per CU
+    size_t lds_pad[kMaxSweepWps + 1] = {};  // dynamic LDS per sweep workgroup for w workgroups per CU
     int wps_fixed = 0;              // sweep waves per SIMD forced by the environment, 0 = model
     bool codes_off = false;         // symbol-code scores disabled by the environment
     bool trace = false;             // SWALIGN_TRACE=1: host wall time of each stage to stderr
@@ -63,6 +67,8 @@ void result_init(swh_result* r);
 // Columns per tile: `want` if given, else one tile per resident sweep wave.
 int64_t tile_cols(swh_ctx* c, int64_t cols, int64_t want, int wps = 1);
 int choose_wps(swh_ctx* c, int64_t rows, int64_t cols);
+// w if the context can hold the sweep to w waves per SIMD (1 <= w <= kMaxSweepWps), else 1.
+int usable_wps(const swh_ctx* c, int w);
 
 // One sweep over `regions`, one launch per tile anti-diagonal (see sw_api.hip).
 int run_sweep(swh_ctx* c, int mode, swk::SweepParams p, std::vector<swk::Region>& regions, int64_t W,
