@@ -102,3 +102,15 @@ def test_served_mixed_alphabets_equal_alone(sw, ctx):
             assert st == 0
             got.append(_key(r))
     assert got == alone
+
+
+@pytest.mark.parametrize("wps", [1, 2])
+def test_served_waves_per_simd_same_result(sw, ctx, wps):
+    """The server's shared launches at one or two sweep waves per SIMD (the default is four)
+    give every job the result it gets alone."""
+    pairs = _pairs(sw, 17, 12, [150, 800, 2500, 7000])
+    alone = [_key(ctx.align(a, b, sw.scoring(*SC))) for a, b in pairs]
+    with sw.Server(sw.scoring(*SC), budget=1 << 30, max_in_flight=8, waves_per_simd=wps) as srv:
+        ids = [srv.submit(a, b) for a, b in pairs]
+        got = [_key(srv.poll(j)[1]) for j in ids]
+    assert got == alone
