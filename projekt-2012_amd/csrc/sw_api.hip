@@ -284,7 +284,8 @@ int64_t plan_mem(int64_t m, int64_t n, int64_t K, int64_t Kc) {
 // `budget` bytes.
 int choose_plan(int64_t m, int64_t n, const swh_opts* o, int64_t budget, Plan* pl) {
     int64_t K = o && o->checkpoint_rows > 0 ? ceil_div(o->checkpoint_rows, kStripeRows) * kStripeRows : 0;
-    int64_t Kc = o && o->checkpoint_cols > 0 ? next_pow2(o->checkpoint_cols) : 0;
+    // the sweeps take Kc as a power of two >= 64 (a lane meets one checkpoint column per chunk)
+    int64_t Kc = o && o->checkpoint_cols > 0 ? std::max<int64_t>(64, next_pow2(o->checkpoint_cols)) : 0;
     if (o && o->score_only) {
         K = INT64_C(1) << 62;
         Kc = INT64_C(1) << 62;

@@ -124,6 +124,23 @@ __device__ __forceinline__ void row_symbol(const SweepParams& p, bool in, uint32
     hi = (uint32_t)(prof >> 32);
 }
 
+// Checkpoint column of a stage-1 chunk: the step s in [0, 64) at which this lane's column
+// C0 + 64c + s - lane (tile columns [0, w)) is one (column + 1 a multiple of Kc), or -1; dst
+// receives the lane's first row of that checkpoint column.  Kc is a power of two >= 64, so a
+// lane meets at most one checkpoint column in the 64 steps of a chunk.
+__device__ __forceinline__ int dump_step(int64_t C0, int64_t w, int64_t Kc, int2* scol, int64_t m, int64_t c,
+                                         int64_t row0, int2*& dst) {
+    dst = nullptr;
+    if (!scol) return -1;
+    const int64_t x0 = 64 * c - wave_lane();  // the lane's tile column at step 0
+    const int64_t X1 = C0 + x0 + 1;
+    const int64_t st = (-X1) & (Kc - 1);      // steps to the next column whose index + 1 is a multiple of Kc
+    const int64_t x = x0 + st;
+    if (st >= 64 || x < 0 || x >= w) return -1;
+    dst = scol + (((X1 + st) >> __builtin_ctzll((unsigned long long)Kc)) - 1) * m + row0;
+    return (int)st;
+}
+
 // 64 steps of one tile.  The chunk's inputs wait in register queues, one column per
 // lane: Qh/Qf hold the top boundary, Qb the column symbols; each step lane 0 takes the head
 // of the queues (DPP wave_shr with the queue as `old`) and the queues shift down one lane
@@ -142,6 +159,8 @@ __device__ __forceinline__ void chunk_steps(const SweepParams& p, const TileCtx&
     const int64_t w = T.w;
     uint16_t* dcol = nullptr;
     if (MODE == kDirs) dcol = T.dirs + (64 * c) * 64 + lane;
+    int2* ddst = nullptr;
+    const int sdump = DUMP ? dump_step(T.C0, T.w, T.Kc, T.scol, T.m, c, row0, ddst) : -1;
     SWK_UNROLL(UNROLL_STEPS)
     for (int s = 0; s < 64; ++s) {
         int Ho_up = shr1(Qh, Ho_bot);
@@ -198,15 +217,10 @@ __device__ __forceinline__ void chunk_steps(const SweepParams& p, const TileCtx&
         Fe_bot = Fe_up;
         if (lane == 63) s_out[s] = make_int2(Ho_bot, Fe_bot);
         if (MODE == kDirs) dcol[s * 64] = (uint16_t)nib;
-        if (DUMP) {
-            const int64_t x = 64 * c + s - lane;
-            const int64_t X = T.C0 + x;
-            if (x >= 0 && x < w && ((X + 1) & (T.Kc - 1)) == 0) {
-                int2* dst = T.scol + ((X + 1) / T.Kc - 1) * T.m;
+        if (DUMP && s == sdump) {
 #pragma unroll
-                for (int k = 0; k < R; ++k)
-                    if (row0 + k <= T.r1) dst[row0 + k] = make_int2(Ho[k] - O, E[k]);
-            }
+            for (int k = 0; k < R; ++k)
+                if (row0 + k <= T.r1) ddst[k] = make_int2(Ho[k] - O, E[k]);
         }
     }
 }
@@ -465,6 +479,12 @@ __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileC
 #pragma unroll
         for (int t = 0; t < 2; ++t) dcol[t] = T[t].dirs ? T[t].dirs + (64 * c) * 64 + lane : nullptr;
     }
+    int2* ddst[2] = {nullptr, nullptr};
+    int sdump[2] = {-1, -1};
+    if (DUMP) {
+#pragma unroll
+        for (int t = 0; t < 2; ++t) sdump[t] = dump_step(T[t].C0, T[t].w, T[t].Kc, T[t].scol, T[t].m, c, row0[t], ddst[t]);
+    }
     SWK_UNROLL(UNROLL_STEPS)
     for (int s = 0; s < 64; ++s) {
         s16x2 Ho_up = pk_from(shr1(Qh, pk_bits(Ho_bot)));
@@ -533,14 +553,11 @@ __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileC
         if (DUMP) {
 #pragma unroll
             for (int t = 0; t < 2; ++t) {
-                const int64_t x = 64 * c + s - lane;
-                const int64_t X = T[t].C0 + x;
-                if (T[t].scol && x >= 0 && x < T[t].w && ((X + 1) & (T[t].Kc - 1)) == 0) {
-                    int2* dst = T[t].scol + ((X + 1) / T[t].Kc - 1) * T[t].m;
+                if (s == sdump[t]) {
 #pragma unroll
                     for (int k = 0; k < R; ++k)
                         if (row0[t] + k <= T[t].r1)
-                            dst[row0[t] + k] = make_int2(Ho[k][t] - p.sc.O + base[t], E[k][t] + base[t]);
+                            ddst[t][k] = make_int2(Ho[k][t] - p.sc.O + base[t], E[k][t] + base[t]);
                 }
             }
         }
