@@ -161,6 +161,7 @@ __device__ __forceinline__ void chunk_steps(const SweepParams& p, const TileCtx&
     if (MODE == kDirs) dcol = T.dirs + (64 * c) * 64 + lane;
     int2* ddst = nullptr;
     const int sdump = DUMP ? dump_step(T.C0, T.w, T.Kc, T.scol, T.m, c, row0, ddst) : -1;
+    const uint32_t col0 = (uint32_t)(64 * c) - (uint32_t)lane;  // the lane's tile column at step 0
     SWK_UNROLL(UNROLL_STEPS)
     for (int s = 0; s < 64; ++s) {
         int Ho_up = shr1(Qh, Ho_bot);
@@ -171,10 +172,13 @@ __device__ __forceinline__ void chunk_steps(const SweepParams& p, const TileCtx&
         Qb = shl1(Qb);
         const uint32_t b = (uint32_t)bl;
         bool valid = true;
-        int64_t col = 0;
+        int col = 0;
         if (EDGE || MODE == kSearch) {
-            col = 64 * c + s - lane;
-            valid = col >= 0 && col < w;
+            // tile column 64c + s - lane, in [-63, w + 126]: mod 2^32, the columns below 0
+            // compare above any width
+            const uint32_t ucol = col0 + (uint32_t)s;
+            valid = ucol < (uint32_t)w;
+            col = (int)ucol;
         }
         int diag = Ho_up_prev;
         if (EDGE) Ho_up_prev = valid ? Ho_up : Ho_up_prev;
@@ -290,6 +294,9 @@ __device__ void sweep_tile(const SweepParams& p, const TileCtx& T, const int2* l
             else
                 chunk_steps<R, MODE, PROF, true, false>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
                                                   Qh, Qf, Qb, bl, s_out, found_col);
+        } else if (interior) {
+            chunk_steps<R, MODE, PROF, false, true>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot, Qh, Qf,
+                                              Qb, bl, s_out, found_col);
         } else {
             chunk_steps<R, MODE, PROF, true, true>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot, Qh, Qf,
                                              Qb, bl, s_out, found_col);
@@ -479,6 +486,7 @@ __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileC
 #pragma unroll
         for (int t = 0; t < 2; ++t) dcol[t] = T[t].dirs ? T[t].dirs + (64 * c) * 64 + lane : nullptr;
     }
+    const uint32_t col0 = (uint32_t)(64 * c) - (uint32_t)lane;  // the lane's tile column at step 0
     int2* ddst[2] = {nullptr, nullptr};
     int sdump[2] = {-1, -1};
     if (DUMP) {
@@ -495,8 +503,9 @@ __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileC
         Qb = shl1(Qb);
         int vmask = -1;
         if (EDGE) {
-            const int64_t col = 64 * c + s - lane;
-            vmask = ((col >= 0 && col < T[0].w) ? 0x0000FFFF : 0) | ((col >= 0 && col < T[1].w) ? (int)0xFFFF0000 : 0);
+            // tile column 64c + s - lane mod 2^32: the columns below 0 compare above any width
+            const uint32_t col = col0 + (uint32_t)s;
+            vmask = (col < (uint32_t)T[0].w ? 0x0000FFFF : 0) | (col < (uint32_t)T[1].w ? (int)0xFFFF0000 : 0);
         }
         s16x2 diag = Ho_up_prev;
         Ho_up_prev = EDGE ? pk_sel(vmask, Ho_up, Ho_up_prev) : Ho_up;
@@ -676,6 +685,9 @@ __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], Sw
             else
                 chunk_steps_pk<R, MODE, PROF, true, false>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
                                                      Qh, Qf, Qb, bl, s_out, zero, base);
+        } else if (interior) {
+            chunk_steps_pk<R, MODE, PROF, false, true>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
+                                                 Qh, Qf, Qb, bl, s_out, zero, base);
         } else {
             chunk_steps_pk<R, MODE, PROF, true, true>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot, Qh,
                                                 Qf, Qb, bl, s_out, zero, base);
