@@ -147,8 +147,12 @@ __device__ __forceinline__ int dump_step(int64_t C0, int64_t w, int64_t Kc, int2
 // (wave_shl), so the inner loop touches no memory but lane 63's output.  bl is the
 // column byte of the lane's current column, passed down the lanes with the diagonal.
 // EDGE: some (lane, step) pairs fall outside [0, w) and must leave the lane state
-// untouched.  DUMP: stage-1 chunk containing a checkpoint column.
-template <int R, int MODE, bool PROF, bool EDGE, bool DUMP>
+// untouched.  DUMP: stage-1 chunk containing a checkpoint column.  START: the first chunk of
+// a tile at least 64 columns wide, where the only steps outside [0, w) are those before a
+// lane's first column (step < lane): those lanes sit the step out under the EXEC mask, which
+// leaves their state untouched without per-value selects; the queue shifts above the branch
+// run on every lane.
+template <int R, int MODE, bool PROF, bool EDGE, bool DUMP, bool START = false>
 __device__ __forceinline__ void chunk_steps(const SweepParams& p, const TileCtx& T, int64_t c, int64_t row0,
                                             const RowSyms<R>& a, int (&Ho)[R], int (&E)[R], int (&rmax)[R],
                                             int& Ho_up_prev, int& Ho_bot, int& Fe_bot, int& Qh, int& Qf, int& Qb,
@@ -170,6 +174,7 @@ __device__ __forceinline__ void chunk_steps(const SweepParams& p, const TileCtx&
         Qh = shl1(Qh);
         Qf = shl1(Qf);
         Qb = shl1(Qb);
+        if (START && lane > s) continue;  // before the lane's first column
         const uint32_t b = (uint32_t)bl;
         bool valid = true;
         int col = 0;
@@ -291,6 +296,9 @@ __device__ void sweep_tile(const SweepParams& p, const TileCtx& T, const int2* l
             if (interior)
                 chunk_steps<R, MODE, PROF, false, false>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
                                                    Qh, Qf, Qb, bl, s_out, found_col);
+            else if (c == 0 && w >= 64)
+                chunk_steps<R, MODE, PROF, false, false, true>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot,
+                                                         Fe_bot, Qh, Qf, Qb, bl, s_out, found_col);
             else
                 chunk_steps<R, MODE, PROF, true, false>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
                                                   Qh, Qf, Qb, bl, s_out, found_col);
@@ -470,7 +478,7 @@ __device__ __forceinline__ s16x2 pk_sel(int mask, s16x2 a, s16x2 b) {
 }
 
 // 64 steps of two tiles (see chunk_steps).  EDGE: per-half validity masks.
-template <int R, int MODE, bool PROF, bool EDGE, bool DUMP>
+template <int R, int MODE, bool PROF, bool EDGE, bool DUMP, bool START = false>
 __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileCtx (&T)[2], int64_t c,
                                                const int64_t (&row0)[2], const int (&a)[R], const int (&a2)[R],
                                                s16x2 (&Ho)[R],
@@ -501,6 +509,7 @@ __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileC
         Qh = shl1(Qh);
         Qf = shl1(Qf);
         Qb = shl1(Qb);
+        if (START && lane > s) continue;  // before the lane's first column (see chunk_steps)
         int vmask = -1;
         if (EDGE) {
             // tile column 64c + s - lane mod 2^32: the columns below 0 compare above any width
@@ -682,6 +691,9 @@ __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], Sw
             if (interior)
                 chunk_steps_pk<R, MODE, PROF, false, false>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
                                                       Qh, Qf, Qb, bl, s_out, zero, base);
+            else if (c == 0 && min(T[0].w, T[1].w) >= 64)
+                chunk_steps_pk<R, MODE, PROF, false, false, true>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev, Ho_bot,
+                                                            Fe_bot, Qh, Qf, Qb, bl, s_out, zero, base);
             else
                 chunk_steps_pk<R, MODE, PROF, true, false>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
                                                      Qh, Qf, Qb, bl, s_out, zero, base);
