@@ -151,8 +151,9 @@ __device__ __forceinline__ int dump_step(int64_t C0, int64_t w, int64_t Kc, int2
 // a tile at least 64 columns wide, where the only steps outside [0, w) are those before a
 // lane's first column (step < lane): those lanes sit the step out under the EXEC mask, which
 // leaves their state untouched without per-value selects; the queue shifts above the branch
-// run on every lane.
-template <int R, int MODE, bool PROF, bool EDGE, bool DUMP, bool START = false>
+// run on every lane.  END: a later chunk, where the only steps outside [0, w) are those past
+// a lane's last column; those lanes sit out the same way.
+template <int R, int MODE, bool PROF, bool EDGE, bool DUMP, bool START = false, bool END = false>
 __device__ __forceinline__ void chunk_steps(const SweepParams& p, const TileCtx& T, int64_t c, int64_t row0,
                                             const RowSyms<R>& a, int (&Ho)[R], int (&E)[R], int (&rmax)[R],
                                             int& Ho_up_prev, int& Ho_bot, int& Fe_bot, int& Qh, int& Qf, int& Qb,
@@ -166,6 +167,7 @@ __device__ __forceinline__ void chunk_steps(const SweepParams& p, const TileCtx&
     int2* ddst = nullptr;
     const int sdump = DUMP ? dump_step(T.C0, T.w, T.Kc, T.scol, T.m, c, row0, ddst) : -1;
     const uint32_t col0 = (uint32_t)(64 * c) - (uint32_t)lane;  // the lane's tile column at step 0
+    const int s_end = END ? (int)(w - (64 * c - lane)) : 64;  // first step past column w - 1
     SWK_UNROLL(UNROLL_STEPS)
     for (int s = 0; s < 64; ++s) {
         int Ho_up = shr1(Qh, Ho_bot);
@@ -175,6 +177,7 @@ __device__ __forceinline__ void chunk_steps(const SweepParams& p, const TileCtx&
         Qf = shl1(Qf);
         Qb = shl1(Qb);
         if (START && lane > s) continue;  // before the lane's first column
+        if (END && s >= s_end) continue;  // past the lane's last column
         const uint32_t b = (uint32_t)bl;
         bool valid = true;
         int col = 0;
@@ -299,12 +302,18 @@ __device__ void sweep_tile(const SweepParams& p, const TileCtx& T, const int2* l
             else if (c == 0 && w >= 64)
                 chunk_steps<R, MODE, PROF, false, false, true>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot,
                                                          Fe_bot, Qh, Qf, Qb, bl, s_out, found_col);
+            else if (c >= 1)  // no column below 0 from the second chunk on
+                chunk_steps<R, MODE, PROF, false, false, false, true>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev,
+                                                                Ho_bot, Fe_bot, Qh, Qf, Qb, bl, s_out, found_col);
             else
                 chunk_steps<R, MODE, PROF, true, false>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
                                                   Qh, Qf, Qb, bl, s_out, found_col);
         } else if (interior) {
             chunk_steps<R, MODE, PROF, false, true>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot, Qh, Qf,
                                               Qb, bl, s_out, found_col);
+        } else if (c >= 1) {
+            chunk_steps<R, MODE, PROF, false, true, false, true>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot,
+                                                           Fe_bot, Qh, Qf, Qb, bl, s_out, found_col);
         } else {
             chunk_steps<R, MODE, PROF, true, true>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot, Qh, Qf,
                                              Qb, bl, s_out, found_col);
@@ -478,7 +487,7 @@ __device__ __forceinline__ s16x2 pk_sel(int mask, s16x2 a, s16x2 b) {
 }
 
 // 64 steps of two tiles (see chunk_steps).  EDGE: per-half validity masks.
-template <int R, int MODE, bool PROF, bool EDGE, bool DUMP, bool START = false>
+template <int R, int MODE, bool PROF, bool EDGE, bool DUMP, bool START = false, bool END = false>
 __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileCtx (&T)[2], int64_t c,
                                                const int64_t (&row0)[2], const int (&a)[R], const int (&a2)[R],
                                                s16x2 (&Ho)[R],
@@ -495,6 +504,7 @@ __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileC
         for (int t = 0; t < 2; ++t) dcol[t] = T[t].dirs ? T[t].dirs + (64 * c) * 64 + lane : nullptr;
     }
     const uint32_t col0 = (uint32_t)(64 * c) - (uint32_t)lane;  // the lane's tile column at step 0
+    const int s_end = END ? (int)(T[0].w - (64 * c - lane)) : 64;  // first step past column w - 1
     int2* ddst[2] = {nullptr, nullptr};
     int sdump[2] = {-1, -1};
     if (DUMP) {
@@ -510,6 +520,7 @@ __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileC
         Qf = shl1(Qf);
         Qb = shl1(Qb);
         if (START && lane > s) continue;  // before the lane's first column (see chunk_steps)
+        if (END && s >= s_end) continue;  // past the lane's last column (both tiles: equal widths)
         int vmask = -1;
         if (EDGE) {
             // tile column 64c + s - lane mod 2^32: the columns below 0 compare above any width
@@ -694,12 +705,18 @@ __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], Sw
             else if (c == 0 && min(T[0].w, T[1].w) >= 64)
                 chunk_steps_pk<R, MODE, PROF, false, false, true>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev, Ho_bot,
                                                             Fe_bot, Qh, Qf, Qb, bl, s_out, zero, base);
+            else if (c >= 1 && T[0].w == T[1].w)  // no column below 0, one last column for both tiles
+                chunk_steps_pk<R, MODE, PROF, false, false, false, true>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev,
+                                                                   Ho_bot, Fe_bot, Qh, Qf, Qb, bl, s_out, zero, base);
             else
                 chunk_steps_pk<R, MODE, PROF, true, false>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
                                                      Qh, Qf, Qb, bl, s_out, zero, base);
         } else if (interior) {
             chunk_steps_pk<R, MODE, PROF, false, true>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
                                                  Qh, Qf, Qb, bl, s_out, zero, base);
+        } else if (c >= 1 && T[0].w == T[1].w) {
+            chunk_steps_pk<R, MODE, PROF, false, true, false, true>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev,
+                                                              Ho_bot, Fe_bot, Qh, Qf, Qb, bl, s_out, zero, base);
         } else {
             chunk_steps_pk<R, MODE, PROF, true, true>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot, Qh,
                                                 Qf, Qb, bl, s_out, zero, base);
