@@ -252,3 +252,21 @@ def test_lookahead_band_with_oblong_blocks(sw, orc, ctx):
         got = ctx.align(a, b, sc, K=256, Kc=64, xpoints=True, lookahead=L)
         _cmp(got, ref)
         assert got["xpoints"] == [tuple(x) for x in ref["xpoints"]], L
+
+
+@pytest.mark.parametrize("W", [37, 63, 64, 65, 100, 127, 200])
+@pytest.mark.parametrize("Kc", [64, 128])
+@pytest.mark.parametrize("precision", [32, 16])
+def test_chunk_variants_by_tile_width(sw, orc, ctx, W, Kc, precision):
+    """Tile widths around one chunk (64 columns) and checkpoint columns every 64 or 128
+    columns put checkpoint columns in a tile's first, interior and last chunks, with tiles
+    narrower than a chunk and packed waves whose two tiles differ in width; every chunk
+    variant (edge masks, lanes sitting out before their first or after their last column,
+    checkpoint dumps) gives the oracle's alignment and crosspoints."""
+    (a, b), sc = sw.synth_pair(7 * W + Kc + precision, 1100, 1000, 700, 0.1, 0.02), sw.scoring(1, -3, -5, -2)
+    got = ctx.align(a, b, sc, K=256, Kc=Kc, xpoints=True, tile_cols=W, precision=precision)
+    assert got["precision"] == precision
+    A, B = orient(a, b)
+    ref = orc.align_linear(A, B, orc.scoring(1, -3, -5, -2), 256, Kc)
+    _cmp(got, ref)
+    assert got["xpoints"] == [tuple(x) for x in ref["xpoints"]]
