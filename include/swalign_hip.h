@@ -213,6 +213,10 @@ typedef struct {
     int64_t hbm_pool_bytes, hbm_in_use_bytes;
     int32_t loop_status;      /* SWH_OK, or the runtime failure that stopped the server       */
     int32_t _pad;
+    int64_t peak_in_flight;       /* most jobs in flight at once                                */
+    int64_t admitted_mid_flight;  /* jobs admitted while another job in flight was mid-way      */
+    int64_t sweep_launches;       /* sweep kernel launches                                      */
+    int64_t shared_launches;      /* of those, launches holding tiles of two or more jobs       */
 } swh_serve_stats;
 
 int swh_server_create(int device, const swh_scoring* sc, const swh_opts* opts, int64_t hbm_budget_bytes,

@@ -113,7 +113,9 @@ def align_full(a, b, sc):
     return _result(al, cig)
 
 
-def align_linear(a, b, sc, K, Kc):
+def align_linear(a, b, sc, K, Kc, threads=1):
+    """Linear-space alignment (checkpoint method); threads > 1 runs its stage 1 on that many
+    host threads (orc_align_linear_mt), with the same result."""
     A, pa, m = _buf(a)
     B, pb, n = _buf(b)
     al = Alignment()
@@ -122,9 +124,14 @@ def align_linear(a, b, sc, K, Kc):
     xcap = m // max(K, 1) + 8
     xp = (XPoint * xcap)()
     nxp = C.c_int64()
-    rc = lib().orc_align_linear(pa, C.c_int64(m), pb, C.c_int64(n), C.byref(sc), C.c_int64(K),
-                                C.c_int64(Kc), C.byref(al), cig, C.c_int64(cap), xp,
-                                C.c_int64(xcap), C.byref(nxp))
+    if threads > 1:
+        rc = lib().orc_align_linear_mt(pa, C.c_int64(m), pb, C.c_int64(n), C.byref(sc), C.c_int64(K),
+                                       C.c_int64(Kc), C.c_int(threads), C.byref(al), cig, C.c_int64(cap), xp,
+                                       C.c_int64(xcap), C.byref(nxp))
+    else:
+        rc = lib().orc_align_linear(pa, C.c_int64(m), pb, C.c_int64(n), C.byref(sc), C.c_int64(K),
+                                    C.c_int64(Kc), C.byref(al), cig, C.c_int64(cap), xp,
+                                    C.c_int64(xcap), C.byref(nxp))
     assert rc == 0
     r = _result(al, cig)
     r["xpoints"] = [(xp[k].row, xp[k].col, xp[k].score, xp[k].layer)

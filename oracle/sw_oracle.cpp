@@ -410,21 +410,74 @@ int orc_checkpoints(const uint8_t* A, int64_t m, const uint8_t* B, int64_t n, co
     return 0;
 }
 
-int orc_align_linear(const uint8_t* A, int64_t m, const uint8_t* B, int64_t n, const orc_scoring* sc,
-                     int64_t K, int64_t Kc, orc_alignment* out, char* cigar, int64_t cigar_cap,
-                     orc_xpoint* xp, int64_t xp_cap, int64_t* n_xp) {
+}  // extern "C"
+
+namespace {
+
+/* Stage 1 of the linear-space method on `threads` host threads: checkpoint rows {H,F}
+ * (rows K-1, 2K-1, ...), checkpoint columns {H,E} (columns Kc-1, 2Kc-1, ...) and row
+ * maxima, by the tiled anti-diagonal wavefront of orc_forward_mt.  Tiles are 256 rows by
+ * min(2048, Kc) columns, so every checkpoint row is a tile row's last row and every
+ * checkpoint column a tile's last column (K a multiple of 256, Kc a multiple of the tile
+ * width); other spacings run on one thread. */
+void linear_stage1(const uint8_t* A, int64_t m, const uint8_t* B, int64_t n, const orc_scoring* sc,
+                   int64_t K, int64_t Kc, int threads, int32_t* srow, int32_t* scol, int32_t* rmax) {
+    const int64_t TR = 256, TC = std::min<int64_t>(2048, Kc);
+    if (threads <= 1 || m < 1024 || n < 1024 || K % TR != 0 || Kc % TC != 0) {
+        orc_checkpoints(A, m, B, n, sc, K, Kc, srow, scol);
+        int32_t sc0;
+        int64_t ei, ej;
+        orc_forward(A, m, B, n, sc, &sc0, &ei, &ej, rmax);
+        return;
+    }
+    Sc s(sc);
+    const int64_t nrt = (m + TR - 1) / TR, nct = (n + TC - 1) / TC;
+    std::vector<int32_t> bndH(n, 0), bndF(n, 0);
+    std::fill(rmax, rmax + m, 0);
+    std::unique_ptr<std::atomic<int64_t>[]> prog(new std::atomic<int64_t>[nrt]);
+    for (int64_t t = 0; t < nrt; ++t) prog[t].store(0);
+    auto worker = [&](int tid) {
+        std::vector<int32_t> lH(TR), lE(TR);
+        for (int64_t ti = tid; ti < nrt; ti += threads) {
+            const int64_t r0 = ti * TR, r1 = std::min(m, r0 + TR) - 1;
+            std::fill(lH.begin(), lH.end(), 0);
+            std::fill(lE.begin(), lE.end(), 0);
+            int32_t corner = 0;
+            for (int64_t tj = 0; tj < nct; ++tj) {
+                const int64_t c0 = tj * TC, c1 = std::min(n, c0 + TC) - 1;
+                if (ti > 0)
+                    while (prog[ti - 1].load(std::memory_order_acquire) < tj + 1) std::this_thread::yield();
+                const int32_t next_corner = bndH[c1];
+                rect_forward(A, B, s, r0, r1, c0, c1, bndH.data() + c0, bndF.data() + c0, lH.data(),
+                             lE.data(), corner, rmax + r0, nullptr);
+                if ((r1 + 1) % K == 0 && srow) {  // checkpoint row r1, columns c0..c1
+                    int32_t* p = srow + ((r1 + 1) / K - 1) * n * 2;
+                    for (int64_t c = c0; c <= c1; ++c) { p[2 * c] = bndH[c]; p[2 * c + 1] = bndF[c]; }
+                }
+                if ((c1 + 1) % Kc == 0 && scol) {  // checkpoint column c1, rows r0..r1
+                    int32_t* p = scol + ((c1 + 1) / Kc - 1) * m * 2;
+                    for (int64_t r = r0; r <= r1; ++r) { p[2 * r] = lH[r - r0]; p[2 * r + 1] = lE[r - r0]; }
+                }
+                corner = (ti > 0) ? next_corner : 0;
+                prog[ti].store(tj + 1, std::memory_order_release);
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; ++t) pool.emplace_back(worker, t);
+    for (auto& th : pool) th.join();
+}
+
+int align_linear_impl(const uint8_t* A, int64_t m, const uint8_t* B, int64_t n, const orc_scoring* sc,
+                      int64_t K, int64_t Kc, int threads, orc_alignment* out, char* cigar, int64_t cigar_cap,
+                      orc_xpoint* xp, int64_t xp_cap, int64_t* n_xp) {
     if (K < 1 || Kc < 1) return 1;
     Sc s(sc);
     Alignment al;
     const int64_t nsr = m / K, nsc = n / Kc;  // checkpoint rows K-1,2K-1,..; columns Kc-1,..
     std::vector<int32_t> srow((size_t)(nsr * n * 2)), scol((size_t)(nsc * m * 2)), rmax(m, 0);
     // Stage 1: forward pass writing checkpoints and row maxima.
-    orc_checkpoints(A, m, B, n, sc, K, Kc, srow.data(), scol.data());
-    {
-        int32_t sc0;
-        int64_t ei, ej;
-        orc_forward(A, m, B, n, sc, &sc0, &ei, &ej, rmax.data());
-    }
+    linear_stage1(A, m, B, n, sc, K, Kc, threads, srow.data(), scol.data(), rmax.data());
     int32_t best = 0;
     int64_t bi = -1;
     for (int64_t i = 0; i < m; ++i)
@@ -501,6 +554,22 @@ int orc_align_linear(const uint8_t* A, int64_t m, const uint8_t* B, int64_t n, c
     if (xp)
         for (int64_t k = 0; k < (int64_t)al.xp.size() && k < xp_cap; ++k) xp[k] = al.xp[k];
     return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_align_linear(const uint8_t* A, int64_t m, const uint8_t* B, int64_t n, const orc_scoring* sc,
+                     int64_t K, int64_t Kc, orc_alignment* out, char* cigar, int64_t cigar_cap,
+                     orc_xpoint* xp, int64_t xp_cap, int64_t* n_xp) {
+    return align_linear_impl(A, m, B, n, sc, K, Kc, 1, out, cigar, cigar_cap, xp, xp_cap, n_xp);
+}
+
+int orc_align_linear_mt(const uint8_t* A, int64_t m, const uint8_t* B, int64_t n, const orc_scoring* sc,
+                        int64_t K, int64_t Kc, int threads, orc_alignment* out, char* cigar,
+                        int64_t cigar_cap, orc_xpoint* xp, int64_t xp_cap, int64_t* n_xp) {
+    return align_linear_impl(A, m, B, n, sc, K, Kc, threads, out, cigar, cigar_cap, xp, xp_cap, n_xp);
 }
 
 int orc_crosspoint_algebra(const uint8_t* A, int64_t m, const uint8_t* B, int64_t n,

@@ -90,6 +90,15 @@ int orc_align_linear(const uint8_t* A, int64_t m, const uint8_t* B, int64_t n,
                      char* cigar, int64_t cigar_cap, orc_xpoint* xp, int64_t xp_cap,
                      int64_t* n_xp);
 
+/* orc_align_linear with its stage 1 (checkpoints and row maxima) on `threads` host threads
+ * (tiled wavefront as in orc_forward_mt when K is a multiple of 256 and Kc a multiple of
+ * min(2048, Kc), else one thread); the end search and traceback are orc_align_linear's.
+ * Same outputs as orc_align_linear, checked against it on the CPU. */
+int orc_align_linear_mt(const uint8_t* A, int64_t m, const uint8_t* B, int64_t n,
+                        const orc_scoring* sc, int64_t K, int64_t Kc, int threads,
+                        orc_alignment* out, char* cigar, int64_t cigar_cap, orc_xpoint* xp,
+                        int64_t xp_cap, int64_t* n_xp);
+
 /* Checkpoint rows (H,F) of rows K-1, 2K-1, ... (< m) and checkpoint columns (H,E) of
  * columns Kc-1, 2Kc-1, ... (< n).  srow: nsr*n*2 int32, scol: nsc*m*2 int32. */
 int orc_checkpoints(const uint8_t* A, int64_t m, const uint8_t* B, int64_t n,

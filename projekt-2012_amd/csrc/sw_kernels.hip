@@ -936,10 +936,13 @@ __device__ void walk_main(const WalkArgs& a, WalkLds<R>& lds) {
         }
         const int64_t rows_here = min((int64_t)HR, rgn.r0 + rgn.rows - R0);
         for (int k = tid; k < HR; k += kWalkThreads) lds.a[k] = k < rows_here ? a.A[R0 + k] : 0;
-        // columns of the band: col = t - l for t in [t0, t1], l in [0, 64): [t0 - 63, t1]
+        // columns of the band: col = t - l for t in [t0, t1], l in [0, 64): [t0 - 63, t1].  The
+        // walk only moves left, so columns right of the current one (cj0; C0 + cj0 < n) are
+        // never read and not loaded: B may end at the current cell's column.
+        const int cj_hi = min(t1, cj0);
         for (int k = tid; k < kWalkSteps + 64; k += kWalkThreads) {
             const int cj = t0 - 63 + k;
-            lds.b[k] = (cj >= 0 && cj <= t1) ? a.B[C0 + cj] : 0;
+            lds.b[k] = (cj >= 0 && cj <= cj_hi) ? a.B[C0 + cj] : 0;
         }
         __syncthreads();
         if (tid < 64) {

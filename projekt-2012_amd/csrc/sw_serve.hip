@@ -159,6 +159,12 @@ struct swh_server {
     std::thread loop;
     // statistics
     int64_t submitted = 0, completed = 0, failed = 0, cells_done = 0;
+    int64_t peak_in_flight = 0;     // most jobs in flight at once
+    int64_t admitted_mid_flight = 0;  // jobs admitted while another job in flight had started
+    int64_t sweep_launches = 0, shared_launches = 0;  // sweep launches; those covering two or more jobs
+    int64_t fail_tick = -1;         // test hook (SWALIGN_SERVE_FAIL_TICK): tick that reports a failure
+    int64_t ticks = 0;
+    bool loop_exited = false;
     std::vector<double> lat_ms;
     Clock::time_point t_first{}, t_last{};
     bool any = false;
@@ -342,6 +348,7 @@ constexpr int kDiagonalsPerTick = 8;
 
 int tick(swh_server* s) {
     swh_ctx* c = s->ctx;
+    if (s->fail_tick >= 0 && s->ticks++ == s->fail_tick) return SWH_ERR_INTERNAL;  // injected from the host
     TickSlot& slot = s->ring[s->ring_at];
     s->ring_at = (s->ring_at + 1) % kTickRing;
     if (slot.pending) {
@@ -368,11 +375,13 @@ int tick(swh_server* s) {
         for (int step = 0; step < kDiagonalsPerTick; ++step) {
             span_at[mo].push_back((int64_t)spans[mo].size());
             int64_t t = 0;
+            int64_t jobs_here = 0;
             for (size_t x = 0; x < s->flight.size(); ++x) {
                 SJob* j = s->flight[x];
                 if (j->phase != kSweep || j->mode != mo) continue;
                 const int64_t d = j->d + step;
                 if (d >= j->D) continue;
+                ++jobs_here;
                 for (size_t q = 0; q < j->regs.size(); ++q) {
                     const int64_t Sq = j->Sr[q], Gq = j->Gr[q];
                     if (d > Sq + Gq - 2) continue;
@@ -382,6 +391,10 @@ int tick(swh_server* s) {
                 }
             }
             tiles_at[mo].push_back(t);
+            if (t) {
+                ++s->sweep_launches;
+                if (jobs_here > 1) ++s->shared_launches;
+            }
         }
         span_at[mo].push_back((int64_t)spans[mo].size());
     }
@@ -558,7 +571,19 @@ int tick(swh_server* s) {
 }
 
 void serve_loop(swh_server* s) {
-    if (hipSetDevice(s->ctx->device) != hipSuccess) return;
+    if (hipSetDevice(s->ctx->device) != hipSuccess) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->loop_status = SWH_ERR_HIP;
+        s->loop_exited = true;
+        for (SJob* j : s->queue) {
+            finish(s, *j, SWH_ERR_HIP);
+            j->published = true;
+            ++s->failed;
+        }
+        s->queue.clear();
+        s->cv_done.notify_all();
+        return;
+    }
     while (true) {
         std::vector<SJob*> done;
         {
@@ -576,10 +601,18 @@ void serve_loop(swh_server* s) {
                 if (off < 0) break;
                 s->queue.pop_front();
                 j->off = off;
+                bool mid = false;  // another job in flight has already run part of its work
+                for (SJob* o : s->flight) mid = mid || o->phase != kQueued;
+                if (mid) ++s->admitted_mid_flight;
                 s->flight.push_back(j);
+                s->peak_in_flight = std::max<int64_t>(s->peak_in_flight, (int64_t)s->flight.size());
             }
             if (s->flight.empty() && done.empty()) {
-                if (s->stop && s->queue.empty()) return;
+                if (s->stop && s->queue.empty()) {
+                    s->loop_exited = true;
+                    s->cv_done.notify_all();
+                    return;
+                }
                 s->cv_work.wait(lk, [&] { return s->stop || !s->queue.empty(); });
                 continue;
             }
@@ -607,9 +640,16 @@ void serve_loop(swh_server* s) {
         if (!rc) rc = tick(s);
         std::lock_guard<std::mutex> lk(s->mu);
         if (rc) {
-            s->loop_status = rc;  // a runtime failure ends every job in flight
+            // a runtime failure ends every job in flight and every queued one: each is
+            // published with the failure, so no poll waits for a job the loop will not run
+            s->loop_status = rc;
             for (SJob* j : s->flight)
                 if (j->phase != kDone) finish(s, *j, rc);
+            for (SJob* j : s->queue) {
+                finish(s, *j, rc);
+                done.push_back(j);
+            }
+            s->queue.clear();
         }
         std::vector<SJob*> keep;
         for (SJob* j : s->flight) (j->phase == kDone ? done : keep).push_back(j);
@@ -629,7 +669,8 @@ void serve_loop(swh_server* s) {
             }
             s->t_last = j->t_done;
         }
-        if (!done.empty()) s->cv_done.notify_all();
+        if (rc) s->loop_exited = true;
+        if (!done.empty() || rc) s->cv_done.notify_all();
         if (rc) return;
     }
 }
@@ -658,6 +699,9 @@ int swh_server_create(int device, const swh_scoring* sc, const swh_opts* opts, i
     s->sc = *sc;
     if (opts) s->opts = *opts;
     s->max_in_flight = max_in_flight > 0 ? max_in_flight : 1024;
+    // test hook: SWALIGN_SERVE_FAIL_TICK=t makes tick t report a runtime failure (no device
+    // work is affected), to exercise how a failure ends queued and running jobs
+    if (const char* ft = getenv("SWALIGN_SERVE_FAIL_TICK")) s->fail_tick = atoll(ft);
     // a tick's launches hold the tiles of every job in flight: four waves per SIMD hide the
     // waits of one (100-job stream, all submitted at once: 815 / 946 / 968 GCUPS at 1 / 2 / 4)
     s->wps = usable_wps(c, s->opts.waves_per_simd > 0 ? s->opts.waves_per_simd : (c->wps_fixed ? c->wps_fixed : 4));
@@ -746,8 +790,16 @@ int swh_server_poll(swh_server* s, int64_t job_id, int32_t wait, swh_result* out
     if (it == s->jobs.end()) return SWH_ERR_NOT_FOUND;
     SJob* j = it->second;
     if (!j->published) {
-        if (!wait) return SWH_PENDING;
-        s->cv_done.wait(lk, [&] { return j->published; });
+        if (!wait && !s->loop_exited) return SWH_PENDING;
+        // a job is published by the loop; once the loop has exited, an unpublished job
+        // (submitted as it stopped) never will be, and reports the loop's status
+        s->cv_done.wait(lk, [&] { return j->published || s->loop_exited; });
+        if (!j->published) {
+            const int st = s->loop_status ? s->loop_status : SWH_ERR_INTERNAL;
+            s->jobs.erase(it);
+            delete j;
+            return st;
+        }
     }
     *out = j->res;
     const int st = j->status;
@@ -774,6 +826,10 @@ int swh_server_stats(swh_server* s, swh_serve_stats* st) {
     st->hbm_pool_bytes = s->pool.size;
     st->hbm_in_use_bytes = s->pool.in_use();
     st->loop_status = s->loop_status;
+    st->peak_in_flight = s->peak_in_flight;
+    st->admitted_mid_flight = s->admitted_mid_flight;
+    st->sweep_launches = s->sweep_launches;
+    st->shared_launches = s->shared_launches;
     return SWH_OK;
 }
 

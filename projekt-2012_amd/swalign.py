@@ -328,7 +328,8 @@ class ServeStats(C.Structure):
                 ("in_flight", C.c_int64), ("queued", C.c_int64), ("cells_completed", C.c_int64),
                 ("gcups", C.c_double), ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("max_ms", C.c_double),
                 ("hbm_pool_bytes", C.c_int64), ("hbm_in_use_bytes", C.c_int64), ("loop_status", C.c_int32),
-                ("_pad", C.c_int32)]
+                ("_pad", C.c_int32), ("peak_in_flight", C.c_int64), ("admitted_mid_flight", C.c_int64),
+                ("sweep_launches", C.c_int64), ("shared_launches", C.c_int64)]
 
 
 SWH_PENDING = 9

@@ -5,11 +5,19 @@
 * kats.json: the expected "Alignment score: S at [r, c]" lines of Tests/test1..5.sh.
 * config0.json / small_pairs.json: oracle outputs (score, end, start, CIGAR) for the
   10 kbp x 10 kbp synthetic pair and for seeded random small pairs.
+* config1.json: the oracle's alignment of the 1 Mbp x 1 Mbp synthetic pair (score, end,
+  start, CIGAR length and SHA-256, crosspoints at checkpoint rows every 8192 rows).
+* batch_sample.json: the oracle's alignments of 500 pairs of the 65,536-pair mixed-length
+  batch, picked at evenly spaced ranks of their cell counts (checkpoints every 1024 rows).
+* serve_jobs.json: the oracle's alignments of the first 40 jobs of the served stream.
 
 Run from the repository root with the oracle built (make -C oracle) and the library
-built (for the synthetic generator):  python tests/golden/make_golden.py
+built (for the synthetic generator):
+    python tests/golden/make_golden.py [small] [config1] [batch] [serve]
+(no argument: all).  The large fixtures take a few minutes on 8 host threads.
 """
 import gzip
+import hashlib
 import json
 import os
 import random
@@ -47,7 +55,74 @@ def copy_fasta():
                 shutil.copy(p, os.path.join(out, f))
 
 
+THREADS = min(16, os.cpu_count() or 4)
+
+
+def digest(r):
+    """A fixture record of an oracle alignment: the CIGAR by length and SHA-256."""
+    c = r["cigar"] or ""
+    return dict(score=r["score"], end=list(r["end"]), start=list(r["start"]), n_ops=r["n_ops"],
+                cigar_len=len(c), cigar_sha256=hashlib.sha256(c.encode()).hexdigest())
+
+
+def config1():
+    (a, b), _ = S.workload_pair(1)
+    A, B = (a, b) if len(a) >= len(b) else (b, a)
+    r = O.align_linear(A, B, O.scoring(1, -3, -5, -2), 8192, 8192, threads=THREADS)
+    rec = dict(seed=2013, scoring=[1, -3, -5, -2], m=len(A), n=len(B), K=8192, Kc=8192, **digest(r),
+               xpoints=[list(x) for x in r["xpoints"]])
+    json.dump(rec, open(os.path.join(HERE, "config1.json"), "w"))
+    print("config1:", rec["score"], rec["end"], rec["start"], rec["n_ops"])
+
+
+def batch_sample(count=500):
+    """Pairs of the batched workload at evenly spaced ranks of their cell counts."""
+    from concurrent.futures import ThreadPoolExecutor
+    pairs, _ = S.workload_batch()
+    order = sorted(range(len(pairs)), key=lambda k: (len(pairs[k][0]) * len(pairs[k][1]), k))
+    picks = [order[(q * (len(order) - 1)) // (count - 1)] for q in range(count)]
+    sc = O.scoring(1, -3, -5, -2)
+
+    def one(k):
+        a, b = pairs[k]
+        A, B = (a, b) if len(a) >= len(b) else (b, a)
+        r = O.align_linear(A, B, sc, 1024, 1024)
+        return dict(index=k, m=len(A), n=len(B), **digest(r), xpoints=[list(x) for x in r["xpoints"]])
+
+    with ThreadPoolExecutor(THREADS) as ex:  # ctypes releases the GIL during each alignment
+        recs = list(ex.map(one, picks))
+    json.dump(dict(npairs=len(pairs), seed=2015, scoring=[1, -3, -5, -2], K=1024, pairs=recs),
+              open(os.path.join(HERE, "batch_sample.json"), "w"))
+    print("batch sample: %d pairs" % len(recs))
+
+
+def serve_jobs(count=40):
+    jobs, _ = S.workload_serve(count)
+    sc = O.scoring(1, -3, -5, -2)
+    recs = []
+    for k, (a, b) in enumerate(jobs):
+        A, B = (a, b) if len(a) >= len(b) else (b, a)
+        K = 8192 if len(A) >= 262144 else 1024
+        r = O.align_linear(A, B, sc, K, K, threads=THREADS)
+        recs.append(dict(index=k, m=len(A), n=len(B), **digest(r)))
+    json.dump(dict(njobs=count, seed=2016, scoring=[1, -3, -5, -2], jobs=recs),
+              open(os.path.join(HERE, "serve_jobs.json"), "w"))
+    print("serve jobs: %d" % len(recs))
+
+
 def main():
+    what = set(sys.argv[1:]) or {"small", "config1", "batch", "serve"}
+    if "small" in what:
+        small()
+    if "config1" in what:
+        config1()
+    if "batch" in what:
+        batch_sample()
+    if "serve" in what:
+        serve_jobs()
+
+
+def small():
     copy_fasta()
     kats = [dict(k, a=os.path.basename(k["a"]), b=os.path.basename(k["b"])) for k in KATS]
     json.dump(kats, open(os.path.join(HERE, "kats.json"), "w"), indent=1)

@@ -153,6 +153,30 @@ def test_multithreaded_forward_equals_single(orc):
         assert part[:2] == full[:2] and part[2] == -1
 
 
+def test_multithreaded_linear_equals_single(orc):
+    """The linear-space method with its stage 1 on several host threads (the golden
+    generator's mode for the 1 Mbp pair and the served jobs) gives the single-thread
+    result, crosspoints included, at spacings whose checkpoints fall on tile edges, at
+    spacings that fall back to one thread, and on pairs below the threading cut-off; the
+    full-matrix alignment agrees where it fits."""
+    rng = random.Random(23)
+    a, b = rand_seq(rng, 3300), rand_seq(rng, 2900)
+    b = b[:400] + a[700:2600] + b[400:]
+    sc = orc.scoring(1, -3, -5, -2)
+    for x, y, K, Kc in [(a, b, 512, 128), (a, b, 256, 4096), (a, b, 768, 2048), (a, b, 300, 100),
+                        (a[:900], b[:800], 256, 64)]:
+        A, B = orient(x, y)
+        one = orc.align_linear(A, B, sc, K, Kc)
+        mt = orc.align_linear(A, B, sc, K, Kc, threads=6)
+        for k in ("score", "end", "start", "cigar", "xpoints"):
+            assert mt[k] == one[k], (k, K, Kc)
+    A, B = orient(a[:1500], b[:1400])
+    full = orc.align_full(A, B, sc)
+    mt = orc.align_linear(A, B, sc, 256, 128, threads=4)
+    for k in ("score", "end", "start", "cigar"):
+        assert mt[k] == full[k], k
+
+
 @pytest.mark.parametrize("name", ["test1", "test2"])
 def test_known_answers(orc, name):
     k = [x for x in json.load(open(os.path.join(GOLDEN, "kats.json"))) if x["name"] == name][0]
