@@ -310,15 +310,19 @@ int choose_plan(int64_t m, int64_t n, const swh_opts* o, int64_t budget, Plan* p
     return SWH_OK;
 }
 
-int64_t default_budget(const swh_opts* o) {
+// Checkpoint budget when the options give none: the checkpoint buffers the context already
+// holds (a later call reuses them) plus half of the memory still free -- HBM, or host memory
+// for host-resident checkpoints.
+int64_t default_budget(const swh_ctx* c, const swh_opts* o) {
     if (o && o->hbm_budget_bytes > 0) return o->hbm_budget_bytes;
-    if (o && o->host_checkpoints) {  // half the available host memory
+    if (o && o->host_checkpoints) {
         const long pages = sysconf(_SC_AVPHYS_PAGES), psz = sysconf(_SC_PAGESIZE);
-        return pages > 0 && psz > 0 ? (int64_t)pages * psz / 2 : (INT64_C(8) << 30);
+        const int64_t held = (int64_t)(c->hsrow.cap + c->hscol.cap);
+        return held + (pages > 0 && psz > 0 ? (int64_t)pages * psz / 2 : (INT64_C(8) << 30));
     }
     size_t freeb = 0, totalb = 0;
     (void)hipMemGetInfo(&freeb, &totalb);
-    return (int64_t)(freeb / 2);
+    return (int64_t)(c->srow.cap + c->scol.cap) + (int64_t)(freeb / 2);
 }
 
 // One pair of an alignment batch (a single alignment is a batch of one).
@@ -981,7 +985,7 @@ int swh_align_pair_device(swh_ctx* c, const uint8_t* d1, int64_t len1, const uin
     if (!scoring_ok(sc)) return SWH_ERR_SCORING;
     HCK(c, hipSetDevice(c->device));
     std::vector<Job> jobs{make_job(d1, len1, d2, len2, !(o && o->score_only))};
-    int rc = choose_plan(jobs[0].m, jobs[0].n, o, default_budget(o), &jobs[0].pl);
+    int rc = choose_plan(jobs[0].m, jobs[0].n, o, default_budget(c, o), &jobs[0].pl);
     if (rc) return rc;
     rc = run_jobs(c, jobs, sc, engine_opts(o, false), out);
     return rc ? rc : jobs[0].status;
@@ -1058,7 +1062,7 @@ int tune_jobs(swh_ctx* c, const std::vector<Job>& jobs, const swh_scoring* sc, c
 // opts->autotune the settings are first chosen by timing candidates on a sample.
 int batch_run(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const swh_opts* o, swh_result* out,
               int32_t* statuses) {
-    const int64_t budget = default_budget(o);
+    const int64_t budget = default_budget(c, o);
     swh_opts ob = o ? *o : swh_opts{};
     if (ob.autotune) {
         swh_batch_tuning t{};
@@ -1124,7 +1128,7 @@ int tune_jobs(swh_ctx* c, const std::vector<Job>& jobs, const swh_scoring* sc, c
         if (rc) return rc;
     }
     int cand = 0;
-    const int64_t budget = default_budget(o);
+    const int64_t budget = default_budget(c, o);
     for (int64_t K : Ks) {
         // a spacing whose checkpoints for the whole batch exceed the budget is not a candidate
         int64_t need = 0;
