@@ -90,3 +90,25 @@ def test_dump_rows_reference_format(tmp_path, orc):
     for q, name in enumerate(files):
         got = np.fromfile(str(tmp_path / name), dtype=np.int32).reshape(-1, 2)
         assert np.array_equal(got, np.asarray(srow[q]).reshape(-1, 2)), name
+
+
+@pytest.mark.gpu
+def test_visual_and_gap_lists_known_answer_2(tmp_path, orc):
+    """SWalign ... --visual --gaps P on the test2 inputs: P.first.gaps / P.second.gaps hold the
+    gap lists of the oracle's alignment and the printed 50-column blocks are those the
+    reference's renderer (src/Visualisation.cpp:62-127, restated in test_cli_units) draws
+    from them."""
+    import struct
+    from conftest import fasta_bytes, orient
+    from test_cli_units import gap_lists, reference_visual
+    pre = str(tmp_path / "kat2")
+    p = run(os.path.join(FA, "NC_005046.fna"), os.path.join(FA, "NC_012557.fna"), "5", "-5", "-5", "-5",
+            "--visual", "--gaps", pre)
+    assert p.returncode == 0, p.stdout + p.stderr
+    A, B = orient(fasta_bytes("NC_005046.fna"), fasta_bytes("NC_012557.fna"))
+    ref = orc.align_full(A, B, orc.scoring(5, -5, -5, -5))
+    ga, gb = gap_lists(ref["cigar"], ref["start"])
+    for path, want in ((pre + ".first.gaps", ga), (pre + ".second.gaps", gb)):
+        assert list(struct.iter_unpack("<ii", open(path, "rb").read())) == want
+    assert p.stdout.endswith(reference_visual(A, B, ga, gb, ref["start"], ref["end"]))
+    assert "\t>CIGAR: %s\n" % ref["cigar"] in p.stdout

@@ -270,3 +270,30 @@ def test_chunk_variants_by_tile_width(sw, orc, ctx, W, Kc, precision):
     ref = orc.align_linear(A, B, orc.scoring(1, -3, -5, -2), 256, Kc)
     _cmp(got, ref)
     assert got["xpoints"] == [tuple(x) for x in ref["xpoints"]]
+
+
+def test_device_inputs_end_at_last_column(sw, orc, ctx):
+    """Sequences in exact-size device buffers with the alignment ending at the last column of
+    B (and, swapped, at the last row): the single-pair and batch device entry points give the
+    oracle's alignment (the traceback stages no byte beyond the current column)."""
+    rng = random.Random(99)
+    a = bytes(rng.choice(b"ACGT") for _ in range(4000))
+    core = bytearray(a[1500:2700])
+    for i in range(len(core) - 60):
+        if rng.random() < 0.08:
+            core[i] = rng.choice(b"ACGT")
+    b = bytes(rng.choice(b"ACGT") for _ in range(300)) + bytes(core)  # ends with A[2640:2700] exactly
+    sc = sw.scoring(1, -3, -5, -2)
+    ref = orc.align_full(*orient(a, b), orc.scoring(1, -3, -5, -2))
+    assert ref["end"][1] == len(b) - 1
+    for x, y in ((a, b), (b, a)):
+        dx, dy = sw.DeviceBuffer(x), sw.DeviceBuffer(y)
+        try:
+            got = ctx.align_device(dx.address, len(x), dy.address, len(y), sc, K=512, Kc=256)
+            _cmp(got, ref)
+            res, st = ctx.batch_device([(dx.address, len(x), dy.address, len(y))], sc, K=512, Kc=256)
+            assert st == [0]
+            _cmp(res[0], ref)
+        finally:
+            dx.free()
+            dy.free()
