@@ -44,7 +44,9 @@ enum {
     SWH_ERR_BUDGET = 6,          /* request rejected: exceeds the HBM admission budget   */
     SWH_ERR_INTERNAL = 7,        /* traceback inconsistency (should never happen)        */
     SWH_ERR_NOT_FOUND = 8,       /* unknown job id / nothing to poll                     */
-    SWH_PENDING = 9              /* swh_server_poll: the job has not finished (no error)  */
+    SWH_PENDING = 9,             /* swh_server_poll: the job has not finished (no error)  */
+    SWH_ERR_SCRATCH = 10         /* the pair's working set exceeds the fixed scratch memory of */
+                                 /* swh_align_pair_device_noalloc                             */
 };
 
 /* Mirrors `scoring` (src/SWutils.h:15-20) field for field. */
@@ -139,6 +141,26 @@ int swh_align_pair_device(swh_ctx* ctx, const uint8_t* d_seq1, int64_t len1, con
                           swh_result* out);
 
 void swh_result_free(swh_result* r);
+
+/* Allocation-free variant of swh_align_pair_device.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Every device buffer beyond the inputs
+ * is a slice of a static device array of SWH_NOALLOC_SCRATCH_BYTES held by each device the
+ * library runs on (its address is resolved once per device by swh_ctx_create); the call
+ * makes no hipMalloc, hipHostMalloc or hipFree.  Limit: the pair's working set (row maxima,
+ * checkpoints, traceback blocks) must fit that array -- about 100 kbp x 100 kbp at the
+ * default checkpoint spacing, which is doubled from swh_align_pair_device's default until it
+ * fits; larger pairs, or a given spacing that does not fit, return SWH_ERR_SCRATCH before
+ * any work.  Options want_xpoints and host_checkpoints are rejected (SWH_ERR_INVALID_ARG).
+ * The CIGAR is written NUL-terminated to `cigar`, which must hold at least
+ * 3 * (len1 + len2) + 64 bytes (its tail stages the traceback); out->cigar stays NULL and
+ * out->cigar_len is the CIGAR's length (swh_result_free on `out` is harmless).  For the same
+ * checkpoint spacing the result equals swh_align_pair_device's.  All calls on one device
+ * share its array: calls that overlap on one device race and must be serialised. */
+#define SWH_NOALLOC_SCRATCH_BYTES (64 * 1024 * 1024)
+int swh_align_pair_device_noalloc(swh_ctx* ctx, const uint8_t* d_seq1, int64_t len1, const uint8_t* d_seq2,
+                                  int64_t len2, const swh_scoring* sc, const swh_opts* opts, swh_result* out,
+                                  char* cigar, int64_t cigar_cap);
 
 /* ---- stage-level entry points (tests and tools) ------------------------------------ */
 

@@ -37,6 +37,7 @@ int map_hip(swh_ctx* c, hipError_t e) {
 
 hipError_t ensure(DevBuf& b, size_t bytes) {
     if (bytes == 0) bytes = 64;
+    if (b.fixed) return b.cap >= bytes ? hipSuccess : hipErrorOutOfMemory;  // a slice: never allocates
     if (b.p && b.cap >= bytes) return hipSuccess;
     if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
@@ -325,31 +326,6 @@ int64_t default_budget(const swh_ctx* c, const swh_opts* o) {
     return (int64_t)(c->srow.cap + c->scol.cap) + (int64_t)(freeb / 2);
 }
 
-// One pair of an alignment batch (a single alignment is a batch of one).
-struct Job {
-    const uint8_t* A = nullptr;  // device, oriented: A = rows (the longer), B = columns
-    const uint8_t* B = nullptr;
-    int64_t m = 0, n = 0;
-    bool swapped = false;
-    bool traceback = true;
-    Plan pl{};
-    int64_t rowmax_off = 0, srow_off = 0, scol_off = 0, ops_off = 0, xps_off = 0, xcap = 0;
-    int32_t best = 0;
-    int64_t bi = -1, bj = -1;
-    WalkState st{};
-    int status = SWH_OK;
-};
-
-struct EngineOpts {
-    int64_t tile_cols = 0;   // 0 = auto
-    int precision = 0;       // 0 = packed 16-bit stage 1 where it cannot overflow, 32 = int32 only
-    int wps = 0;             // sweep waves per SIMD, 0 = auto
-    int lookahead = 7;       // traceback blocks ahead per direction (0 = none)
-    bool want_xp = false;
-    bool batch = false;      // many pairs: fixed tile width, two waves per SIMD
-    bool host_ckpt = false;  // checkpoints in mapped pinned host memory
-};
-
 // Columns per tile of a batch sweep: wide enough that the 63 fill steps of a tile cost
 // about a tenth, narrow enough that short pairs still cut into several tiles.
 constexpr int64_t kBatchTileCols = 512;
@@ -637,7 +613,9 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
     };
     // direction codes per round: a quarter of the free memory, between 2 and 32 GiB
     int64_t dirs_cap = kTracebackDirsCap;
-    {
+    if (c->dirs_cap_fixed > 0) {
+        dirs_cap = c->dirs_cap_fixed;  // the codes' share of a fixed scratch area
+    } else {
         size_t freeb = 0, totalb = 0;
         (void)hipMemGetInfo(&freeb, &totalb);
         dirs_cap = std::max<int64_t>(kTracebackDirsCap, std::min<int64_t>((int64_t)(freeb / 4), INT64_C(32) << 30));
@@ -772,16 +750,21 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
     clk.mark("traceback", guard);
 
     // ---- results: ops (reversed) -> run-length CIGAR, crosspoints start -> end ---------
-    if ((size_t)ops_elems > c->host_ops_cap) {  // pinned, grow-only: the copy runs at full link speed
-        if (c->host_ops) (void)hipHostFree(c->host_ops);
-        c->host_ops = nullptr;
-        c->host_ops_cap = 0;
-        HCK(c, hipHostMalloc((void**)&c->host_ops, (size_t)ops_elems, 0));
-        c->host_ops_cap = (size_t)ops_elems;
+    uint8_t* ops_h = c->ops_dst;  // the caller's memory in the allocation-free entry point
+    if (ops_h) {
+        if ((size_t)ops_elems > c->ops_dst_cap) return SWH_ERR_INTERNAL;
+    } else {
+        if ((size_t)ops_elems > c->host_ops_cap) {  // pinned, grow-only: the copy runs at full link speed
+            if (c->host_ops) (void)hipHostFree(c->host_ops);
+            c->host_ops = nullptr;
+            c->host_ops_cap = 0;
+            HCK(c, hipHostMalloc((void**)&c->host_ops, (size_t)ops_elems, 0));
+            c->host_ops_cap = (size_t)ops_elems;
+        }
+        ops_h = c->host_ops;
     }
-    const uint8_t* ops_h = c->host_ops;
-    std::vector<XPoint> xps_h((size_t)xps_elems);
-    HCK(c, hipMemcpyAsync(c->host_ops, c->ops.p, (size_t)ops_elems, hipMemcpyDeviceToHost, c->stream));
+    std::vector<XPoint> xps_h(eo.want_xp ? (size_t)xps_elems : 0);
+    HCK(c, hipMemcpyAsync(ops_h, c->ops.p, (size_t)ops_elems, hipMemcpyDeviceToHost, c->stream));
     if (eo.want_xp)
         HCK(c, hipMemcpyAsync(xps_h.data(), c->xps.p, sizeof(XPoint) * xps_elems, hipMemcpyDeviceToHost, c->stream));
     HCK(c, hipStreamSynchronize(c->stream));
@@ -797,8 +780,8 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
             if (!j.traceback || j.status != SWH_OK) continue;
             r.ms_traceback = ms_tb;
             r.ms_total = r.ms_forward + r.ms_end_search + ms_tb;
-            if (fill_alignment(r, states[(size_t)k], ops_h + j.ops_off, xps_h.data() + j.xps_off, j.xcap,
-                               j.best, eo.want_xp) != SWH_OK) {
+            if (fill_alignment(r, states[(size_t)k], ops_h + j.ops_off, eo.want_xp ? xps_h.data() + j.xps_off : nullptr,
+                               j.xcap, j.best, eo.want_xp, c->cigar_dst, c->cigar_dst_cap) != SWH_OK) {
                 fail[(size_t)tid] = 1;
                 return;
             }
@@ -814,7 +797,7 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
         for (auto& t : pool) t.join();
     }
     for (int f : fail)
-        if (f) return SWH_ERR_OUT_OF_MEMORY;
+        if (f) return c->cigar_dst ? SWH_ERR_INVALID_ARG : SWH_ERR_OUT_OF_MEMORY;
     clk.mark("cigars", (int64_t)scored.size());
     return SWH_OK;
 }
@@ -822,11 +805,12 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
 void result_init(swh_result* r) { memset(r, 0, sizeof *r); }
 
 int fill_alignment(swh_result& r, const WalkState& st, const uint8_t* ops, const XPoint* xs, int64_t xcap,
-                   int32_t best, bool want_xp) {
+                   int32_t best, bool want_xp, char* dst, int64_t dst_cap) {
     r.start_row = st.start_i;
     r.start_col = st.start_j;
     // a run of n ops takes at most (digits of n) + 1 <= 2n characters
-    char* out = (char*)malloc((size_t)(2 * st.nops + 1));
+    if (dst && dst_cap < 2 * st.nops + 1) return SWH_ERR_INVALID_ARG;
+    char* out = dst ? dst : (char*)malloc((size_t)(2 * st.nops + 1));
     if (!out) return SWH_ERR_OUT_OF_MEMORY;
     size_t len = 0;
     for (int64_t b = st.nops - 1; b >= 0;) {  // ops were written end -> start
@@ -841,9 +825,10 @@ int fill_alignment(swh_result& r, const WalkState& st, const uint8_t* ops, const
     }
     out[len] = 0;
     r.n_ops = st.nops;
+    r.cigar_len = (int64_t)len;
+    if (dst) return SWH_OK;  // the caller's buffer: r.cigar stays NULL, nothing to free
     char* shrunk = (char*)realloc(out, len + 1);
     r.cigar = shrunk ? shrunk : out;
-    r.cigar_len = (int64_t)len;
     if (want_xp) {
         const int64_t nx = std::min<int64_t>(st.nxp, xcap);
         r.xpoints = (swh_xpoint*)malloc(sizeof(swh_xpoint) * (size_t)(nx + 1));
@@ -900,6 +885,9 @@ const char* swh_strerror(int status) {
         case SWH_ERR_BUDGET: return "Request exceeds the HBM budget.";
         case SWH_ERR_INTERNAL: return "Internal traceback inconsistency.";
         case SWH_ERR_NOT_FOUND: return "Not found.";
+        case SWH_PENDING: return "Pending.";
+        case SWH_ERR_SCRATCH:
+            return "The pair does not fit the fixed scratch memory of the allocation-free entry point.";
         default: return "Unknown status.";
     }
 }
@@ -942,6 +930,9 @@ int swh_ctx_create(int device, swh_ctx** out) {
         (void)hipGetLastError();
         for (size_t& pad : c->lds_pad) pad = 0;
     }
+    // the allocation-free entry point's scratch array on this device (resolved once per
+    // device); a failure here only disables that entry point
+    c->scratch_status = scratch_resolve(device, &c->scratch);
     *out = c;
     return SWH_OK;
 }

@@ -10,10 +10,12 @@
 #include "../../include/swalign_hip.h"
 #include "sw_internal.h"
 
-// A grow-only device allocation.
+// A grow-only device allocation, or (fixed) a slice of memory the context does not own:
+// ensure() then only checks the capacity and never allocates.
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
+    bool fixed = false;
 };
 
 // A grow-only pinned host allocation mapped into the device's address space.
@@ -44,6 +46,16 @@ struct swh_ctx {
     bool codes_off = false;         // symbol-code scores disabled by the environment
     bool trace = false;             // SWALIGN_TRACE=1: host wall time of each stage to stderr
     hipError_t last = hipSuccess;
+    // The allocation-free entry point runs on a copy of the context whose DevBufs are fixed
+    // slices of the device's static scratch array; these then replace the allocations of
+    // the traceback's host buffers (nullptr: the grow-only pinned buffers above).
+    void* scratch = nullptr;        // this device's address of the scratch array
+    int scratch_status = 0;         // SWH_OK, or why it could not be resolved
+    uint8_t* ops_dst = nullptr;     // host destination of the traceback ops
+    size_t ops_dst_cap = 0;
+    char* cigar_dst = nullptr;      // caller's CIGAR buffer (the result's cigar stays NULL)
+    int64_t cigar_dst_cap = 0;
+    int64_t dirs_cap_fixed = 0;     // direction-code bytes per traceback round, 0 = from free HBM
 };
 
 namespace swh_host {
@@ -111,8 +123,45 @@ struct Plan {
 int64_t plan_mem(int64_t m, int64_t n, int64_t K, int64_t Kc);
 int choose_plan(int64_t m, int64_t n, const swh_opts* o, int64_t budget, Plan* pl);
 
-// Fills r's start, CIGAR (run-length, from ops written end -> start) and crosspoints.
+// Fills r's start, CIGAR (run-length, from ops written end -> start) and crosspoints.  With
+// dst the CIGAR is written there (dst_cap bytes, NUL included) and r.cigar stays NULL.
 int fill_alignment(swh_result& r, const swk::WalkState& st, const uint8_t* ops, const swk::XPoint* xps,
-                   int64_t xcap, int32_t best, bool want_xp);
+                   int64_t xcap, int32_t best, bool want_xp, char* dst = nullptr, int64_t dst_cap = 0);
+
+// The device's static scratch array of the allocation-free entry point (sw_noalloc.hip):
+// resolves its address on `device` (which must be current) once per device and process.
+int scratch_resolve(int device, void** out);
+
+// The stage driver (sw_api.hip), shared with the allocation-free entry point.
+// One pair of an alignment batch (a single alignment is a batch of one).
+struct Job {
+    const uint8_t* A = nullptr;  // device, oriented: A = rows (the longer), B = columns
+    const uint8_t* B = nullptr;
+    int64_t m = 0, n = 0;
+    bool swapped = false;
+    bool traceback = true;
+    Plan pl{};
+    int64_t rowmax_off = 0, srow_off = 0, scol_off = 0, ops_off = 0, xps_off = 0, xcap = 0;
+    int32_t best = 0;
+    int64_t bi = -1, bj = -1;
+    swk::WalkState st{};
+    int status = SWH_OK;
+};
+
+struct EngineOpts {
+    int64_t tile_cols = 0;   // 0 = auto
+    int precision = 0;       // 0 = packed 16-bit stage 1 where it cannot overflow, 32 = int32 only
+    int wps = 0;             // sweep waves per SIMD, 0 = auto
+    int lookahead = 7;       // traceback blocks ahead per direction (0 = none)
+    bool want_xp = false;
+    bool batch = false;      // many pairs: fixed tile width, two waves per SIMD
+    bool host_ckpt = false;  // checkpoints in mapped pinned host memory
+};
+
+// Stage 1, end search and traceback of every job, all jobs side by side in each sweep.
+int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const EngineOpts& eo, swh_result* out);
+EngineOpts engine_opts(const swh_opts* o, bool batch);
+// A job for device-resident sequences, oriented as the reference does (longer = rows).
+Job make_job(const uint8_t* d1, int64_t len1, const uint8_t* d2, int64_t len2, bool traceback);
 
 }  // namespace swh_host

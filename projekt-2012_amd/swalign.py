@@ -20,6 +20,8 @@ SWH_ERR_OUT_OF_MEMORY = 5
 SWH_ERR_BUDGET = 6
 SWH_ERR_INTERNAL = 7
 SWH_ERR_NOT_FOUND = 8
+SWH_ERR_SCRATCH = 10
+NOALLOC_SCRATCH_BYTES = 64 * 1024 * 1024
 
 
 class Scoring(C.Structure):
@@ -69,6 +71,9 @@ EXPORTS = [
     ("swh_align_pair_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
                                         C.POINTER(Scoring), C.POINTER(Opts), C.POINTER(Result)]),
     ("swh_result_free", None, [C.POINTER(Result)]),
+    ("swh_align_pair_device_noalloc", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
+                                                C.POINTER(Scoring), C.POINTER(Opts), C.POINTER(Result),
+                                                C.c_char_p, C.c_int64]),
     ("swh_forward", C.c_int, [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int64,
                               C.POINTER(Scoring), C.c_int64, C.c_int64, C.POINTER(C.c_int32),
                               C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.c_void_p, C.c_void_p,
@@ -214,6 +219,25 @@ class Context:
             if rc != SWH_OK:
                 raise SWError(rc, "swh_align_pair_device")
             return _result_dict(r)
+        finally:
+            lib().swh_result_free(C.byref(r))
+
+    def align_device_noalloc(self, ptr1, len1, ptr2, len2, sc, **kw):
+        """swh_align_pair_device_noalloc: the library allocates no device memory; the CIGAR
+        comes back through a host buffer this binding passes in."""
+        r = Result()
+        o = make_opts(**kw)
+        cap = 3 * (len1 + len2) + 64
+        buf = C.create_string_buffer(cap)
+        rc = lib().swh_align_pair_device_noalloc(self._h, C.c_void_p(ptr1), len1, C.c_void_p(ptr2), len2,
+                                                 C.byref(sc), C.byref(o), C.byref(r), buf, cap)
+        try:
+            if rc != SWH_OK:
+                raise SWError(rc, "swh_align_pair_device_noalloc")
+            d = _result_dict(r)
+            if not kw.get("score_only"):
+                d["cigar"] = buf.value[: r.cigar_len].decode() if r.score > 0 else None
+            return d
         finally:
             lib().swh_result_free(C.byref(r))
 

@@ -36,8 +36,8 @@ def test_library_exports_every_declared_symbol(sw):
 
 def test_status_strings(sw):
     L = sw.lib()
-    for code in range(0, 9):
-        assert L.swh_strerror(code)
+    for code in range(0, 11):
+        assert L.swh_strerror(code) and L.swh_strerror(code) != b"Unknown status."
     assert b"scoring" in L.swh_strerror(sw.SWH_ERR_SCORING)
 
 
@@ -48,6 +48,8 @@ def test_invalid_arguments_return_status(sw):
     o = sw.make_opts()
     assert L.swh_align_pair(None, b"A", 1, b"A", 1, C.byref(sc), C.byref(o), C.byref(r)) == sw.SWH_ERR_INVALID_ARG
     assert L.swh_ctx_create(0, None) == sw.SWH_ERR_INVALID_ARG
+    assert L.swh_align_pair_device_noalloc(None, None, 0, None, 0, C.byref(sc), C.byref(o), C.byref(r),
+                                           None, 0) == sw.SWH_ERR_INVALID_ARG
 
 
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a HIP device is present")
