@@ -311,19 +311,24 @@ int choose_plan(int64_t m, int64_t n, const swh_opts* o, int64_t budget, Plan* p
     return SWH_OK;
 }
 
-// Checkpoint budget when the options give none: the checkpoint buffers the context already
-// holds (a later call reuses them) plus half of the memory still free -- HBM, or host memory
-// for host-resident checkpoints.
+// Checkpoint budget when the options give none: half of the memory available to the call --
+// what is still free plus the checkpoint buffers the context already holds (a later call
+// reuses them) -- and never less than those buffers, so that repeated calls keep one plan.
+// HBM, or host memory for host-resident checkpoints.
 int64_t default_budget(const swh_ctx* c, const swh_opts* o) {
     if (o && o->hbm_budget_bytes > 0) return o->hbm_budget_bytes;
+    int64_t held, freeb;
     if (o && o->host_checkpoints) {
         const long pages = sysconf(_SC_AVPHYS_PAGES), psz = sysconf(_SC_PAGESIZE);
-        const int64_t held = (int64_t)(c->hsrow.cap + c->hscol.cap);
-        return held + (pages > 0 && psz > 0 ? (int64_t)pages * psz / 2 : (INT64_C(8) << 30));
+        held = (int64_t)(c->hsrow.cap + c->hscol.cap);
+        freeb = pages > 0 && psz > 0 ? (int64_t)pages * psz : (INT64_C(16) << 30);
+    } else {
+        size_t f = 0, t = 0;
+        (void)hipMemGetInfo(&f, &t);
+        held = (int64_t)(c->srow.cap + c->scol.cap);
+        freeb = (int64_t)f;
     }
-    size_t freeb = 0, totalb = 0;
-    (void)hipMemGetInfo(&freeb, &totalb);
-    return (int64_t)(c->srow.cap + c->scol.cap) + (int64_t)(freeb / 2);
+    return std::max(held, (freeb + held) / 2);
 }
 
 // Columns per tile of a batch sweep: wide enough that the 63 fill steps of a tile cost
