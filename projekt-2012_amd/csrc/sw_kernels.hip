@@ -76,6 +76,26 @@ struct TileCtx {
     uint16_t* dirs;     // kDirs: [step][lane] codes of this tile
 };
 
+// Per-wave LDS of the sweep: lane 63's bottom-row outputs of the current chunk.  Every lane
+// stores its bottom row each step, with no EXEC mask: lane 63 into out[step], lane l < 63
+// into spare[l + step], which nothing reads (a masked store for lane 63 alone cost three
+// more instructions per step).
+struct SweepLds {
+    int h[64], f[64];               // lane 63's {H+O, F+E} of steps 0..63
+    int spare_h[128], spare_f[128];
+};
+// Where a lane's per-step bottom-row stores of a chunk go (see SweepLds): H and F go to
+// separate arrays, so that the compiler pairs the stores of two steps without first moving
+// the values into adjacent registers.
+struct BottomSlots {
+    int* h;
+    int* f;
+};
+__device__ __forceinline__ BottomSlots bottom_slots(SweepLds& lds) {
+    const int l = wave_lane();
+    return l == 63 ? BottomSlots{lds.h, lds.f} : BottomSlots{lds.spare_h + l, lds.spare_f + l};
+}
+
 // Column symbol carried down the lanes: the byte itself, or with symbol codes the
 // v_perm_b32 selector of its code (byte 0 picks profile byte `code`, bytes 1-3 read 0).
 // Pad columns get a value that matches nothing (0x100 / all-zero selector).
@@ -157,7 +177,7 @@ template <int R, int MODE, bool PROF, bool EDGE, bool DUMP, bool START = false, 
 __device__ __forceinline__ void chunk_steps(const SweepParams& p, const TileCtx& T, int64_t c, int64_t row0,
                                             const RowSyms<R>& a, int (&Ho)[R], int (&E)[R], int (&rmax)[R],
                                             int& Ho_up_prev, int& Ho_bot, int& Fe_bot, int& Qh, int& Qf, int& Qb,
-                                            int& bl, int2* s_out, int& found_col) {
+                                            int& bl, BottomSlots s_out, int& found_col) {
     const int lane = wave_lane();
     const int O = p.sc.O, Ev = p.sc.E;
     const int MO = to_vgpr(p.sc.M - O), XO = to_vgpr(p.sc.X - O);
@@ -227,7 +247,8 @@ __device__ __forceinline__ void chunk_steps(const SweepParams& p, const TileCtx&
         }
         Ho_bot = Ho_up;
         Fe_bot = Fe_up;
-        if (lane == 63) s_out[s] = make_int2(Ho_bot, Fe_bot);
+        s_out.h[s] = Ho_bot;  // every lane stores; only lane 63's slots are read
+        s_out.f[s] = Fe_bot;
         if (MODE == kDirs) dcol[s * 64] = (uint16_t)nib;
         if (DUMP && s == sdump) {
 #pragma unroll
@@ -237,18 +258,13 @@ __device__ __forceinline__ void chunk_steps(const SweepParams& p, const TileCtx&
     }
 }
 
-// Per-wave LDS of the sweep: lane 63's bottom-row outputs of the current chunk.
-struct SweepLds {
-    int2 out[64];
-};
-
 // Sweeps one tile.  left/left_r0: {H,E} of column C0-1 at absolute row r is
 // left[r - left_r0] (nullptr = zero column); corner = H(R0-1, C0-1); right (nullable)
 // receives the tile's last column {H,E} at index row - right_r0.
 template <int R, int MODE, bool PROF>
 __device__ void sweep_tile(const SweepParams& p, const TileCtx& T, const int2* left, int64_t left_r0, int corner,
                            int2* right, int64_t right_r0, bool first_segment, int32_t* found, SweepLds& lds) {
-    int2* s_out = lds.out;
+    const BottomSlots s_slots = bottom_slots(lds);
     const int lane = wave_lane();
     const int O = p.sc.O;
     const int64_t w = T.w;
@@ -292,38 +308,37 @@ __device__ void sweep_tile(const SweepParams& p, const TileCtx& T, const int2* l
         bool dump = false;
         if (MODE == kStage1 && T.scol) {
             const int64_t lo = T.C0 + max(64 * c - 63, (int64_t)0), hi = T.C0 + min(64 * c + 63, w - 1);
-            const int64_t x = ((lo + T.Kc) / T.Kc) * T.Kc - 1;  // smallest x >= lo, x = -1 mod Kc
-            dump = x <= hi;
+            dump = (lo | (T.Kc - 1)) <= hi;  // smallest x >= lo with x = -1 mod Kc (a power of two)
         }
         if (!dump) {
             if (interior)
                 chunk_steps<R, MODE, PROF, false, false>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
-                                                   Qh, Qf, Qb, bl, s_out, found_col);
+                                                   Qh, Qf, Qb, bl, s_slots, found_col);
             else if (c == 0 && w >= 64)
                 chunk_steps<R, MODE, PROF, false, false, true>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot,
-                                                         Fe_bot, Qh, Qf, Qb, bl, s_out, found_col);
+                                                         Fe_bot, Qh, Qf, Qb, bl, s_slots, found_col);
             else if (c >= 1)  // no column below 0 from the second chunk on
                 chunk_steps<R, MODE, PROF, false, false, false, true>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev,
-                                                                Ho_bot, Fe_bot, Qh, Qf, Qb, bl, s_out, found_col);
+                                                                Ho_bot, Fe_bot, Qh, Qf, Qb, bl, s_slots, found_col);
             else
                 chunk_steps<R, MODE, PROF, true, false>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
-                                                  Qh, Qf, Qb, bl, s_out, found_col);
+                                                  Qh, Qf, Qb, bl, s_slots, found_col);
         } else if (interior) {
             chunk_steps<R, MODE, PROF, false, true>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot, Qh, Qf,
-                                              Qb, bl, s_out, found_col);
+                                              Qb, bl, s_slots, found_col);
         } else if (c >= 1) {
             chunk_steps<R, MODE, PROF, false, true, false, true>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot,
-                                                           Fe_bot, Qh, Qf, Qb, bl, s_out, found_col);
+                                                           Fe_bot, Qh, Qf, Qb, bl, s_slots, found_col);
         } else {
             chunk_steps<R, MODE, PROF, true, true>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot, Qh, Qf,
-                                             Qb, bl, s_out, found_col);
+                                             Qb, bl, s_slots, found_col);
         }
         // Bottom row of the tile for columns 64c-63+lane.  The LDS hand-off from lane 63 to
         // the other lanes needs no hardware barrier (one wave, LDS operations in order) but
         // must not be reordered by the compiler.
         __builtin_amdgcn_wave_barrier();
         {
-            const int2 o = s_out[lane];
+            const int2 o = make_int2(lds.h[lane], lds.f[lane]);
             const int64_t col = 64 * c + lane - 63;
             if (col >= 0 && col < w) {
                 const int2 v = make_int2(o.x - O, o.y - p.sc.E);
@@ -492,7 +507,7 @@ __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileC
                                                const int64_t (&row0)[2], const int (&a)[R], const int (&a2)[R],
                                                s16x2 (&Ho)[R],
                                                s16x2 (&E)[R], s16x2 (&rmax)[R], s16x2& Ho_up_prev, s16x2& Ho_bot,
-                                               s16x2& Fe_bot, int& Qh, int& Qf, int& Qb, int& bl, int2* s_out,
+                                               s16x2& Fe_bot, int& Qh, int& Qf, int& Qb, int& bl, BottomSlots s_out,
                                                const s16x2 zero, const int (&base)[2]) {
     const int lane = wave_lane();
     const s16x2 vO = pk2(p.sc.O, p.sc.O), vE = pk2(p.sc.E, p.sc.E), one = pk2(1, 1);
@@ -574,7 +589,8 @@ __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileC
         }
         Ho_bot = Ho_up;
         Fe_bot = Fe_up;
-        if (lane == 63) s_out[s] = make_int2(pk_bits(Ho_bot), pk_bits(Fe_bot));
+        s_out.h[s] = pk_bits(Ho_bot);  // see chunk_steps
+        s_out.f[s] = pk_bits(Fe_bot);
         if (MODE == kDirs) {
             if (dcol[0]) dcol[0][s * 64] = (uint16_t)nib;
             if (dcol[1]) dcol[1][s * 64] = (uint16_t)(nib >> 16);
@@ -597,7 +613,7 @@ __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileC
 // stores nothing).
 template <int R, int MODE, bool PROF>
 __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], SweepLds& lds) {
-    int2* s_out = lds.out;
+    const BottomSlots s_slots = bottom_slots(lds);
     const int lane = wave_lane();
     const int O = p.sc.O, Ev = p.sc.E;
     TileCtx T[2] = {ts[0].T, ts[1].T};
@@ -694,36 +710,35 @@ __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], Sw
         for (int t = 0; t < 2; ++t) {
             if (MODE == kStage1 && T[t].scol && T[t].w > 0) {
                 const int64_t lo = T[t].C0 + max(64 * c - 63, (int64_t)0), hi = T[t].C0 + min(64 * c + 63, T[t].w - 1);
-                const int64_t x = ((lo + T[t].Kc) / T[t].Kc) * T[t].Kc - 1;
-                dump = dump || x <= hi;
+                dump = dump || (lo | (T[t].Kc - 1)) <= hi;  // as in sweep_tile
             }
         }
         if (!dump) {
             if (interior)
                 chunk_steps_pk<R, MODE, PROF, false, false>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
-                                                      Qh, Qf, Qb, bl, s_out, zero, base);
+                                                      Qh, Qf, Qb, bl, s_slots, zero, base);
             else if (c == 0 && min(T[0].w, T[1].w) >= 64)
                 chunk_steps_pk<R, MODE, PROF, false, false, true>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev, Ho_bot,
-                                                            Fe_bot, Qh, Qf, Qb, bl, s_out, zero, base);
+                                                            Fe_bot, Qh, Qf, Qb, bl, s_slots, zero, base);
             else if (c >= 1 && T[0].w == T[1].w)  // no column below 0, one last column for both tiles
                 chunk_steps_pk<R, MODE, PROF, false, false, false, true>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev,
-                                                                   Ho_bot, Fe_bot, Qh, Qf, Qb, bl, s_out, zero, base);
+                                                                   Ho_bot, Fe_bot, Qh, Qf, Qb, bl, s_slots, zero, base);
             else
                 chunk_steps_pk<R, MODE, PROF, true, false>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
-                                                     Qh, Qf, Qb, bl, s_out, zero, base);
+                                                     Qh, Qf, Qb, bl, s_slots, zero, base);
         } else if (interior) {
             chunk_steps_pk<R, MODE, PROF, false, true>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
-                                                 Qh, Qf, Qb, bl, s_out, zero, base);
+                                                 Qh, Qf, Qb, bl, s_slots, zero, base);
         } else if (c >= 1 && T[0].w == T[1].w) {
             chunk_steps_pk<R, MODE, PROF, false, true, false, true>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev,
-                                                              Ho_bot, Fe_bot, Qh, Qf, Qb, bl, s_out, zero, base);
+                                                              Ho_bot, Fe_bot, Qh, Qf, Qb, bl, s_slots, zero, base);
         } else {
             chunk_steps_pk<R, MODE, PROF, true, true>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot, Qh,
-                                                Qf, Qb, bl, s_out, zero, base);
+                                                Qf, Qb, bl, s_slots, zero, base);
         }
         __builtin_amdgcn_wave_barrier();
         {
-            const int2 o = s_out[lane];
+            const int2 o = make_int2(lds.h[lane], lds.f[lane]);
             const int64_t col = 64 * c + lane - 63;
             const s16x2 oh = pk_from(o.x), of = pk_from(o.y);
 #pragma unroll
