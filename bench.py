@@ -206,10 +206,17 @@ def pair_record(torch, sw, workload, steps, warmup, K=None):
             "device": info["name"], "cus": info["cus"], "clock_mhz": round(clock_ghz * 1000),
         },
         "roofline": roofline(cells, fwd_ms, r0["forward_launches"], workload, precision=r0["precision"],
-                             codes=r0["score_codes"], W=-(-n // (4 * int(info["cus"] or 256))),
+                             codes=r0["score_codes"], W=tile_width(m, n, r0["forward_launches"]),
                              K=r0["checkpoint_rows"] or m, Kc=r0["checkpoint_cols"] or n),
     }
     return rec, (a, b)
+
+
+def tile_width(m, n, launches):
+    """Stage-1 tile width of a single-pair sweep from its launch count: one launch per tile
+    anti-diagonal, S + G - 1 launches for S 256-row stripes and G segments."""
+    G = max(1, launches - (-(-m // 256)) + 1)
+    return -(-n // G)
 
 
 def device_pairs(torch, pairs):

@@ -97,9 +97,11 @@ int64_t tile_cols(swh_ctx* c, int64_t cols, int64_t want, int wps) {
 }
 
 // Relative time of a wave that shares its SIMD with a second sweep wave: two waves per
-// SIMD retire 1.28x the steps of one (measured on MI355X: the second wave issues while the
-// first waits on scalar, LDS and branch instructions).
-constexpr double kTwoWaveCost = 2.0 / 1.28;
+// SIMD retire 1.31x the steps of one (measured on MI355X on the 1 Mbp pair: 91.9 us per
+// launch of 489-column tiles at two waves against 113.5 us for 977-column tiles at one; the
+// second wave issues while the first waits on dependencies, LDS and branches).  With it the
+// model predicts the measured ratio of the two sweeps (0.978 against 0.977).
+constexpr double kTwoWaveCost = 2.0 / 1.31;
 
 // Modelled duration (in tile steps) of a sweep over rows x cols with wps waves per SIMD:
 // one launch per tile anti-diagonal, each lasting ceil(tiles / resident waves) tile times.

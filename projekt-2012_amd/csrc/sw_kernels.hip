@@ -30,8 +30,14 @@ namespace swk {
 
 namespace {
 
+// Steps unrolled per loop iteration: the int32 sweeps (one wave per SIMD for single pairs)
+// gain from the longer blocks the scheduler sees; the packed sweep's larger body runs best
+// at 8 (its 16-step loop no longer fits the instruction cache well).
 #ifndef UNROLL_STEPS
-#define UNROLL_STEPS 4
+#define UNROLL_STEPS 16
+#endif
+#ifndef UNROLL_STEPS_PK
+#define UNROLL_STEPS_PK 8
 #endif
 #define SWK_DO_PRAGMA(x) _Pragma(#x)
 #define SWK_UNROLL(n) SWK_DO_PRAGMA(unroll n)
@@ -526,7 +532,7 @@ __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileC
 #pragma unroll
         for (int t = 0; t < 2; ++t) sdump[t] = dump_step(T[t].C0, T[t].w, T[t].Kc, T[t].scol, T[t].m, c, row0[t], ddst[t]);
     }
-    SWK_UNROLL(UNROLL_STEPS)
+    SWK_UNROLL(UNROLL_STEPS_PK)
     for (int s = 0; s < 64; ++s) {
         s16x2 Ho_up = pk_from(shr1(Qh, pk_bits(Ho_bot)));
         s16x2 Fe_up = pk_from(shr1(Qf, pk_bits(Fe_bot)));
