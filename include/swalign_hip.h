@@ -71,7 +71,8 @@ typedef struct {
                                /* 0 = auto (about four tiles per CU across the row, >= 256)   */
     int32_t tb_lookahead;      /* traceback: checkpoint blocks recomputed ahead of the walk in  */
                                /* each direction per round (beyond 3, only blocks within 2 of  */
-                               /* the diagonal); 0 = auto (7; batches 0), -1 = none            */
+                               /* the diagonal); 0 = auto (7; batches: per round, as many as  */
+                               /* the walking pairs' blocks leave room for), -1 = none        */
     int32_t waves_per_simd;    /* sweep waves resident per SIMD (1 to 4), 0 = auto (batches    */
                                /* and the request server 4; single pairs 1 or 2 by a model)    */
     int32_t autotune;          /* batch: 1 = time candidate settings on a sample first          */

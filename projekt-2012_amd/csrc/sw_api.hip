@@ -131,7 +131,8 @@ int choose_wps(swh_ctx* c, int64_t rows, int64_t cols) {
 // widest region's, or G_fix when given: kDirs lays codes out by it).  Optional events are
 // recorded immediately before the first and after the last launch.
 int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions, int64_t W, int64_t G_fix,
-              hipEvent_t ev_before, hipEvent_t ev_after, int wps, int64_t* launches, int64_t tstride, bool packed) {
+              hipEvent_t ev_before, hipEvent_t ev_after, int wps, int64_t* launches, int64_t tstride, bool packed,
+              bool sync) {
     const int64_t nr = (int64_t)regions.size();
     std::vector<int64_t> Sr((size_t)nr), Gr((size_t)nr);
     int64_t G = 0, D = 0, cb_elems = 0, rb_elems = 0, cor_elems = 0;
@@ -175,7 +176,8 @@ int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions,
     std::sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
         return Sr[(size_t)x] + Gr[(size_t)x] > Sr[(size_t)y] + Gr[(size_t)y];
     });
-    std::vector<TileSpan> spans;
+    std::vector<TileSpan>& spans = c->host_spans;  // kept by the context: feeds an asynchronous copy
+    spans.clear();
     {
         int64_t total = 0;
         for (int64_t q = 0; q < nr; ++q) total += Sr[(size_t)q] + Gr[(size_t)q] - 1;
@@ -212,8 +214,10 @@ int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions,
     }
     if (ev_after) HCK(c, hipEventRecord(ev_after, c->stream));
     if (launches) *launches = D;
-    // the host vectors above fed asynchronous copies: keep them alive until those finish
-    HCK(c, hipStreamSynchronize(c->stream));
+    // `regions` and the context's span list fed asynchronous copies: either wait for them
+    // here, or (sync == false) the caller synchronises the stream before it changes
+    // `regions` or sweeps again
+    if (sync) HCK(c, hipStreamSynchronize(c->stream));
     return SWH_OK;
 }
 
@@ -628,7 +632,8 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
         dirs_cap = std::max<int64_t>(kTracebackDirsCap, std::min<int64_t>((int64_t)(freeb / 4), INT64_C(32) << 30));
     }
     int64_t guard = 0;
-    int64_t max_blocks = 0;
+    int64_t max_blocks = 0, max_blk_rows = 0;
+    for (int64_t k : active) max_blk_rows = std::max(max_blk_rows, jobs[k].pl.K);
     for (int64_t k : active)
         max_blocks = std::max(max_blocks, jobs[k].m / jobs[k].pl.K + jobs[k].n / jobs[k].pl.Kc + 8);
     while (!active.empty()) {
@@ -637,6 +642,19 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
         int64_t L = std::max(0, eo.lookahead);
         int64_t need = 0;
         for (int64_t k : active) need += blk_dirs(jobs[k]);
+        if (eo.lookahead_auto) {
+            // batches: blocks ahead of the walk while the codes sweep's launches still fit the
+            // resident waves (the rounds of the last, long pairs are latency-bound: their extra
+            // blocks cost no time and save rounds); none while many pairs walk
+            int64_t tiles = 0;  // tiles of one block of every walking pair
+            for (int64_t k : active) tiles += ceil_div(std::min(jobs[k].m, jobs[k].pl.K), kStripeRows) * Gblk;
+            const int64_t diags = ceil_div(std::min(max_m, max_blk_rows), kStripeRows) + Gblk - 1;
+            const int64_t slots = (int64_t)kSweepWaves * tb_wps * (tb_packed ? 2 : 1) *
+                                  std::max(1, c->props.multiProcessorCount);
+            L = 0;
+            for (int64_t l : {1, 2, 3, 5, 7})
+                if (lookahead_blocks(l) * tiles <= slots * diags) L = l;
+        }
         while (L > 0 && lookahead_blocks(L) * need > dirs_cap) --L;
         std::vector<int64_t> round;
         int64_t used = 0;
@@ -670,7 +688,15 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
                     r.r0 = (brow - da) * j.pl.K;
                     r.r1 = da == 0 ? st.i : std::min(j.m, r.r0 + j.pl.K) - 1;
                     r.c0 = (bcol - db) * j.pl.Kc;
-                    r.c1 = db == 0 ? st.j : std::min(j.n, r.c0 + j.pl.Kc) - 1;
+                    const int64_t c_end = std::min(j.n, r.c0 + j.pl.Kc) - 1;  // the block's last column
+                    r.c1 = db == 0 ? st.j : c_end;
+                    w.cols = r.c1 - r.c0 + 1;
+                    // The block cut at the current cell would end in a ragged tile; the columns
+                    // right of the cell are computed too, up to whole tiles within the block, so
+                    // that the packed sweep's two tiles per wave have equal widths (a wave whose
+                    // tiles differ in width takes per-value selects every step).  The walk
+                    // never visits them.
+                    r.c1 = std::min(c_end, r.c0 + ceil_div(w.cols, Wtb) * Wtb - 1);
                     r.top = r.r0 > 0 ? srow + j.srow_off + (r.r0 / j.pl.K - 1) * j.n + r.c0 : nullptr;
                     r.left = r.c0 > 0 ? scol + j.scol_off + (r.c0 / j.pl.Kc - 1) * j.m : nullptr;
                     r.K = j.pl.K;
@@ -678,7 +704,6 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
                     w.r0 = r.r0;
                     w.c0 = r.c0;
                     w.rows = r.r1 - r.r0 + 1;
-                    w.cols = r.c1 - r.c0 + 1;
                     slots.emplace_back(wi, dirs_elems);
                     dirs_elems += ceil_div(w.rows, kStripeRows) * Gblk * tstride * 64;
                     regions.push_back(r);
@@ -693,8 +718,22 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
         }
         // batches: packed 16-bit codes sweep (two tiles per wave) when stage 1 ran packed and the
         // tiles fit, two waves per SIMD; a single pair's rounds are latency-bound: int32, one wave
-        rc = run_sweep(c, kDirs, p, regions, Wtb, Gblk, nullptr, nullptr, tb_wps, nullptr, tstride, tb_packed);
+        // no synchronisation: the walk's arguments are built while the codes sweep runs (the
+        // round ends with one synchronisation for the walk states)
+        int64_t round_launches = 0;
+        rc = run_sweep(c, kDirs, p, regions, Wtb, Gblk, nullptr, nullptr, tb_wps, &round_launches, tstride, tb_packed,
+                       false);
         if (rc) return rc;
+        if (c->trace) {
+            int64_t cells = 0, tiles = 0;
+            for (const Region& r : regions) {
+                cells += (r.r1 - r.r0 + 1) * (r.c1 - r.c0 + 1);
+                tiles += ceil_div(r.r1 - r.r0 + 1, kStripeRows) * ceil_div(r.c1 - r.c0 + 1, Wtb);
+            }
+            fprintf(stderr, "swalign: round %lld: %lld pairs, lookahead %lld, %zu blocks, %lld tiles, %.3g cells, "
+                    "%lld launches\n", (long long)guard, (long long)nr, (long long)L, regions.size(), (long long)tiles,
+                    (double)cells, (long long)round_launches);
+        }
         // device arrays of the round; allocated before their addresses go into the walk args
         HCK(c, ensure(c->wregs, sizeof(WalkRegion) * wregs.size()));
         HCK(c, ensure(c->wargs, sizeof(WalkArgs) * wargs.size()));
@@ -859,6 +898,7 @@ EngineOpts engine_opts(const swh_opts* o, bool batch) {
     eo.batch = batch;
     eo.host_ckpt = o && o->host_checkpoints;
     eo.lookahead = batch ? 0 : 7;
+    eo.lookahead_auto = batch && !(o && o->tb_lookahead != 0);
     if (o && o->tb_lookahead != 0) eo.lookahead = std::max(0, o->tb_lookahead);
     return eo;
 }

@@ -56,6 +56,7 @@ struct swh_ctx {
     char* cigar_dst = nullptr;      // caller's CIGAR buffer (the result's cigar stays NULL)
     int64_t cigar_dst_cap = 0;
     int64_t dirs_cap_fixed = 0;     // direction-code bytes per traceback round, 0 = from free HBM
+    std::vector<swk::TileSpan> host_spans;  // launch spans of the latest sweep (source of its upload)
 };
 
 namespace swh_host {
@@ -85,7 +86,8 @@ int usable_wps(const swh_ctx* c, int w);
 // One sweep over `regions`, one launch per tile anti-diagonal (see sw_api.hip).
 int run_sweep(swh_ctx* c, int mode, swk::SweepParams p, std::vector<swk::Region>& regions, int64_t W,
               int64_t G_fix = 0, hipEvent_t ev_before = nullptr, hipEvent_t ev_after = nullptr, int wps = 1,
-              int64_t* launches = nullptr, int64_t tstride = 0, bool packed = false);
+              int64_t* launches = nullptr, int64_t tstride = 0, bool packed = false,
+              bool sync = true);
 
 // Widest stage-1 tile (columns) whose values stay in int16 relative to the tile's base under
 // scoring sc, for the packed stage 1; 0 if none.
@@ -153,6 +155,7 @@ struct EngineOpts {
     int precision = 0;       // 0 = packed 16-bit stage 1 where it cannot overflow, 32 = int32 only
     int wps = 0;             // sweep waves per SIMD, 0 = auto
     int lookahead = 7;       // traceback blocks ahead per direction (0 = none)
+    bool lookahead_auto = false;  // batches: lookahead chosen per round from the walking pairs
     bool want_xp = false;
     bool batch = false;      // many pairs: fixed tile width, two waves per SIMD
     bool host_ckpt = false;  // checkpoints in mapped pinned host memory
