@@ -487,9 +487,12 @@ __device__ void sweep_main(const SweepParams& p, SweepLds& lds) {
 typedef short s16x2 __attribute__((vector_size(4)));
 typedef unsigned short u16x2 __attribute__((vector_size(4)));
 // Per half: 1 if the (non-negative, below 65536) difference d is not zero, else 0.
+// One v_pk_min_u16, written as inline asm: the compiler recognises min(x, 1) as "x != 0" and
+// would lower it, per half, to two compares and two selects.
 __device__ __forceinline__ uint32_t differs(s16x2 d) {
-    const u16x2 one = {1, 1};
-    return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(u16x2, d), one));
+    uint32_t r;
+    asm("v_pk_min_u16 %0, %1, %2" : "=v"(r) : "v"(__builtin_bit_cast(uint32_t, d)), "v"(0x00010001u));
+    return r;
 }
 
 __device__ __forceinline__ int pk_bits(s16x2 v) { return __builtin_bit_cast(int, v); }
@@ -519,10 +522,13 @@ __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileC
     const s16x2 vO = pk2(p.sc.O, p.sc.O), vE = pk2(p.sc.E, p.sc.E), one = pk2(1, 1);
     const s16x2 vMO = pk2(p.sc.M - p.sc.O, p.sc.M - p.sc.O);
     const s16x2 vD = pk2(p.sc.M - p.sc.X, p.sc.M - p.sc.X);  // match - mismatch
+    // kDirs: both tiles' code columns.  A wave without a second tile stores the second half's
+    // codes into the first tile's slot before the first half's own, which the later store
+    // overwrites (one lane's stores to one address complete in program order): no branch.
     uint16_t* dcol[2] = {nullptr, nullptr};
     if (MODE == kDirs) {
-#pragma unroll
-        for (int t = 0; t < 2; ++t) dcol[t] = T[t].dirs ? T[t].dirs + (64 * c) * 64 + lane : nullptr;
+        dcol[0] = T[0].dirs + (64 * c) * 64 + lane;
+        dcol[1] = T[1].dirs ? T[1].dirs + (64 * c) * 64 + lane : dcol[0];
     }
     const uint32_t col0 = (uint32_t)(64 * c) - (uint32_t)lane;  // the lane's tile column at step 0
     const int s_end = END ? (int)(T[0].w - (64 * c - lane)) : 64;  // first step past column w - 1
@@ -573,12 +579,15 @@ __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileC
                 // (diagonal), 2 (F), 3 (E); bit 2: E extends; bit 3: F extends.  Differences
                 // H - x are >= 0, so min(., 1) is the half's "differs" flag; the sign bit of
                 // Ho - Ee is "E extends".  Halves never carry into each other (values <= 15).
+                // The sign bits go straight to their nibble positions (one shift and one
+                // and-or each) and h is shifted in with the or.
                 const uint32_t nz = differs(H - zero), nd = differs(H - Hd), nf = differs(H - F);
                 const uint32_t a1 = nz & nd;
                 const uint32_t h = nz + a1 + (a1 & nf);
-                const uint32_t ee = ((uint32_t)pk_bits(Ho[k] - Ee) >> 15) & 0x00010001u;
-                const uint32_t fe = ((uint32_t)pk_bits(Ho_up - Fe_up) >> 15) & 0x00010001u;
-                nib |= (h | (ee << 2) | (fe << 3)) << (4 * k);
+                const uint32_t eb = 0x00010001u << (4 * k + 2), fb = 0x00010001u << (4 * k + 3);
+                nib = (((uint32_t)pk_bits(Ho[k] - Ee) >> (13 - 4 * k)) & eb) | nib;
+                nib = (((uint32_t)pk_bits(Ho_up - Fe_up) >> (12 - 4 * k)) & fb) | nib;
+                nib = (h << (4 * k)) | nib;
             }
             const s16x2 Hon = H + vO;
             if (EDGE) {
@@ -598,8 +607,8 @@ __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileC
         s_out.h[s] = pk_bits(Ho_bot);  // see chunk_steps
         s_out.f[s] = pk_bits(Fe_bot);
         if (MODE == kDirs) {
-            if (dcol[0]) dcol[0][s * 64] = (uint16_t)nib;
-            if (dcol[1]) dcol[1][s * 64] = (uint16_t)(nib >> 16);
+            dcol[1][s * 64] = (uint16_t)(nib >> 16);  // first: see dcol
+            dcol[0][s * 64] = (uint16_t)nib;
         }
         if (DUMP) {
 #pragma unroll
