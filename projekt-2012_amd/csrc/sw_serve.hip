@@ -80,13 +80,55 @@ struct Pool {
     }
 };
 
-enum Phase { kQueued, kSweep, kArgmax, kWalk, kDone };
+enum Phase { kQueued, kSweep, kArgmax, kWalk, kFetch, kDone };
+
+// The server's small transfers are batched per tick: uploads (sequences of new jobs, reset
+// values) are packed into one pinned buffer, copied with one transfer and scattered by one
+// kernel; read-backs (end keys, found columns, walk states, finished jobs' ops) are gathered
+// by one kernel into one buffer and copied back with one transfer.  One copy dispatch per
+// value cost ~4 us of device time each (tens of thousands per second of small jobs).
+struct Xfer {
+    uint8_t* dst;
+    const uint8_t* src;
+    int64_t len;
+};
+constexpr int64_t kXferChunk = INT64_C(64) << 10;  // bytes per copy workgroup
+
+// One workgroup per entry; 16-byte words when the entry is aligned.
+__global__ __launch_bounds__(256) void k_copy_list(const Xfer* x) {
+    const Xfer e = x[blockIdx.x];
+    if ((((uintptr_t)e.dst | (uintptr_t)e.src | (uintptr_t)e.len) & 15) == 0) {
+        const uint4* src = (const uint4*)e.src;
+        uint4* dst = (uint4*)e.dst;
+        for (int64_t k = threadIdx.x; k < e.len / 16; k += 256) dst[k] = src[k];
+    } else {
+        for (int64_t k = threadIdx.x; k < e.len; k += 256) e.dst[k] = e.src[k];
+    }
+}
+
+// A grow-only pinned host buffer.
+struct PinBuf {
+    uint8_t* h = nullptr;
+    size_t cap = 0;
+};
+hipError_t ensure_pin(PinBuf& b, size_t bytes) {
+    if (b.h && b.cap >= bytes) return hipSuccess;
+    if (b.h) (void)hipHostFree(b.h);
+    b = PinBuf{};
+    const hipError_t e = hipHostMalloc((void**)&b.h, std::max<size_t>(bytes, 4096), 0);
+    if (e == hipSuccess) b.cap = std::max<size_t>(bytes, 4096);
+    return e;
+}
+
+int64_t al16(int64_t x) { return (x + 15) & ~INT64_C(15); }
 
 // Device and host copies of one tick's sweep descriptors.  Ticks that need no read-back
 // are not synchronised, so the host runs up to kTickRing ticks ahead of the device; a
 // slot is reused once the event recorded after its launches has completed.
 struct TickSlot {
     DevBuf regs[3], spans[3];
+    PinBuf up_h;   // this tick's uploads: data, then the copy list
+    DevBuf up_d;
     std::vector<Region> hregs[3];
     std::vector<TileSpan> hspans[3];
     hipEvent_t done = nullptr;
@@ -131,6 +173,8 @@ struct SJob {
     WalkState st{};
     unsigned long long key_h = 0;
     int32_t found_h = -1;
+    std::vector<uint8_t> ops_h;      // kFetch: the walk's ops, read back
+    std::vector<XPoint> xps_h;
     int status = SWH_OK;
     bool published = false;  // result visible to swh_server_poll (set under the server mutex)
     swh_result res{};
@@ -149,6 +193,23 @@ struct swh_server {
     TickSlot ring[kTickRing];
     int ring_at = 0;
     DevBuf argjobs, wargs, wregs;
+    // batched transfers: uploads queued for the next tick (a value copied here, or a host
+    // buffer that outlives the tick), read-backs of the current tick
+    struct Up {
+        uint8_t* dst;
+        const uint8_t* host;  // nullptr: up_vals[val]
+        int64_t len, val;
+    };
+    std::vector<Up> up;
+    std::vector<uint8_t> up_vals;
+    struct Rb {
+        const uint8_t* dev;
+        int64_t len;
+        void* host;
+    };
+    std::vector<Rb> rb;
+    PinBuf rb_h, rbx_h;
+    DevBuf rb_d, rbx_d;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<SJob*> queue;
@@ -201,6 +262,82 @@ void layout(swh_server* s, SJob& j) {
     const int64_t tb = lookahead_blocks(j.L) * blk;
     j.work_bytes = std::max({stage1, search, tb});
     j.bytes = persist + j.work_bytes;
+}
+
+void queue_value(swh_server* s, void* dst, const void* v, int64_t len) {
+    const int64_t at = (int64_t)s->up_vals.size();
+    s->up_vals.insert(s->up_vals.end(), (const uint8_t*)v, (const uint8_t*)v + len);
+    s->up.push_back(swh_server::Up{(uint8_t*)dst, nullptr, len, at});
+}
+void queue_ref(swh_server* s, void* dst, const void* host, int64_t len) {
+    if (len > 0) s->up.push_back(swh_server::Up{(uint8_t*)dst, (const uint8_t*)host, len, 0});
+}
+
+// Appends the copy of [src, src + len) to dst to xs, in pieces of at most kXferChunk bytes.
+void add_xfer(std::vector<Xfer>& xs, uint8_t* dst, const uint8_t* src, int64_t len) {
+    for (int64_t o = 0; o < len; o += kXferChunk)
+        xs.push_back(Xfer{dst + o, src + o, std::min(kXferChunk, len - o)});
+}
+
+// Issues the uploads queued since the last tick (one transfer, one scatter kernel).
+int flush_uploads(swh_server* s, TickSlot& slot) {
+    swh_ctx* c = s->ctx;
+    if (s->up.empty()) return SWH_OK;
+    int64_t data = 0;
+    for (const auto& u : s->up) data += al16(u.len);
+    int64_t nx = 0;
+    for (const auto& u : s->up) nx += ceil_div(u.len, kXferChunk);
+    const int64_t total = data + nx * (int64_t)sizeof(Xfer);
+    HCK(c, ensure_pin(slot.up_h, (size_t)total));
+    HCK(c, ensure(slot.up_d, (size_t)total));
+    std::vector<Xfer> xs;
+    xs.reserve((size_t)nx);
+    int64_t off = 0;
+    for (const auto& u : s->up) {
+        memcpy(slot.up_h.h + off, u.host ? u.host : s->up_vals.data() + u.val, (size_t)u.len);
+        add_xfer(xs, u.dst, (const uint8_t*)slot.up_d.p + off, u.len);
+        off += al16(u.len);
+    }
+    memcpy(slot.up_h.h + data, xs.data(), sizeof(Xfer) * xs.size());
+    HCK(c, hipMemcpyAsync(slot.up_d.p, slot.up_h.h, (size_t)total, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_copy_list, dim3((unsigned)xs.size()), dim3(256), 0, c->stream,
+                       (const Xfer*)((const uint8_t*)slot.up_d.p + data));
+    HCK(c, hipGetLastError());
+    s->up.clear();
+    s->up_vals.clear();
+    return SWH_OK;
+}
+
+// Gathers this tick's read-backs (one kernel, one transfer) and waits for them.
+int run_readbacks(swh_server* s) {
+    swh_ctx* c = s->ctx;
+    int64_t data = 0;
+    for (const auto& r : s->rb) data += al16(r.len);
+    std::vector<Xfer> xs;
+    HCK(c, ensure(s->rb_d, (size_t)std::max<int64_t>(data, 16)));
+    int64_t off = 0;
+    for (const auto& r : s->rb) {
+        add_xfer(xs, (uint8_t*)s->rb_d.p + off, r.dev, r.len);
+        off += al16(r.len);
+    }
+    if (!xs.empty()) {
+        HCK(c, ensure_pin(s->rbx_h, sizeof(Xfer) * xs.size()));
+        HCK(c, ensure(s->rbx_d, sizeof(Xfer) * xs.size()));
+        HCK(c, ensure_pin(s->rb_h, (size_t)std::max<int64_t>(data, 16)));
+        memcpy(s->rbx_h.h, xs.data(), sizeof(Xfer) * xs.size());
+        HCK(c, hipMemcpyAsync(s->rbx_d.p, s->rbx_h.h, sizeof(Xfer) * xs.size(), hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_copy_list, dim3((unsigned)xs.size()), dim3(256), 0, c->stream, (const Xfer*)s->rbx_d.p);
+        HCK(c, hipGetLastError());
+        HCK(c, hipMemcpyAsync(s->rb_h.h, s->rb_d.p, (size_t)data, hipMemcpyDeviceToHost, c->stream));
+    }
+    HCK(c, hipStreamSynchronize(c->stream));
+    off = 0;
+    for (const auto& r : s->rb) {
+        memcpy(r.host, s->rb_h.h + off, (size_t)r.len);
+        off += al16(r.len);
+    }
+    s->rb.clear();
+    return SWH_OK;
 }
 
 void place(swh_server* s, SJob& j) {
@@ -355,6 +492,10 @@ int tick(swh_server* s) {
         HCK(c, hipEventSynchronize(slot.done));
         slot.pending = false;
     }
+    {
+        const int rc = flush_uploads(s, slot);
+        if (rc) return rc;
+    }
     // ---- sweeps: per mode, kDiagonalsPerTick launches, each over every job's next diagonal
     std::vector<Region>* regs = slot.hregs;
     std::vector<TileSpan>* spans = slot.hspans;
@@ -434,8 +575,6 @@ int tick(swh_server* s) {
             max_m = std::max(max_m, j->m);
         }
     if (!aj.empty()) {
-        for (SJob* j : s->flight)
-            if (j->phase == kArgmax) HCK(c, hipMemsetAsync(j->key, 0, 8, c->stream));
         HCK(c, ensure(s->argjobs, sizeof(ArgmaxJob) * aj.size()));
         HCK(c, hipMemcpyAsync(s->argjobs.p, aj.data(), sizeof(ArgmaxJob) * aj.size(), hipMemcpyHostToDevice,
                               c->stream));
@@ -484,20 +623,28 @@ int tick(swh_server* s) {
     // ---- read back what decides each job's next step (only when some job needs it) -------
     bool readback = false;
     for (SJob* j : s->flight) {
-        readback = readback || j->phase == kArgmax || j->phase == kWalk ||
+        readback = readback || j->phase == kArgmax || j->phase == kWalk || j->phase == kFetch ||
                    (j->phase == kSweep && j->mode == kSearch && j->d + kDiagonalsPerTick >= j->D);
     }
     for (SJob* j : s->flight) {
         if (!readback) break;
-        if (j->phase == kArgmax)
-            HCK(c, hipMemcpyAsync(&j->key_h, j->key, 8, hipMemcpyDeviceToHost, c->stream));
-        else if (j->phase == kSweep && j->mode == kSearch && j->d + kDiagonalsPerTick >= j->D)
-            HCK(c, hipMemcpyAsync(&j->found_h, j->found, 4, hipMemcpyDeviceToHost, c->stream));
-        else if (j->phase == kWalk)
-            HCK(c, hipMemcpyAsync(&j->st, j->walk, sizeof(WalkState), hipMemcpyDeviceToHost, c->stream));
+        if (j->phase == kArgmax) {
+            s->rb.push_back(swh_server::Rb{(const uint8_t*)j->key, 8, &j->key_h});
+        } else if (j->phase == kSweep && j->mode == kSearch && j->d + kDiagonalsPerTick >= j->D) {
+            s->rb.push_back(swh_server::Rb{(const uint8_t*)j->found, 4, &j->found_h});
+        } else if (j->phase == kWalk) {
+            s->rb.push_back(swh_server::Rb{(const uint8_t*)j->walk, (int64_t)sizeof(WalkState), &j->st});
+        } else if (j->phase == kFetch) {
+            j->ops_h.resize((size_t)j->st.nops + 1);
+            if (j->st.nops) s->rb.push_back(swh_server::Rb{j->ops, j->st.nops, j->ops_h.data()});
+            const int64_t nx = s->opts.want_xpoints ? std::min(j->st.nxp, j->xcap) : 0;
+            j->xps_h.resize((size_t)std::max<int64_t>(nx, 1));
+            if (nx) s->rb.push_back(swh_server::Rb{(const uint8_t*)j->xps, nx * (int64_t)sizeof(XPoint), j->xps_h.data()});
+        }
     }
     if (readback) {
-        HCK(c, hipStreamSynchronize(c->stream));
+        const int rc = run_readbacks(s);
+        if (rc) return rc;
         for (TickSlot& t : s->ring) t.pending = false;
     }
     // ---- advance ------------------------------------------------------------------------
@@ -508,6 +655,8 @@ int tick(swh_server* s) {
                 if (j->d < j->D) break;
                 if (j->mode == kStage1) {
                     j->phase = kArgmax;
+                    const unsigned long long zero = 0;
+                    queue_value(s, j->key, &zero, 8);
                 } else if (j->mode == kSearch) {
                     if (j->found_h == INT32_MAX || j->found_h < 0) {
                         finish(s, *j, SWH_ERR_INTERNAL);
@@ -526,7 +675,7 @@ int tick(swh_server* s) {
                     j->st.start_i = j->bi;
                     j->st.start_j = j->bj;
                     j->st.last_row = j->bi;
-                    HCK(c, hipMemcpyAsync(j->walk, &j->st, sizeof(WalkState), hipMemcpyHostToDevice, c->stream));
+                    queue_value(s, j->walk, &j->st, sizeof(WalkState));
                     setup_dirs(s, *j);
                 } else {
                     j->phase = kWalk;
@@ -540,27 +689,27 @@ int tick(swh_server* s) {
                     finish(s, *j, SWH_OK);
                     break;
                 }
-                HCK(c, hipMemcpyAsync(j->found, &kNotFound, 4, hipMemcpyHostToDevice, c->stream));
+                queue_value(s, j->found, &kNotFound, 4);
                 setup_search(s, *j);
                 break;
             }
             case kWalk: {
                 const Region& r = j->regs[0];
                 if (j->st.done) {
-                    std::vector<uint8_t> ops((size_t)j->st.nops + 1);
-                    std::vector<XPoint> xps((size_t)std::max<int64_t>(j->st.nxp, 1));
-                    if (j->st.nops)
-                        HCK(c, hipMemcpy(ops.data(), j->ops, (size_t)j->st.nops, hipMemcpyDeviceToHost));
-                    if (s->opts.want_xpoints && j->st.nxp)
-                        HCK(c, hipMemcpy(xps.data(), j->xps, sizeof(XPoint) * j->st.nxp, hipMemcpyDeviceToHost));
-                    const int rc = fill_alignment(j->res, j->st, ops.data(), xps.data(), j->xcap, j->best,
-                                                  s->opts.want_xpoints != 0);
-                    finish(s, *j, rc);
+                    j->phase = kFetch;  // its ops are read back next tick
                 } else if (j->st.i >= r.r0 && j->st.j >= r.c0) {
                     finish(s, *j, SWH_ERR_INTERNAL);  // no progress
                 } else {
                     setup_dirs(s, *j);
                 }
+                break;
+            }
+            case kFetch: {
+                const int rc = fill_alignment(j->res, j->st, j->ops_h.data(), j->xps_h.data(),
+                                              std::min(j->st.nxp, j->xcap), j->best, s->opts.want_xpoints != 0);
+                std::vector<uint8_t>().swap(j->ops_h);
+                std::vector<XPoint>().swap(j->xps_h);
+                finish(s, *j, rc);
                 break;
             }
             default:
@@ -621,12 +770,8 @@ void serve_loop(swh_server* s) {
         for (SJob* j : s->flight) {
             if (j->phase != kQueued) continue;
             place(s, *j);
-            if (j->m)
-                rc = rc ? rc : (hipMemcpyAsync(j->dA, j->a.data(), (size_t)j->m, hipMemcpyHostToDevice,
-                                               s->ctx->stream) == hipSuccess ? SWH_OK : SWH_ERR_HIP);
-            if (j->n)
-                rc = rc ? rc : (hipMemcpyAsync(j->dB, j->b.data(), (size_t)j->n, hipMemcpyHostToDevice,
-                                               s->ctx->stream) == hipSuccess ? SWH_OK : SWH_ERR_HIP);
+            queue_ref(s, j->dA, j->a.data(), j->m);  // uploaded at the start of the next tick
+            queue_ref(s, j->dB, j->b.data(), j->n);
             j->res.m = j->m;
             j->res.n = j->n;
             j->res.swapped = j->swapped ? 1 : 0;
@@ -853,9 +998,13 @@ int swh_server_destroy(swh_server* s) {
             if (t.regs[mo].p) (void)hipFree(t.regs[mo].p);
             if (t.spans[mo].p) (void)hipFree(t.spans[mo].p);
         }
+        if (t.up_d.p) (void)hipFree(t.up_d.p);
+        if (t.up_h.h) (void)hipHostFree(t.up_h.h);
         if (t.done) (void)hipEventDestroy(t.done);
     }
-    DevBuf* bufs[] = {&s->argjobs, &s->wargs, &s->wregs};
+    for (PinBuf* b : {&s->rb_h, &s->rbx_h})
+        if (b->h) (void)hipHostFree(b->h);
+    DevBuf* bufs[] = {&s->argjobs, &s->wargs, &s->wregs, &s->rb_d, &s->rbx_d};
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
     if (s->pool_base) (void)hipFree(s->pool_base);
