@@ -188,6 +188,8 @@ struct swh_server {
     swh_opts opts{};
     int32_t max_in_flight = 0;
     int wps = 1;  // sweep waves per SIMD of the shared launches
+    int64_t w16 = 0;      // widest tile whose values fit int16 around its base (0: none)
+    bool packed = false;  // stage-1 and codes sweeps in packed 16-bit arithmetic
     uint8_t* pool_base = nullptr;
     Pool pool;
     TickSlot ring[kTickRing];
@@ -234,7 +236,20 @@ struct swh_server {
 
 namespace {
 
+// Tile width of a server sweep over `cols` columns: the options', else at least the batch
+// width (512: a tile's 63 fill steps cost about a tenth) and wider for long rows; never
+// wider than the packed sweep's int16 bound, so that every stage-1 and codes launch can run
+// packed (two tiles per wave).
+int64_t server_tile(const swh_server* s, int64_t cols);
+
 // Working bytes of one job: everything it keeps, plus the largest per-phase set.
+int64_t server_tile(const swh_server* s, int64_t cols) {
+    if (s->opts.tile_cols > 0) return s->opts.tile_cols;
+    int64_t W = std::max<int64_t>(512, tile_cols(s->ctx, cols, 0));
+    if (s->packed) W = std::min(W, s->w16);
+    return W;
+}
+
 void layout(swh_server* s, SJob& j) {
     const int64_t m = j.m, n = j.n;
     auto al = [](int64_t x) { return ceil_div(x, 256) * 256; };
@@ -251,7 +266,7 @@ void layout(swh_server* s, SJob& j) {
     const int64_t stage1 = al(16 * n) + al(16 * m) + al(8 * S);
     const int64_t bw = std::min(n, j.pl.Kc), bh = std::min(m, j.pl.K), nq = ceil_div(n, bw);
     const int64_t search = al(16 * n) + nq * (al(16 * bh) + al(8 * ceil_div(bh, kStripeRows)));
-    const int64_t W = tile_cols(s->ctx, bw, s->opts.tile_cols);
+    const int64_t W = server_tile(s, bw);
     const int64_t tstride = ceil_div(W + 63, 64) * 64, G = ceil_div(bw, W);
     const int64_t dirs = ceil_div(bh, kStripeRows) * G * tstride * 64 * 2;
     const int64_t blk = al(dirs) + al(16 * bw) + al(16 * bh) + al(8 * ceil_div(bh, kStripeRows));
@@ -416,7 +431,7 @@ void setup_stage1(swh_server* s, SJob& j) {
     r.srow = j.srow;
     r.scol = j.scol;
     j.regs.assign(1, r);
-    start_sweep(j, kStage1, tile_cols(s->ctx, j.n, s->opts.tile_cols), 0, 0);
+    start_sweep(j, kStage1, server_tile(s, j.n), 0, 0);
 }
 
 void setup_search(swh_server* s, SJob& j) {
@@ -435,14 +450,14 @@ void setup_search(swh_server* s, SJob& j) {
         r.found = j.found;
         j.regs.push_back(r);
     }
-    start_sweep(j, kSearch, tile_cols(s->ctx, bw, s->opts.tile_cols), 0, 0);
+    start_sweep(j, kSearch, server_tile(s, bw), 0, 0);
 }
 
 // A traceback round: the block of the walk's current cell (cut at the cell, region 0) and
 // the full blocks up to L above and to the left, each from its own checkpoints.
 void setup_dirs(swh_server* s, SJob& j) {
     const int64_t bw = std::min(j.pl.Kc, j.n);
-    const int64_t W = tile_cols(s->ctx, bw, s->opts.tile_cols);
+    const int64_t W = server_tile(s, bw);
     const int64_t tstride = ceil_div(W + 63, 64) * 64, G = ceil_div(bw, W);
     const int64_t brow = j.st.i / j.pl.K, bcol = j.st.j / j.pl.Kc, L = j.L;
     j.regs.clear();
@@ -546,6 +561,7 @@ int tick(swh_server* s) {
         for (SJob* j : s->flight)
             for (int w = 0; w < 8; ++w) mask[w] |= j->alpha[w];
         p.code_shift1 = code_shift1_for(mask, &s->sc);
+        p.code2_shift1 = code2_shift1_for(mask, &s->sc);
     }
     for (int mo = 0; mo < 3; ++mo) {
         if (regs[mo].empty() || spans[mo].empty()) continue;
@@ -561,7 +577,8 @@ int tick(swh_server* s) {
             p.spans = (const TileSpan*)slot.spans[mo].p + span_at[mo][(size_t)step];
             p.nspans = (int32_t)(span_at[mo][(size_t)step + 1] - span_at[mo][(size_t)step]);
             p.ntiles = tiles_at[mo][(size_t)step];
-            HCK(c, launch_sweep(mo, p, c->lds_pad[s->wps], c->stream));
+            if (s->packed && mo != kSearch) HCK(c, launch_sweep_pk(mo, p, c->lds_pad[s->wps], c->stream));
+            else HCK(c, launch_sweep(mo, p, c->lds_pad[s->wps], c->stream));
         }
     }
     HCK(c, hipEventRecord(slot.done, c->stream));
@@ -776,7 +793,7 @@ void serve_loop(swh_server* s) {
             j->res.n = j->n;
             j->res.swapped = j->swapped ? 1 : 0;
             j->res.end_row = j->res.end_col = j->res.start_row = j->res.start_col = -1;
-            j->res.precision_used = 32;
+            j->res.precision_used = s->packed ? 16 : 32;
             j->res.checkpoint_rows = j->pl.nsr ? j->pl.K : 0;
             j->res.checkpoint_cols = j->pl.nsc ? j->pl.Kc : 0;
             if (j->m == 0 || j->n == 0) finish(s, *j, SWH_OK);
@@ -850,6 +867,10 @@ int swh_server_create(int device, const swh_scoring* sc, const swh_opts* opts, i
     // a tick's launches hold the tiles of every job in flight: four waves per SIMD hide the
     // waits of one (100-job stream, all submitted at once: 815 / 946 / 968 GCUPS at 1 / 2 / 4)
     s->wps = usable_wps(c, s->opts.waves_per_simd > 0 ? s->opts.waves_per_simd : (c->wps_fixed ? c->wps_fixed : 4));
+    // packed 16-bit stage 1 and codes sweeps, as for batches (values held relative to each
+    // tile's base, so any absolute score), unless int32 is asked for or no tile width fits
+    s->w16 = int16_tile_cols(sc);
+    s->packed = s->opts.precision != 32 && s->w16 > 0 && (s->opts.tile_cols <= 0 || s->opts.tile_cols <= s->w16);
     size_t freeb = 0, totalb = 0;
     (void)hipMemGetInfo(&freeb, &totalb);
     int64_t budget = hbm_budget_bytes > 0 ? hbm_budget_bytes : (int64_t)(freeb / 2);
