@@ -363,6 +363,155 @@ struct StageClock {
     }
 };
 
+// Traceback rounds of a batch planned on the device, one block per walking pair and round
+// (no lookahead), while the walking pairs' tiles fill the device: the plan kernel builds each
+// pair's region and walk arguments from its walk state in device memory, the codes sweep and
+// the walk run as in host-planned rounds, and a compaction kernel lists the pairs that still
+// walk.  The host reads the number of walking pairs back once every kRoundsPerSync rounds
+// (the launches shrink to it) and stops when the rest would be better served by host-planned
+// rounds with lookahead.  On return `active` and `states` hold the pairs still walking.
+constexpr int kRoundsPerSync = 4;
+
+int device_rounds(swh_ctx* c, std::vector<Job>& jobs, std::vector<int64_t>& active, std::vector<WalkState>& states,
+                  SweepParams p, int2* srow, int2* scol, int64_t Wtb, int64_t tstride, int64_t Gblk, bool tb_packed,
+                  int tb_wps, int64_t dirs_cap, int64_t& guard, int64_t guard_max) {
+    const int64_t nj = (int64_t)jobs.size();
+    const int64_t N0 = (int64_t)active.size();
+    int64_t Kb = 0, Wb = 0;  // rows and columns of the largest block
+    for (int64_t k : active) {
+        Kb = std::max(Kb, std::min(jobs[k].m, jobs[k].pl.K));
+        Wb = std::max(Wb, std::min(jobs[k].n, jobs[k].pl.Kc));
+    }
+    const int64_t Sb = ceil_div(Kb, kStripeRows);
+    const int64_t D = Sb + Gblk - 1;
+    const int64_t slot_dirs = Sb * Gblk * tstride * 64;  // code elements per slot
+    const int64_t slots = (int64_t)kSweepWaves * tb_wps * (tb_packed ? 2 : 1) * std::max(1, c->props.multiProcessorCount);
+    // the host-planned rounds would look ahead (tiles of all walking pairs at one block
+    // ahead fit the resident waves): leave the rest to them
+    auto fills = [&](int64_t nact) { return lookahead_blocks(1) * nact * Sb * Gblk > slots * D; };
+    // every walking pair holds a code slot: up to half of the free memory (the code buffer
+    // already held counts as free), at least the host-planned rounds' share
+    size_t freeb = 0, totalb = 0;
+    (void)hipMemGetInfo(&freeb, &totalb);
+    const int64_t cap = std::max<int64_t>(dirs_cap, ((int64_t)freeb + (int64_t)c->dirs.cap) / 2);
+    if (N0 == 0 || !fills(N0) || N0 >= INT32_MAX || N0 * slot_dirs * (int64_t)sizeof(uint16_t) > cap)
+        return SWH_OK;
+
+    std::vector<TbPair> tp((size_t)nj);
+    for (int64_t k : active) {
+        const Job& j = jobs[k];
+        TbPair& q = tp[(size_t)k];
+        q.A = j.A;
+        q.B = j.B;
+        q.m = j.m;
+        q.n = j.n;
+        q.K = j.pl.K;
+        q.Kc = j.pl.Kc;
+        q.srow = j.pl.nsr ? srow + j.srow_off : nullptr;
+        q.scol = j.pl.nsc ? scol + j.scol_off : nullptr;
+        q.ops = (uint8_t*)c->ops.p + j.ops_off;
+        q.xps = (XPoint*)c->xps.p + j.xps_off;
+        q.xcap = j.xcap;
+    }
+    std::vector<int32_t> act0((size_t)N0 + 2);  // two counters, then the first list
+    act0[0] = (int32_t)N0;
+    act0[1] = 0;
+    for (int64_t x = 0; x < N0; ++x) act0[(size_t)x + 2] = (int32_t)active[(size_t)x];
+    std::vector<TileSpan> spans((size_t)(D * N0));
+    std::vector<int64_t> per_diag((size_t)D);
+    for (int64_t d = 0; d < D; ++d) {
+        const int64_t s_lo = std::max<int64_t>(0, d - (Gblk - 1)), s_hi = std::min(Sb - 1, d);
+        per_diag[(size_t)d] = s_hi - s_lo + 1;
+        for (int64_t x = 0; x < N0; ++x)
+            spans[(size_t)(d * N0 + x)] = TileSpan{x * per_diag[(size_t)d], (int32_t)x, (int32_t)s_lo, d};
+    }
+    HCK(c, ensure(c->tb_pairs, sizeof(TbPair) * (size_t)nj));
+    HCK(c, ensure(c->tb_states, sizeof(WalkState) * (size_t)nj));
+    HCK(c, ensure(c->tb_act, sizeof(int32_t) * (size_t)(2 * N0 + 2)));
+    HCK(c, ensure(c->tb_pre, sizeof(TbPre) * (size_t)N0));
+    HCK(c, ensure(c->regions, sizeof(Region) * (size_t)N0));
+    HCK(c, ensure(c->wregs, sizeof(WalkRegion) * (size_t)N0));
+    HCK(c, ensure(c->wargs, sizeof(WalkArgs) * (size_t)N0));
+    HCK(c, ensure(c->spans, sizeof(TileSpan) * spans.size()));
+    HCK(c, ensure(c->cb, sizeof(int2) * (size_t)(N0 * 2 * Kb)));
+    HCK(c, ensure(c->rbuf, sizeof(int2) * (size_t)(N0 * 2 * Wb)));
+    HCK(c, ensure(c->cor, sizeof(int32_t) * (size_t)(N0 * 2 * Sb)));
+    HCK(c, ensure(c->dirs, sizeof(uint16_t) * (size_t)(N0 * slot_dirs)));
+    HCK(c, hipMemcpyAsync(c->tb_pairs.p, tp.data(), sizeof(TbPair) * tp.size(), hipMemcpyHostToDevice, c->stream));
+    HCK(c, hipMemcpyAsync(c->tb_states.p, states.data(), sizeof(WalkState) * (size_t)nj, hipMemcpyHostToDevice,
+                          c->stream));
+    HCK(c, hipMemcpyAsync(c->tb_act.p, act0.data(), sizeof(int32_t) * act0.size(), hipMemcpyHostToDevice, c->stream));
+    HCK(c, hipMemcpyAsync(c->spans.p, spans.data(), sizeof(TileSpan) * spans.size(), hipMemcpyHostToDevice,
+                          c->stream));
+    int32_t* cnt = (int32_t*)c->tb_act.p;
+    int32_t* lists[2] = {cnt + 2, cnt + 2 + N0};
+    TbRoundArgs ra{};
+    ra.pairs = (const TbPair*)c->tb_pairs.p;
+    ra.states = (WalkState*)c->tb_states.p;
+    ra.regions = (Region*)c->regions.p;
+    ra.wregs = (WalkRegion*)c->wregs.p;
+    ra.wargs = (WalkArgs*)c->wargs.p;
+    ra.pre = (TbPre*)c->tb_pre.p;
+    ra.W = Wtb;
+    ra.G = Gblk;
+    ra.tstride = tstride;
+    ra.cb = (int2*)c->cb.p;
+    ra.cb_stride = 2 * Kb;
+    ra.rb = (int2*)c->rbuf.p;
+    ra.rb_stride = 2 * Wb;
+    ra.cor = (int32_t*)c->cor.p;
+    ra.cor_stride = 2 * Sb;
+    ra.dirs = (uint16_t*)c->dirs.p;
+    ra.dirs_stride = slot_dirs;
+    p.regions = (const Region*)c->regions.p;
+    const size_t lds = c->lds_pad[usable_wps(c, tb_wps)];
+    int cur = 0;
+    int64_t nb = N0;
+    int32_t walking = (int32_t)N0;
+    while (walking > 0 && fills(walking)) {
+        for (int r = 0; r < kRoundsPerSync; ++r) {
+            if (++guard > guard_max) return SWH_ERR_INTERNAL;
+            ra.act_in = lists[cur];
+            ra.nact_in = cnt + cur;
+            ra.act_out = lists[1 - cur];
+            ra.nact_out = cnt + 1 - cur;
+            ra.nslots = nb;
+            HCK(c, launch_tb_plan(ra, c->stream));
+            for (int64_t d = 0; d < D; ++d) {
+                p.spans = (const TileSpan*)c->spans.p + d * N0;
+                p.nspans = (int32_t)nb;
+                p.ntiles = nb * per_diag[(size_t)d];
+                if (tb_packed) HCK(c, launch_sweep_pk(kDirs, p, lds, c->stream));
+                else HCK(c, launch_sweep(kDirs, p, lds, c->stream));
+            }
+            HCK(c, launch_walk((const WalkArgs*)c->wargs.p, (int)nb, c->stream));
+            HCK(c, launch_tb_compact(ra, c->stream));
+            cur = 1 - cur;
+        }
+        HCK(c, hipMemcpyAsync(&walking, cnt + cur, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        HCK(c, hipStreamSynchronize(c->stream));
+        if (c->trace)
+            fprintf(stderr, "swalign: device rounds to %lld: %d pairs walking (%lld slots, %lld launches per round)\n",
+                    (long long)guard, walking, (long long)nb, (long long)(D + 3));
+        nb = walking;
+    }
+    HCK(c, hipMemcpyAsync(states.data(), c->tb_states.p, sizeof(WalkState) * (size_t)nj, hipMemcpyDeviceToHost,
+                          c->stream));
+    HCK(c, hipStreamSynchronize(c->stream));
+    std::vector<int64_t> next;
+    for (int64_t k : active) {
+        WalkState& st = states[(size_t)k];
+        if (st.done == -1) {
+            jobs[k].status = SWH_ERR_INTERNAL;  // the walk stopped moving
+            st.done = 1;
+        } else if (!st.done) {
+            next.push_back(k);
+        }
+    }
+    active.swap(next);
+    return SWH_OK;
+}
+
 int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const EngineOpts& eo, swh_result* out) {
     const int64_t nj = (int64_t)jobs.size();
     StageClock clk{c->trace};
@@ -636,8 +785,14 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
     for (int64_t k : active) max_blk_rows = std::max(max_blk_rows, jobs[k].pl.K);
     for (int64_t k : active)
         max_blocks = std::max(max_blocks, jobs[k].m / jobs[k].pl.K + jobs[k].n / jobs[k].pl.Kc + 8);
+    const int64_t guard_max = 4 * (max_blocks + (int64_t)active.size());  // rounds, at most
+    if (eo.batch && c->dirs_cap_fixed == 0) {
+        rc = device_rounds(c, jobs, active, states, p, srow, scol, Wtb, tstride, Gblk, tb_packed, tb_wps, dirs_cap,
+                           guard, guard_max);
+        if (rc) return rc;
+    }
     while (!active.empty()) {
-        if (++guard > 4 * (max_blocks + (int64_t)active.size())) return SWH_ERR_INTERNAL;
+        if (++guard > guard_max) return SWH_ERR_INTERNAL;
         // lookahead and the pairs walked this round, within the direction-code budget
         int64_t L = std::max(0, eo.lookahead);
         int64_t need = 0;
@@ -990,7 +1145,7 @@ int swh_ctx_destroy(swh_ctx* c) {
     DevBuf* bufs[] = {&c->seqA, &c->seqB, &c->rbuf, &c->cb, &c->cor, &c->rowmax, &c->srow, &c->scol,
                       &c->dirs, &c->regions, &c->key, &c->found, &c->walk, &c->ops, &c->xps, &c->wregs,
                       &c->argjobs, &c->wargs, &c->seqs, &c->spans, &c->alpha, &c->ranges, &c->rowseg,
-                      &c->segs};
+                      &c->segs, &c->tb_pairs, &c->tb_states, &c->tb_act, &c->tb_pre};
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);

@@ -35,6 +35,7 @@ struct swh_ctx {
     hipDeviceProp_t props{};
     DevBuf seqA, seqB, rbuf, cb, cor, rowmax, srow, scol, dirs, regions, key, found, walk, ops, xps, wregs,
         argjobs, wargs, seqs, spans, alpha, ranges, rowseg, segs;
+    DevBuf tb_pairs, tb_states, tb_act, tb_pre;  // traceback rounds planned on the device
     uint8_t* staging = nullptr;  // pinned host staging of batch inputs
     size_t staging_cap = 0;
     uint8_t* host_ops = nullptr;  // pinned host copy of the traceback ops

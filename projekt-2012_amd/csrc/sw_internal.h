@@ -150,4 +150,57 @@ hipError_t launch_alphabet(const ByteRange* ranges, int64_t nranges, uint32_t* m
 
 hipError_t launch_walk(const WalkArgs* jobs, int njobs, hipStream_t s);  // one workgroup per job
 
+// ---- traceback rounds planned on the device ------------------------------------------
+// While many pairs walk, each round's regions, walk arguments and the next round's list of
+// walking pairs are built by kernels from the walk states in device memory, so that rounds
+// follow each other without the host.  Round slot x (the x-th walking pair) owns fixed
+// slices of the hand-off buffers and of the direction codes.
+
+// What the round kernels need of pair k; constant over the traceback.
+struct TbPair {
+    const uint8_t* A;
+    const uint8_t* B;
+    int64_t m, n, K, Kc;
+    const int2* srow;  // the pair's checkpoint rows, or nullptr (none)
+    const int2* scol;  // the pair's checkpoint columns, or nullptr (none)
+    uint8_t* ops;
+    XPoint* xps;
+    int64_t xcap;
+};
+
+// Position of slot x's pair when its round was planned (progress check).
+struct TbPre {
+    int64_t i, j;
+    int32_t layer, _pad;
+};
+
+struct TbRoundArgs {
+    const TbPair* pairs;
+    WalkState* states;        // per pair; done = -1 marks a pair whose walk stopped moving
+    const int32_t* act_in;    // walking pairs of this round, *nact_in of them
+    const int32_t* nact_in;
+    int32_t* act_out;         // the next round's, built by the compaction
+    int32_t* nact_out;        // zeroed by the plan kernel
+    Region* regions;          // [slot]
+    WalkRegion* wregs;        // [slot]
+    WalkArgs* wargs;          // [slot]; st == nullptr for an empty slot
+    TbPre* pre;               // [slot]
+    int64_t nslots;           // slots of the launch (at least *nact_in)
+    int64_t W, G, tstride;    // tile columns, segments per stripe of the code layout, steps per tile
+    int2* cb;                 // + slot * cb_stride
+    int64_t cb_stride;
+    int2* rb;                 // + slot * rb_stride
+    int64_t rb_stride;
+    int32_t* cor;             // + slot * cor_stride
+    int64_t cor_stride;
+    uint16_t* dirs;           // + slot * dirs_stride
+    int64_t dirs_stride;
+};
+// Regions, walk arguments and progress records of one round (block of each walking pair's
+// current cell, cut at the cell); slots past *nact_in get empty regions.
+hipError_t launch_tb_plan(const TbRoundArgs& a, hipStream_t s);
+// After the round's walk: the pairs still walking go to act_out (any order), and a pair
+// whose walk did not move is marked done = -1 and dropped.
+hipError_t launch_tb_compact(const TbRoundArgs& a, hipStream_t s);
+
 }  // namespace swk

@@ -794,9 +794,14 @@ __device__ void sweep_main_pk(const SweepParams& p, SweepLds& lds) {
     const int64_t x = (int64_t)blockIdx.x * kSweepWaves + wid;
     TileSetup ts[2];
     int span = -1;
-    if (!make_tile<R, MODE>(p, 2 * x, ts[0], &span)) return;
-    if (!make_tile<R, MODE>(p, 2 * x + 1, ts[1], &span)) {
-        // no second tile: an empty one (no rows, no columns, nothing stored)
+    // Either tile may be missing (spans built on the device keep a fixed count of tiles per
+    // region, some of which a smaller region does not have); a lone second tile runs first.
+    const bool has0 = make_tile<R, MODE>(p, 2 * x, ts[0], &span);
+    const bool has1 = make_tile<R, MODE>(p, 2 * x + 1, ts[1], &span);
+    if (!has0 && !has1) return;
+    if (!has0) ts[0] = ts[1];
+    if (!has0 || !has1) {
+        // one tile: the second is an empty one (no rows, no columns, nothing stored)
         ts[1] = ts[0];
         ts[1].T.w = 0;
         ts[1].T.r1 = ts[1].T.R0 - 1;
@@ -1133,7 +1138,117 @@ template <int R>
 __global__ __launch_bounds__(kWalkThreads) void k_walk(const WalkArgs* jobs) {
     __shared__ WalkLds<R> lds;
     const WalkArgs a = jobs[blockIdx.x];
+    if (!a.st) return;  // an empty slot of a round planned on the device
     walk_main<R>(a, lds);
+}
+
+// One thread per round slot: the checkpoint block of the pair's current cell, cut at the
+// cell (rows up to the cell's, columns up to the cell's rounded up to whole tiles within the
+// block, so that the two tiles of a packed wave have equal widths; the walk never visits the
+// columns right of the cell), as the host-planned rounds build it without lookahead.
+__global__ __launch_bounds__(256) void k_tb_plan(TbRoundArgs a) {
+    const int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (x == 0) *a.nact_out = 0;
+    if (x >= a.nslots) return;
+    Region& r = a.regions[x];
+    WalkArgs& wa = a.wargs[x];
+    if (x >= *a.nact_in) {
+        Region q{};
+        q.r1 = -1;  // no rows or columns: the sweep finds no tile in it
+        q.c1 = -1;
+        q.W = a.W;
+        q.G = a.G;
+        q.tstride = a.tstride;
+        r = q;
+        wa.st = nullptr;
+        return;
+    }
+    const int32_t k = a.act_in[x];
+    const TbPair P = a.pairs[k];
+    WalkState* stp = a.states + k;
+    const int64_t i = stp->i, j = stp->j;
+    const int64_t brow = i / P.K, bcol = j / P.Kc;
+    const int64_t r0 = brow * P.K, c0 = bcol * P.Kc;
+    const int64_t c_end = min(P.n, c0 + P.Kc) - 1;
+    const int64_t cols = j - c0 + 1;
+    uint16_t* dirs = a.dirs + x * a.dirs_stride;
+    Region q{};
+    q.A = P.A;
+    q.B = P.B;
+    q.m = P.m;
+    q.n = P.n;
+    q.r0 = r0;
+    q.r1 = i;
+    q.c0 = c0;
+    q.c1 = min(c_end, c0 + (cols + a.W - 1) / a.W * a.W - 1);
+    q.W = a.W;
+    q.G = a.G;
+    q.tstride = a.tstride;
+    q.top = r0 > 0 && P.srow ? P.srow + (r0 / P.K - 1) * P.n + c0 : nullptr;
+    q.left = c0 > 0 && P.scol ? P.scol + (c0 / P.Kc - 1) * P.m : nullptr;
+    q.rb = a.rb + x * a.rb_stride;
+    q.cb = a.cb + x * a.cb_stride;
+    q.cor = a.cor + x * a.cor_stride;
+    q.K = P.K;
+    q.Kc = P.Kc;
+    q.dirs = dirs;
+    r = q;
+    WalkRegion w;
+    w.r0 = r0;
+    w.c0 = c0;
+    w.rows = i - r0 + 1;
+    w.cols = cols;
+    w.dirs = dirs;
+    a.wregs[x] = w;
+    WalkArgs g{};
+    g.A = P.A;
+    g.B = P.B;
+    g.regs = a.wregs + x;
+    g.L = 0;
+    g.brow = brow;
+    g.bcol = bcol;
+    g.tstride = a.tstride;
+    g.G = a.G;
+    g.W = a.W;
+    g.K = P.K;
+    g.Kc = P.Kc;
+    g.srow = P.srow;
+    g.n = P.n;
+    g.st = stp;
+    g.ops = P.ops;
+    g.xps = P.xps;
+    g.xcap = P.xcap;
+    wa = g;
+    TbPre pv;
+    pv.i = i;
+    pv.j = j;
+    pv.layer = stp->layer;
+    pv._pad = 0;
+    a.pre[x] = pv;
+}
+
+// One thread per round slot, after the walk: slots whose pair still walks append it to
+// act_out, one atomic per wave.
+__global__ __launch_bounds__(256) void k_tb_compact(TbRoundArgs a) {
+    const int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    bool keep = false;
+    int32_t k = -1;
+    if (x < a.nslots && x < *a.nact_in) {
+        k = a.act_in[x];
+        WalkState* st = a.states + k;
+        if (!st->done) {
+            const TbPre pv = a.pre[x];
+            if (st->i == pv.i && st->j == pv.j && st->layer == pv.layer) st->done = -1;  // no progress
+            else keep = true;
+        }
+    }
+    const uint64_t b = __ballot(keep);
+    if (!b) return;
+    const int lane = (int)(threadIdx.x & 63u);
+    int base = 0;
+    if (lane == 0) base = atomicAdd(a.nact_out, (int32_t)__popcll(b));
+    base = __shfl(base, 0);
+    if (keep) a.act_out[base + __popcll(b & ((UINT64_C(1) << lane) - 1))] = k;
 }
 
 }  // namespace
@@ -1212,6 +1327,18 @@ hipError_t launch_alphabet(const ByteRange* ranges, int64_t nranges, uint32_t* m
 hipError_t launch_walk(const WalkArgs* jobs, int njobs, hipStream_t s) {
     if (njobs <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_walk<kRowsPerLane>, dim3((unsigned)njobs), dim3(kWalkThreads), 0, s, jobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_tb_plan(const TbRoundArgs& a, hipStream_t s) {
+    if (a.nslots <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tb_plan, dim3((unsigned)((a.nslots + 255) / 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_tb_compact(const TbRoundArgs& a, hipStream_t s) {
+    if (a.nslots <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tb_compact, dim3((unsigned)((a.nslots + 255) / 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

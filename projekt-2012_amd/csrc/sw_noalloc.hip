@@ -174,6 +174,7 @@ extern "C" int swh_align_pair_device_noalloc(swh_ctx* c, const uint8_t* d1, int6
     DevBuf none{};
     none.fixed = true;  // capacity 0: any use fails instead of allocating
     lc.seqA = lc.seqB = lc.seqs = none;
+    lc.tb_pairs = lc.tb_states = lc.tb_act = lc.tb_pre = none;  // batches only
     slice(lc.rowmax, y.rowmax);
     slice(lc.rowseg, y.rowseg);
     slice(lc.segs, y.segs);

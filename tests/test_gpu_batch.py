@@ -162,3 +162,63 @@ def test_batch_packed_high_scores_equal_int32(sw, ctx):
         one = ctx.align(a, b, sw.scoring(1, -3, -5, -2), xpoints=True, K=1024, Kc=1024, precision=32)
         for k in ("score", "end", "start", "cigar", "xpoints"):
             assert r[k] == one[k], (k, len(a), len(b))
+
+
+def _traced_batch(sw, monkeypatch, capfd, pairs, **kw):
+    """Runs a batch on a fresh context that traces its stages; returns the results and
+    whether traceback rounds were planned on the device."""
+    monkeypatch.setenv("SWALIGN_TRACE", "1")
+    with sw.Context(0) as c:
+        capfd.readouterr()
+        res, st = c.batch(pairs, sw.scoring(*SC), **kw)
+        err = capfd.readouterr().err
+    return res, st, "device rounds" in err
+
+
+def test_batch_device_planned_rounds(sw, orc, monkeypatch, capfd):
+    """2600 pairs of 150-1200 bp at 256-row checkpoint blocks: enough walking pairs that the
+    traceback rounds are planned, swept, walked and compacted on the device until the rest
+    go to host-planned rounds with lookahead; every pair gets the oracle's alignment and
+    crosspoints."""
+    rng = random.Random(29)
+    pairs = []
+    for k in range(2600):
+        L = rng.randint(150, 1200)
+        a, b = sw.synth_pair(7000 + k, L, L + rng.randint(-L // 5, L // 5), max(1, L // 2), 0.1, 0.02)
+        pairs.append((a, b) if rng.random() < 0.5 else (b, a))
+    res, st, dev = _traced_batch(sw, monkeypatch, capfd, pairs, xpoints=True, K=256, Kc=256)
+    assert dev
+    _check(orc, pairs, res, st, K=256)
+
+
+def test_batch_device_rounds_equal_host_rounds(sw, orc, ctx, monkeypatch, capfd):
+    """Blocks of two stripes by four tiles (512 x 1024, 256-column tiles: two tiles of one
+    pair on some launches): a 2400-pair batch whose rounds start on the device gives exactly
+    what the same pairs give in batches of 400, whose rounds are all planned on the host,
+    crosspoints included; every CIGAR re-scores to its score, and a sample equals the
+    oracle's alignment."""
+    rng = random.Random(31)
+    pairs = []
+    for k in range(2400):
+        L = rng.randint(1300, 2600)
+        a, b = sw.synth_pair(9000 + k, L, L + rng.randint(-L // 6, L // 6), L // 2, 0.1, 0.02)
+        pairs.append((a, b))
+    kw = dict(xpoints=True, K=512, Kc=1024, tile_cols=256)
+    res, st, dev = _traced_batch(sw, monkeypatch, capfd, pairs, **kw)
+    assert dev and all(s == 0 for s in st)
+    for q in range(0, len(pairs), 400):
+        part, pst = ctx.batch(pairs[q:q + 400], sw.scoring(*SC), **kw)
+        assert all(s == 0 for s in pst)
+        for x, r in enumerate(part):
+            g = res[q + x]
+            for k in ("score", "end", "start", "cigar", "xpoints", "swapped"):
+                assert g[k] == r[k], (k, q + x)
+    osc = orc.scoring(*SC)
+    for (a, b), r in zip(pairs, res):
+        A, B = orient(a, b)
+        assert orc.rescore(A, B, osc, r["start"], r["cigar"]) == (r["score"], r["end"])
+    for q in range(0, len(pairs), 97):
+        A, B = orient(*pairs[q])
+        ref = orc.align_linear(A, B, osc, 512, 1024)
+        for k in ("score", "end", "start", "cigar"):
+            assert res[q][k] == ref[k], (k, q)
