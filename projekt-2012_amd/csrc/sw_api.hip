@@ -119,6 +119,11 @@ double sweep_model(swh_ctx* c, int64_t rows, int64_t cols, int wps) {
 
 int usable_wps(const swh_ctx* c, int w) { return w >= 1 && w <= kMaxSweepWps && c->lds_pad[w] ? w : 1; }
 
+bool many_walks(const swh_ctx* c, int64_t njobs) {
+    const int64_t per_cu = (int64_t)c->props.maxSharedMemoryPerMultiProcessor / (int64_t)walk_lds_bytes();
+    return njobs > std::max<int64_t>(1, per_cu) * std::max(1, c->props.multiProcessorCount);
+}
+
 // Sweep waves per SIMD for a rows x cols sweep: whichever of one or two the model prefers.
 int choose_wps(swh_ctx* c, int64_t rows, int64_t cols) {
     if (c->wps_fixed) return c->wps_fixed;
@@ -484,7 +489,7 @@ int device_rounds(swh_ctx* c, std::vector<Job>& jobs, std::vector<int64_t>& acti
                 if (tb_packed) HCK(c, launch_sweep_pk(kDirs, p, lds, c->stream));
                 else HCK(c, launch_sweep(kDirs, p, lds, c->stream));
             }
-            HCK(c, launch_walk((const WalkArgs*)c->wargs.p, (int)nb, c->stream));
+            HCK(c, launch_walk((const WalkArgs*)c->wargs.p, (int)nb, many_walks(c, nb), c->stream));
             HCK(c, launch_tb_compact(ra, c->stream));
             cur = 1 - cur;
         }
@@ -924,7 +929,7 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
                               c->stream));
         HCK(c, hipMemcpyAsync(c->wargs.p, wargs.data(), sizeof(WalkArgs) * wargs.size(), hipMemcpyHostToDevice,
                               c->stream));
-        HCK(c, launch_walk((const WalkArgs*)c->wargs.p, (int)nr, c->stream));
+        HCK(c, launch_walk((const WalkArgs*)c->wargs.p, (int)nr, many_walks(c, nr), c->stream));
         std::vector<WalkState> after((size_t)nr);
         HCK(c, hipMemcpyAsync(after.data(), c->walk.p, sizeof(WalkState) * nr, hipMemcpyDeviceToHost, c->stream));
         HCK(c, hipStreamSynchronize(c->stream));

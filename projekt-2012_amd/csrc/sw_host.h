@@ -83,6 +83,8 @@ int64_t tile_cols(swh_ctx* c, int64_t cols, int64_t want, int wps = 1);
 int choose_wps(swh_ctx* c, int64_t rows, int64_t cols);
 // w if the context can hold the sweep to w waves per SIMD (1 <= w <= kMaxSweepWps), else 1.
 int usable_wps(const swh_ctx* c, int w);
+// True if njobs walks exceed what the device's LDS holds at once with large walk windows.
+bool many_walks(const swh_ctx* c, int64_t njobs);
 
 // One sweep over `regions`, one launch per tile anti-diagonal (see sw_api.hip).
 int run_sweep(swh_ctx* c, int mode, swk::SweepParams p, std::vector<swk::Region>& regions, int64_t W,

@@ -148,7 +148,10 @@ struct ByteRange {
 // ORs into mask[8] (zeroed by the caller) bit x for every byte x present in the ranges.
 hipError_t launch_alphabet(const ByteRange* ranges, int64_t nranges, uint32_t* mask, hipStream_t s);
 
-hipError_t launch_walk(const WalkArgs* jobs, int njobs, hipStream_t s);  // one workgroup per job
+// One workgroup per job.  many: more walks than the device holds at once with the large
+// window (walk_lds_bytes() of LDS each), which then run with small windows.
+hipError_t launch_walk(const WalkArgs* jobs, int njobs, bool many, hipStream_t s);
+size_t walk_lds_bytes();
 
 // ---- traceback rounds planned on the device ------------------------------------------
 // While many pairs walk, each round's regions, walk arguments and the next round's list of

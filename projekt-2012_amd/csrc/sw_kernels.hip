@@ -891,17 +891,24 @@ __global__ __launch_bounds__(256) void k_end_segment(const ArgmaxJob* jobs, int 
 // by all kWalkThreads threads with every load in flight at once (the codes were written
 // by the previous launch and are read from HBM); wave 0 walks.  The walk stops at the start of the alignment or at a cell outside every
 // region of the round.
+// Two window sizes: kWalkSteps for few walks at a time (each walk's latency counts: fewer,
+// larger windows), kWalkStepsMany when more walks are in flight than the CUs' LDS can host
+// at that size (smaller windows let more walks share a CU; many walks at once, 65,536-pair
+// batch with 512-row blocks: 64 steps 147 ms, 128 152 ms, 256 164 ms of traceback).
 constexpr int kWalkSteps = 256;
+constexpr int kWalkStepsMany = 64;
 constexpr int kWalkThreads = 256;
-constexpr int kWalkLoads = kWalkSteps * 64 * 2 / 16 / kWalkThreads;  // uint4 loads per thread per window
-// Ops of one window: the path stays in the window's 64R rows and kWalkSteps + 63 columns.
-constexpr int kWalkOps = 64 * kRowsPerLane + kWalkSteps + 64;
+template <int WS>
+constexpr int walk_loads() { return WS * 64 * 2 / 16 / kWalkThreads; }  // uint4 loads per thread per window
+// Ops of one window: the path stays in the window's 64R rows and WS + 63 columns.
+template <int WS>
+constexpr int walk_ops() { return 64 * kRowsPerLane + WS + 64; }
 
-template <int R>
+template <int R, int WS>
 struct WalkLds {
-    uint16_t win[kWalkSteps * 64];                 // steps [t0, t0 + kWalkSteps) x 64 lanes
-    uint8_t a[64 * R], b[kWalkSteps + 64];         // rows of the stripe, columns of the band
-    uint8_t ops[kWalkOps];                         // ops of the window, flushed when it ends
+    uint16_t win[WS * 64];                         // steps [t0, t0 + WS) x 64 lanes
+    uint8_t a[64 * R], b[WS + 64];                 // rows of the stripe, columns of the band
+    uint8_t ops[walk_ops<WS>()];                   // ops of the window, flushed when it ends
     XPoint xp;                                     // crosspoint of the window (at most one)
     int32_t nw, nxw;                               // ops and crosspoints of the window
     int64_t ops_at;                                // where the window's ops go in a.ops
@@ -926,8 +933,8 @@ __device__ __forceinline__ const WalkRegion* walk_region(const WalkArgs& a, int6
     return r;
 }
 
-template <int R>
-__device__ void walk_main(const WalkArgs& a, WalkLds<R>& lds) {
+template <int R, int WS>
+__device__ void walk_main(const WalkArgs& a, WalkLds<R, WS>& lds) {
     constexpr int HR = 64 * R;
     WalkState& sst = lds.sst;
     const int tid = threadIdx.x;
@@ -949,22 +956,23 @@ __device__ void walk_main(const WalkArgs& a, WalkLds<R>& lds) {
         const int64_t R0 = uni64(rgn.r0 + s * HR), C0 = uni64(rgn.c0 + sg * a.W);  // tile origin (absolute)
         const int ri0 = (int)(st.i - R0), cj0 = (int)(st.j - C0);
         const int t1 = cj0 + ri0 / R;
-        const int t0 = t1 >= kWalkSteps ? t1 - kWalkSteps + 1 : 0;
+        const int t0 = t1 >= WS ? t1 - WS + 1 : 0;
         const uint16_t* tile = rgn.dirs + (s * a.G + sg) * a.tstride * 64;
         {
             // steps [t0, t1] are contiguous: (t1 - t0 + 1) * 64 codes, at most
-            // kWalkLoads * kWalkThreads uint4; all loads are issued before the first store
+            // walk_loads<WS>() * kWalkThreads uint4; all loads are issued before the first store
             const uint4* src = (const uint4*)(tile + (int64_t)t0 * 64);
             uint4* dst = (uint4*)lds.win;
             const int n16 = (t1 - t0 + 1) * 64 * 2 / 16;
-            uint4 v[kWalkLoads];
+            constexpr int kLoads = walk_loads<WS>();
+            uint4 v[kLoads];
 #pragma unroll
-            for (int q = 0; q < kWalkLoads; ++q) {
+            for (int q = 0; q < kLoads; ++q) {
                 const int k = tid + q * kWalkThreads;
                 if (k < n16) v[q] = src[k];
             }
 #pragma unroll
-            for (int q = 0; q < kWalkLoads; ++q) {
+            for (int q = 0; q < kLoads; ++q) {
                 const int k = tid + q * kWalkThreads;
                 if (k < n16) dst[k] = v[q];
             }
@@ -975,7 +983,7 @@ __device__ void walk_main(const WalkArgs& a, WalkLds<R>& lds) {
         // walk only moves left, so columns right of the current one (cj0; C0 + cj0 < n) are
         // never read and not loaded: B may end at the current cell's column.
         const int cj_hi = min(t1, cj0);
-        for (int k = tid; k < kWalkSteps + 64; k += kWalkThreads) {
+        for (int k = tid; k < WS + 64; k += kWalkThreads) {
             const int cj = t0 - 63 + k;
             lds.b[k] = (cj >= 0 && cj <= cj_hi) ? a.B[C0 + cj] : 0;
         }
@@ -986,7 +994,7 @@ __device__ void walk_main(const WalkArgs& a, WalkLds<R>& lds) {
             // with readfirstlane, so the loop's control flow is scalar.  Lane 0 alone writes
             // the ops and the crosspoint, to LDS; they are flushed after the window, so the
             // loop issues no global memory operation.
-            static_assert(64 * R + kWalkSteps + 64 <= kWalkOps, "window ops buffer");
+            static_assert(64 * R + WS + 64 <= walk_ops<WS>(), "window ops buffer");
             int ri = uni(ri0), cj = uni(cj0);  // tile-local row and column
             int layer = uni(st.layer);
             int nw = 0, nxw = 0, done = 0;
@@ -1134,12 +1142,12 @@ __device__ void walk_main(const WalkArgs& a, WalkLds<R>& lds) {
     if (tid == 0) *a.st = sst;
 }
 
-template <int R>
+template <int R, int WS>
 __global__ __launch_bounds__(kWalkThreads) void k_walk(const WalkArgs* jobs) {
-    __shared__ WalkLds<R> lds;
+    __shared__ WalkLds<R, WS> lds;
     const WalkArgs a = jobs[blockIdx.x];
     if (!a.st) return;  // an empty slot of a round planned on the device
-    walk_main<R>(a, lds);
+    walk_main<R, WS>(a, lds);
 }
 
 // One thread per round slot: the checkpoint block of the pair's current cell, cut at the
@@ -1283,6 +1291,7 @@ hipError_t launch_sweep_pk(int mode, const SweepParams& p, size_t lds_pad, hipSt
 }
 
 size_t sweep_static_lds() { return sizeof(SweepLds) * kSweepWaves; }
+size_t walk_lds_bytes() { return sizeof(WalkLds<kRowsPerLane, kWalkSteps>); }
 
 hipError_t sweep_allow_lds(size_t dyn_bytes) {
     constexpr int R = kRowsPerLane;
@@ -1324,9 +1333,13 @@ hipError_t launch_alphabet(const ByteRange* ranges, int64_t nranges, uint32_t* m
     return hipGetLastError();
 }
 
-hipError_t launch_walk(const WalkArgs* jobs, int njobs, hipStream_t s) {
+hipError_t launch_walk(const WalkArgs* jobs, int njobs, bool many, hipStream_t s) {
     if (njobs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_walk<kRowsPerLane>, dim3((unsigned)njobs), dim3(kWalkThreads), 0, s, jobs);
+    if (many)
+        hipLaunchKernelGGL((k_walk<kRowsPerLane, kWalkStepsMany>), dim3((unsigned)njobs), dim3(kWalkThreads), 0, s,
+                           jobs);
+    else
+        hipLaunchKernelGGL((k_walk<kRowsPerLane, kWalkSteps>), dim3((unsigned)njobs), dim3(kWalkThreads), 0, s, jobs);
     return hipGetLastError();
 }
 

@@ -635,7 +635,7 @@ int tick(swh_server* s) {
                               c->stream));
         HCK(c, hipMemcpyAsync(s->wargs.p, wa.data(), sizeof(WalkArgs) * wa.size(), hipMemcpyHostToDevice,
                               c->stream));
-        HCK(c, launch_walk((const WalkArgs*)s->wargs.p, (int)wa.size(), c->stream));
+        HCK(c, launch_walk((const WalkArgs*)s->wargs.p, (int)wa.size(), many_walks(c, (int64_t)wa.size()), c->stream));
     }
     // ---- read back what decides each job's next step (only when some job needs it) -------
     bool readback = false;
