@@ -468,6 +468,7 @@ int device_rounds(swh_ctx* c, std::vector<Job>& jobs, std::vector<int64_t>& acti
     ra.cor_stride = 2 * Sb;
     ra.dirs = (uint16_t*)c->dirs.p;
     ra.dirs_stride = slot_dirs;
+    ra.sc = p.sc;
     p.regions = (const Region*)c->regions.p;
     const size_t lds = c->lds_pad[usable_wps(c, tb_wps)];
     int cur = 0;
@@ -765,6 +766,7 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
         st.start_i = st.i;
         st.start_j = st.j;
         st.last_row = st.i;
+        st.v = jobs[k].best;
     }
     int64_t wmax = 0;
     for (int64_t k : active) wmax = std::max(wmax, std::min(jobs[k].n, jobs[k].pl.Kc));
@@ -920,6 +922,7 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
             wa.ops = (uint8_t*)c->ops.p + j.ops_off;
             wa.xps = (XPoint*)c->xps.p + j.xps_off;
             wa.xcap = j.xcap;
+            wa.sc = p.sc;
         }
         // the round's walk states, compacted
         std::vector<WalkState> rst((size_t)nr);

@@ -89,6 +89,8 @@ struct WalkState {
     int64_t last_row;          // last row visited (crosspoint detection)
     int32_t layer;             // 0 = H, 1 = E, 2 = F
     int32_t done;              // 1 when the start of the alignment was reached
+    int32_t v;                 // value of the current layer's matrix at (i, j) (the score at the end cell)
+    int32_t _pad;
 };
 
 struct XPoint {
@@ -119,6 +121,7 @@ struct WalkArgs {
     uint8_t* ops;
     XPoint* xps;
     int64_t xcap;
+    Score sc;                    // the walk follows the values of the cells it visits
 };
 
 // Launchers (sw_kernels.hip).
@@ -198,6 +201,7 @@ struct TbRoundArgs {
     int64_t cor_stride;
     uint16_t* dirs;           // + slot * dirs_stride
     int64_t dirs_stride;
+    Score sc;
 };
 // Regions, walk arguments and progress records of one round (block of each walking pair's
 // current cell, cut at the cell); slots past *nact_in get empty regions.

@@ -229,8 +229,11 @@ __device__ __forceinline__ void chunk_steps(const SweepParams& p, const TileCtx&
             const int F = max(Fe_up, Ho_up);
             const int H = max(max(Hd, 0), max(En, F));
             if (MODE == kDirs) {
-                const uint32_t h = (H == 0) ? 0u : (H == Hd ? 1u : (H == F ? 2u : 3u));
-                nib |= (h | ((uint32_t)(Ee > Ho[k]) << 2) | ((uint32_t)(Fe_up > Ho_up) << 3)) << (4 * k);
+                // the cell's 4-bit code: bit 0 H != H(i-1,j-1) + s (not diagonal), bit 1 H != F,
+                // bit 2 E extends, bit 3 F extends (the walk knows where H is 0 from its values)
+                nib |= ((uint32_t)(H != Hd) | ((uint32_t)(H != F) << 1) | ((uint32_t)(Ee > Ho[k]) << 2) |
+                        ((uint32_t)(Fe_up > Ho_up) << 3))
+                       << (4 * k);
             }
             const int Hon = H + O;
             if (EDGE) {
@@ -575,19 +578,17 @@ __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileC
             const s16x2 F = pk_max(Fe_up, Ho_up);
             const s16x2 H = pk_max(pk_max(Hd, zero), pk_max(En, F));
             if (MODE == kDirs) {
-                // the 4-bit code of each half, as k_sweep's: h = 0 (H at the zero floor), 1
-                // (diagonal), 2 (F), 3 (E); bit 2: E extends; bit 3: F extends.  Differences
-                // H - x are >= 0, so min(., 1) is the half's "differs" flag; the sign bit of
-                // Ho - Ee is "E extends".  Halves never carry into each other (values <= 15).
-                // The sign bits go straight to their nibble positions (one shift and one
-                // and-or each) and h is shifted in with the or.
-                const uint32_t nz = differs(H - zero), nd = differs(H - Hd), nf = differs(H - F);
-                const uint32_t a1 = nz & nd;
-                const uint32_t h = nz + a1 + (a1 & nf);
+                // the 4-bit code of each half, as k_sweep's: bit 0 H != Hd, bit 1 H != F, bit
+                // 2 E extends, bit 3 F extends.  Differences H - x are >= 0, so min(., 1) is
+                // the half's "differs" flag; the sign bit of Ho - Ee is "E extends" (all values
+                // of a tile lie within 32000 of each other).  Halves never carry into each
+                // other (values <= 15).  The sign bits go straight to their nibble positions
+                // (one shift and one and-or each).
+                const uint32_t nd = differs(H - Hd), nf = differs(H - F);
                 const uint32_t eb = 0x00010001u << (4 * k + 2), fb = 0x00010001u << (4 * k + 3);
                 nib = (((uint32_t)pk_bits(Ho[k] - Ee) >> (13 - 4 * k)) & eb) | nib;
                 nib = (((uint32_t)pk_bits(Ho_up - Fe_up) >> (12 - 4 * k)) & fb) | nib;
-                nib = (h << (4 * k)) | nib;
+                nib = ((nd | (nf << 1)) << (4 * k)) | nib;
             }
             const s16x2 Hon = H + vO;
             if (EDGE) {
@@ -997,6 +998,8 @@ __device__ void walk_main(const WalkArgs& a, WalkLds<R, WS>& lds) {
             static_assert(64 * R + WS + 64 <= walk_ops<WS>(), "window ops buffer");
             int ri = uni(ri0), cj = uni(cj0);  // tile-local row and column
             int layer = uni(st.layer);
+            int v = uni(st.v);  // value of the current layer's matrix at the current cell
+            const int sM = a.sc.M, sX = a.sc.X, sO = a.sc.O, sE = a.sc.E;
             int nw = 0, nxw = 0, done = 0;
             int64_t last_row = uni64(st.last_row);
             int64_t si = uni64(st.start_i), sj = uni64(st.start_j);
@@ -1010,21 +1013,30 @@ __device__ void walk_main(const WalkArgs& a, WalkLds<R, WS>& lds) {
                 const int l = ri / R, t = cj + l;
                 if (t < t0) break;  // left the window
                 if (layer == 0) {
-                    // Diagonal run: lane k reads the code of cell (ri - k, cj - k); the moves
-                    // up to the first lane whose cell is not a diagonal move (or lies outside
-                    // the tile or the window) are taken at once, one op per lane.
+                    // Diagonal run: lane k reads the code of cell (ri - k, cj - k) and its H,
+                    // the current value less the scores of the cells before it (matches among
+                    // the lanes below k: mbcnt of the ballot); the moves up to the first lane
+                    // whose cell is not a diagonal move, holds H = 0 (the alignment starts
+                    // after it) or lies outside the tile or the window are taken at once, one
+                    // op per lane.
                     const int rik = ri - lane, cjk = cj - lane;
                     const int lk = rik >= 0 ? rik / R : 0, tk = cjk + lk;
                     const bool in = rik >= 0 && cjk >= 0 && tk >= t0;
-                    uint32_t ck = 0;
+                    uint32_t ck = 1;
                     bool eqk = false;
                     if (in) {
-                        ck = (lds.win[(tk - t0) * 64 + lk] >> (4 * (rik % R))) & 3u;
+                        ck = (lds.win[(tk - t0) * 64 + lk] >> (4 * (rik % R))) & 1u;
                         eqk = lds.a[rik] == lds.b[cjk - bofs];
                     }
-                    const uint64_t stop = __ballot(!(in && ck == 1u));
+                    const uint64_t eqm = __ballot(eqk);
+                    const int mk = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(eqm >> 32),
+                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)eqm, 0u));
+                    const int vk = v - mk * sM - (lane - mk) * sX;
+                    const uint64_t stop = __ballot(!(in && ck == 0u && vk != 0));
                     const int run = stop ? (int)__builtin_ctzll(stop) : 64;
                     if (run > 0) {
+                        const int mr = __popcll(run == 64 ? eqm : (eqm & ((UINT64_C(1) << run) - 1)));
+                        v -= mr * sM + (run - mr) * sX;
                         if (lane < run) lds.ops[nw + lane] = eqk ? (uint8_t)'=' : (uint8_t)'X';
                         // checkpoint-row bookkeeping of the rows i, i-1, ..., i-run+1, as the
                         // single steps below would do it (xrow <= i holds after row i)
@@ -1063,7 +1075,7 @@ __device__ void walk_main(const WalkArgs& a, WalkLds<R, WS>& lds) {
                 }
                 const uint32_t word = lds.win[(t - t0) * 64 + l];
                 const uint32_t eq = lds.a[ri] == lds.b[cj - bofs] ? 16u : 0u;
-                const uint32_t v = __builtin_amdgcn_readfirstlane(((word >> (4 * (ri % R))) & 15u) | eq);
+                const uint32_t cv = __builtin_amdgcn_readfirstlane(((word >> (4 * (ri % R))) & 15u) | eq);
                 const int64_t i = R0 + ri;
                 if (i != last_row) {
                     last_row = i;
@@ -1079,30 +1091,43 @@ __device__ void walk_main(const WalkArgs& a, WalkLds<R, WS>& lds) {
                 }
                 uint32_t op;
                 if (layer == 0) {
-                    const uint32_t h = v & 3u;
-                    if (h == 1u) {
-                        op = (v & 16u) ? (uint32_t)'=' : (uint32_t)'X';
+                    if (v == 0) {  // H = 0: the alignment starts at the last diagonal move
+                        done = 1;
+                        break;
+                    }
+                    if (!(cv & 1u)) {  // diagonal: H(i-1, j-1) = H - s(i, j)
+                        const bool m = (cv & 16u) != 0;
+                        op = m ? (uint32_t)'=' : (uint32_t)'X';
+                        v -= m ? sM : sX;
                         si = i;
                         sj = C0 + cj;
                         --ri;
                         --cj;
-                    } else if (h == 2u) {
+                    } else if (!(cv & 2u)) {  // H = F(i, j)
                         layer = 2;
                         continue;
-                    } else if (h == 3u) {
+                    } else {  // H = E(i, j)
                         layer = 1;
                         continue;
-                    } else {
-                        done = 1;
-                        break;
                     }
                 } else if (layer == 2) {
+                    // F(i, j) = F(i-1, j) + gap_extend, or H(i-1, j) + gap_open
                     op = 'D';
-                    layer = (v >> 3) & 1u ? 2 : 0;
+                    if ((cv >> 3) & 1u) {
+                        v -= sE;
+                    } else {
+                        v -= sO;
+                        layer = 0;
+                    }
                     --ri;
                 } else {
                     op = 'I';
-                    layer = (v >> 2) & 1u ? 1 : 0;
+                    if ((cv >> 2) & 1u) {
+                        v -= sE;
+                    } else {
+                        v -= sO;
+                        layer = 0;
+                    }
                     --cj;
                 }
                 if (lane == 0) lds.ops[nw] = (uint8_t)op;
@@ -1115,6 +1140,7 @@ __device__ void walk_main(const WalkArgs& a, WalkLds<R, WS>& lds) {
                 st.i = R0 + ri;
                 st.j = C0 + cj;
                 st.layer = layer;
+                st.v = v;
                 st.nops += nw;
                 st.last_row = last_row;
                 st.start_i = si;
@@ -1226,6 +1252,7 @@ __global__ __launch_bounds__(256) void k_tb_plan(TbRoundArgs a) {
     g.ops = P.ops;
     g.xps = P.xps;
     g.xcap = P.xcap;
+    g.sc = a.sc;
     wa = g;
     TbPre pv;
     pv.i = i;

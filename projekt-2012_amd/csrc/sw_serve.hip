@@ -628,6 +628,7 @@ int tick(swh_server* s) {
             a.ops = j->ops;
             a.xps = j->xps;
             a.xcap = j->xcap;
+            a.sc = Score{s->sc.match, s->sc.mismatch, s->sc.gap_open, s->sc.gap_extend};
             wa.push_back(a);
         }
         HCK(c, ensure(s->wargs, sizeof(WalkArgs) * wa.size()));
@@ -692,6 +693,7 @@ int tick(swh_server* s) {
                     j->st.start_i = j->bi;
                     j->st.start_j = j->bj;
                     j->st.last_row = j->bi;
+                    j->st.v = j->best;
                     queue_value(s, j->walk, &j->st, sizeof(WalkState));
                     setup_dirs(s, *j);
                 } else {
