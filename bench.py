@@ -261,7 +261,7 @@ def batch_record(args, torch, sw, pairs, sc, steps, warmup, cpu=True):
         tuning = ctx.batch_tune(dp, sc, sample=len(dp) // 8)
         tuning["tune_s"] = round(time.perf_counter() - t0, 2)
         kw = dict(tile_cols=tuning["tile_cols"], waves_per_simd=tuning["waves_per_simd"],
-                  K=tuning["checkpoint_rows"], Kc=tuning["checkpoint_rows"])
+                  K=tuning["checkpoint_rows"], Kc=tuning["checkpoint_cols"])
         log("batch autotune %s" % tuning)
     for k in range(warmup):
         ctx.batch_device(dp, sc, **kw)

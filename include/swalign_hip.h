@@ -188,7 +188,8 @@ typedef struct {
  * anti-diagonal, so short pairs fill the device while long ones proceed.  out[npairs]
  * receives one result each (free each with swh_result_free); per-pair errors are
  * reported in statuses[npairs] (nullable) without failing the batch.  Checkpoints are
- * spaced alike for the whole batch (from 1024 rows, doubled to fit the budget); stage
+ * spaced alike for the whole batch (from blocks of 256 rows by 512 columns, doubled to fit
+ * the budget); stage
  * times in the results are those of the whole batch.  No reference counterpart (the
  * reference aligns one pair per process, src/main.cu:102-250). */
 int swh_batch_align(swh_ctx* ctx, const swh_pair* pairs, int64_t npairs, const swh_scoring* sc,
@@ -200,7 +201,8 @@ int swh_batch_align_device(swh_ctx* ctx, const swh_pair* pairs, int64_t npairs, 
                            const swh_opts* opts, swh_result* out, int32_t* statuses);
 
 /* Batch autotuner: times every combination of tile width {256, 384, 512, 768, 1024},
- * sweep waves per SIMD {2, 3, 4} and checkpoint spacing {256, 512, 1024, 2048} on an evenly spaced
+ * sweep waves per SIMD {2, 3, 4} and checkpoint blocks (rows x columns) {256 x 512,
+ * 512 x 512, 1024 x 1024, 2048 x 2048} on an evenly spaced
  * sample of `sample` pairs (device-resident pairs if `device` != 0) and reports the
  * fastest, to be passed back in swh_opts (tile_cols, waves_per_simd, checkpoint_rows and
  * checkpoint_cols).  Spacings whose checkpoints for the whole batch exceed hbm_budget_bytes
@@ -212,6 +214,7 @@ typedef struct {
     int64_t checkpoint_rows;
     double ms_best, ms_default;   /* sample time of the best and of the default settings */
     int32_t candidates, _pad;
+    int64_t checkpoint_cols;
 } swh_batch_tuning;
 int swh_batch_tune(swh_ctx* ctx, const swh_pair* pairs, int64_t npairs, int32_t device, const swh_scoring* sc,
                    const swh_opts* opts, int64_t sample, swh_batch_tuning* out);

@@ -345,6 +345,11 @@ int64_t default_budget(const swh_ctx* c, const swh_opts* o) {
 // Columns per tile of a batch sweep: wide enough that the 63 fill steps of a tile cost
 // about a tenth, narrow enough that short pairs still cut into several tiles.
 constexpr int64_t kBatchTileCols = 512;
+// Checkpoint blocks of a batch, rows by columns: the traceback recomputes about the block
+// height per cell of the path while stage 1 writes a checkpoint row per stripe, and a column
+// spacing that is a multiple of the tile width costs stage 1 nothing per step (65,536-pair
+// batch, W = 512: 256 x 512 723 ms a batch, 512 x 512 744, 256 x 256 751, 1024 x 1024 892).
+constexpr int64_t kBatchCkptRows = 256, kBatchCkptCols = 512;
 // Sweep waves per SIMD of a batch: a batch has tiles to spare, and four waves per SIMD hide
 // the per-tile set-up and memory waits that one or two leave exposed (65,536-pair batch,
 // forward sweep: 1210 / 763 / 668 / 650 ms at 1 / 2 / 3 / 4 waves per SIMD).
@@ -1259,7 +1264,8 @@ int tune_jobs(swh_ctx* c, const std::vector<Job>& jobs, const swh_scoring* sc, c
               swh_batch_tuning* out);
 
 // Shared by the batch entry points: jobs over device-resident pairs, one checkpoint
-// spacing for the batch (from 1024 rows, or the options, doubled to fit the budget); with
+// spacing for the batch (kBatchCkptRows x kBatchCkptCols, or the options, doubled to fit
+// the budget); with
 // opts->autotune the settings are first chosen by timing candidates on a sample.
 int batch_run(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const swh_opts* o, swh_result* out,
               int32_t* statuses) {
@@ -1273,11 +1279,13 @@ int batch_run(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const s
         ob.tile_cols = t.tile_cols;
         ob.waves_per_simd = t.waves_per_simd;
         ob.checkpoint_rows = t.checkpoint_rows;
-        ob.checkpoint_cols = t.checkpoint_rows;
+        ob.checkpoint_cols = t.checkpoint_cols;
     }
     if (!ob.score_only) {
-        int64_t K = ob.checkpoint_rows > 0 ? ceil_div(ob.checkpoint_rows, kStripeRows) * kStripeRows : 1024;
-        int64_t Kc = ob.checkpoint_cols > 0 ? next_pow2(ob.checkpoint_cols) : next_pow2(K);
+        int64_t K = ob.checkpoint_rows > 0 ? ceil_div(ob.checkpoint_rows, kStripeRows) * kStripeRows : kBatchCkptRows;
+        int64_t Kc = ob.checkpoint_cols > 0   ? std::max<int64_t>(64, next_pow2(ob.checkpoint_cols))
+                     : ob.checkpoint_rows > 0 ? next_pow2(K)
+                                              : kBatchCkptCols;
         int64_t ckpt = 0;
         while (true) {
             ckpt = 0;
@@ -1310,7 +1318,7 @@ int tune_jobs(swh_ctx* c, const std::vector<Job>& jobs, const swh_scoring* sc, c
     for (int64_t k = 0; k < sample; ++k) base.push_back(jobs[(size_t)(k * n / sample)]);
     const int Ws[] = {256, 384, 512, 768, 1024};
     const int Ps[] = {2, 3, 4};
-    const int64_t Ks[] = {256, 512, 1024, 2048};
+    const int64_t Ks[][2] = {{256, 512}, {512, 512}, {1024, 1024}, {2048, 2048}};  // rows x columns
     std::vector<swh_result> res((size_t)sample);
     double best = 1e300;
     *out = swh_batch_tuning{};
@@ -1330,11 +1338,12 @@ int tune_jobs(swh_ctx* c, const std::vector<Job>& jobs, const swh_scoring* sc, c
     }
     int cand = 0;
     const int64_t budget = default_budget(c, o);
-    for (int64_t K : Ks) {
+    for (const auto& KK : Ks) {
+        const int64_t K = KK[0], Kc = KK[1];
         // a spacing whose checkpoints for the whole batch exceed the budget is not a candidate
         int64_t need = 0;
         if (!(o && o->score_only))
-            for (const Job& j : jobs) need += plan_mem(j.m, j.n, K, K);
+            for (const Job& j : jobs) need += plan_mem(j.m, j.n, K, Kc);
         if (need > budget) continue;
         for (int P : Ps) {
             const int Pu = usable_wps(c, P);  // 1 where the LDS reservation is unavailable
@@ -1346,7 +1355,7 @@ int tune_jobs(swh_ctx* c, const std::vector<Job>& jobs, const swh_scoring* sc, c
                 ot.tile_cols = W;
                 ot.waves_per_simd = Pu;
                 ot.checkpoint_rows = K;
-                ot.checkpoint_cols = K;
+                ot.checkpoint_cols = Kc;
                 for (Job& j : js) {
                     const int rc = choose_plan(j.m, j.n, &ot, INT64_MAX / 4, &j.pl);
                     if (rc) return rc;
@@ -1361,12 +1370,14 @@ int tune_jobs(swh_ctx* c, const std::vector<Job>& jobs, const swh_scoring* sc, c
                 if (rc == SWH_ERR_OUT_OF_MEMORY) continue;  // does not fit beside other allocations
                 if (rc) return rc;
                 ++cand;
-                if (W == kBatchTileCols && K == 1024 && Pu == usable_wps(c, kBatchWps)) out->ms_default = ms;
+                if (W == kBatchTileCols && K == kBatchCkptRows && Kc == kBatchCkptCols && Pu == usable_wps(c, kBatchWps))
+                    out->ms_default = ms;
                 if (ms < best) {
                     best = ms;
                     out->tile_cols = W;
                     out->waves_per_simd = Pu;
                     out->checkpoint_rows = K;
+                    out->checkpoint_cols = Kc;
                     out->ms_best = ms;
                 }
             }

@@ -316,7 +316,9 @@ __device__ void sweep_tile(const SweepParams& p, const TileCtx& T, const int2* l
         const bool interior = c >= 1 && c <= last_interior;
         bool dump = false;
         if (MODE == kStage1 && T.scol) {
-            const int64_t lo = T.C0 + max(64 * c - 63, (int64_t)0), hi = T.C0 + min(64 * c + 63, w - 1);
+            // checkpoint columns among the chunk's columns but the tile's last, which goes out
+            // with the right edge at the end (no per-step dump when Kc is a multiple of W)
+            const int64_t lo = T.C0 + max(64 * c - 63, (int64_t)0), hi = T.C0 + min(64 * c + 63, w - 2);
             dump = (lo | (T.Kc - 1)) <= hi;  // smallest x >= lo with x = -1 mod Kc (a power of two)
         }
         if (!dump) {
@@ -364,6 +366,13 @@ __device__ void sweep_tile(const SweepParams& p, const TileCtx& T, const int2* l
 #pragma unroll
         for (int k = 0; k < R; ++k)
             if (row0 + k <= T.r1) right[row0 + k - right_r0] = make_int2(Ho[k] - O, E[k]);
+    }
+    if (MODE == kStage1 && T.scol && w > 0 && ((T.C0 + w) & (T.Kc - 1)) == 0) {
+        // the last column is a checkpoint column: every lane's state is that column's
+        int2* dst = T.scol + (((T.C0 + w) >> __builtin_ctzll((unsigned long long)T.Kc)) - 1) * T.m;
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+            if (row0 + k <= T.r1) dst[row0 + k] = make_int2(Ho[k] - O, E[k]);
     }
     if (MODE == kStage1) {
         // segments of a stripe run in increasing order (one launch each), so a strictly
@@ -725,8 +734,8 @@ __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], Sw
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
             if (MODE == kStage1 && T[t].scol && T[t].w > 0) {
-                const int64_t lo = T[t].C0 + max(64 * c - 63, (int64_t)0), hi = T[t].C0 + min(64 * c + 63, T[t].w - 1);
-                dump = dump || (lo | (T[t].Kc - 1)) <= hi;  // as in sweep_tile
+                const int64_t lo = T[t].C0 + max(64 * c - 63, (int64_t)0), hi = T[t].C0 + min(64 * c + 63, T[t].w - 2);
+                dump = dump || (lo | (T[t].Kc - 1)) <= hi;  // as in sweep_tile (last column: below)
             }
         }
         if (!dump) {
@@ -779,6 +788,9 @@ __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], Sw
             if (row > T[t].r1) continue;
             if (ts[t].right) ts[t].right[row - ts[t].right_r0] = make_int2(Ho[k][t] - O + base[t], E[k][t] + base[t]);
             if (MODE != kStage1) continue;
+            if (T[t].scol && T[t].w > 0 && ((T[t].C0 + T[t].w) & (T[t].Kc - 1)) == 0)  // as in sweep_tile
+                T[t].scol[(((T[t].C0 + T[t].w) >> __builtin_ctzll((unsigned long long)T[t].Kc)) - 1) * T[t].m + row] =
+                    make_int2(Ho[k][t] - O + base[t], E[k][t] + base[t]);
             const int r = rmax[k][t] + base[t];
             if (ts[t].first_segment || r > T[t].rowmax[row]) {  // first segment holding the maximum
                 T[t].rowmax[row] = r;

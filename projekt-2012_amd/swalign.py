@@ -344,7 +344,8 @@ def synth_mutate(seed, src, p_sub, p_indel, flank_frac):
 
 class BatchTuning(C.Structure):
     _fields_ = [("tile_cols", C.c_int32), ("waves_per_simd", C.c_int32), ("checkpoint_rows", C.c_int64),
-                ("ms_best", C.c_double), ("ms_default", C.c_double), ("candidates", C.c_int32), ("_pad", C.c_int32)]
+                ("ms_best", C.c_double), ("ms_default", C.c_double), ("candidates", C.c_int32), ("_pad", C.c_int32),
+                ("checkpoint_cols", C.c_int64)]
 
 
 class ServeStats(C.Structure):

@@ -25,14 +25,16 @@ def _mixed_pairs(sw, seed, n, lens):
     return pairs
 
 
-def _check(orc, pairs, res, st, K=1024, xpoints=True):
+def _check(orc, pairs, res, st, K=256, xpoints=True, Kc=512):
+    """Every result equals the oracle's alignment at checkpoint blocks of K x Kc (the batch
+    default: 256 x 512); crosspoints sit on checkpoint rows, so they depend on K."""
     for (a, b), r, s in zip(pairs, res, st):
         assert s == 0
         A, B = orient(a, b)
         if not A or not B:
             assert r["score"] == 0
             continue
-        ref = orc.align_linear(A, B, orc.scoring(*SC), K, K)
+        ref = orc.align_linear(A, B, orc.scoring(*SC), K, Kc)
         if ref["score"] == 0:
             assert r["score"] == 0
             continue
@@ -106,16 +108,16 @@ def test_batch_autotune(sw, orc, ctx):
     t = ctx.batch_tune(dp, sw.scoring(*SC), sample=64)
     assert t["tile_cols"] in (256, 384, 512, 768, 1024)
     assert t["waves_per_simd"] in (2, 3, 4)
-    assert t["checkpoint_rows"] in (256, 512, 1024, 2048)
+    assert (t["checkpoint_rows"], t["checkpoint_cols"]) in ((256, 512), (512, 512), (1024, 1024), (2048, 2048))
     assert t["candidates"] == 60
     assert 0 < t["ms_best"] <= t["ms_default"] * 1.0001
     # crosspoints sit on checkpoint rows, so they depend on the spacing the tuner picks
     res, st = ctx.batch_device(dp, sw.scoring(*SC), autotune=True)
     _check(orc, pairs, res, st, xpoints=False)
-    K = t["checkpoint_rows"]
+    K, Kc = t["checkpoint_rows"], t["checkpoint_cols"]
     res2, st2 = ctx.batch_device(dp, sw.scoring(*SC), xpoints=True, tile_cols=t["tile_cols"],
-                                 waves_per_simd=t["waves_per_simd"], K=K, Kc=K)
-    _check(orc, pairs, res2, st2, K=K)
+                                 waves_per_simd=t["waves_per_simd"], K=K, Kc=Kc)
+    _check(orc, pairs, res2, st2, K=K, Kc=Kc)
     dev.free()
 
 
@@ -188,7 +190,7 @@ def test_batch_device_planned_rounds(sw, orc, monkeypatch, capfd):
         pairs.append((a, b) if rng.random() < 0.5 else (b, a))
     res, st, dev = _traced_batch(sw, monkeypatch, capfd, pairs, xpoints=True, K=256, Kc=256)
     assert dev
-    _check(orc, pairs, res, st, K=256)
+    _check(orc, pairs, res, st, K=256, Kc=256)
 
 
 def test_batch_device_rounds_equal_host_rounds(sw, orc, ctx, monkeypatch, capfd):
