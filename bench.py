@@ -68,16 +68,16 @@ def _counter_avg(fname, counter, kernel_part):
     return None
 
 
-TRAFFIC_PROFILES = {1: "k_sweep_stage1_1Mbp_pair", 2: "k_sweep_stage1_10Mbp_pair"}
+TRAFFIC_PROFILES = {1: "k_sweep_stage1_1Mbp_pair", 2: "k_sweep_stage1_10Mbp_pair", 3: "k_sweep_pk_65536_pair_batch"}
 
 
-def measured_traffic(workload, codes=1):
+def measured_traffic(workload, codes=1, precision=32):
     """HBM bytes per stage-1 launch from the committed FETCH_SIZE / WRITE_SIZE profiles of
     this workload, each scaled by what the counter reads for a 256 MiB copy."""
     if workload not in TRAFFIC_PROFILES:
         return None, None
     name = TRAFFIC_PROFILES[workload]
-    k = "k_sweep<4, 0, %s>" % ("true" if codes else "false")
+    k = "k_sweep%s<4, 0, %s>" % ("_pk" if precision == 16 else "", "true" if codes else "false")
     fetch = _counter_avg(name + "_fetch_size.txt", "FETCH_SIZE", k)
     write = _counter_avg(name + "_write_size.txt", "WRITE_SIZE", k)
     cf = _counter_avg("copy_256MiB_fetch_size.txt", "FETCH_SIZE", "copyBuffer")
@@ -102,7 +102,7 @@ def roofline(cells, fwd_ms, launches, workload=None, W=None, K=None, Kc=None, pr
     packed int16 pairs at twice the operation rate)."""
     achieved = cells * OPS_PER_CELL / (fwd_ms / 1000.0) / 1e12
     peak = PEAK_INT16X2_TOPS if precision == 16 else PEAK_INT32_TOPS
-    traffic, source = measured_traffic(workload, codes)
+    traffic, source = measured_traffic(workload, codes, precision)
     extra = {}
     if W and K and Kc:
         extra["algorithmic_bytes_per_launch"] = int(algorithmic_bytes(cells, W, K, Kc) / max(launches, 1))
@@ -254,7 +254,10 @@ def batch_record(args, torch, sw, pairs, sc, steps, warmup, cpu=True):
     cells = sum(len(a) * len(b) for a, b in pairs)
     torch.cuda.synchronize()
     kw, tuning = {}, None
-    if not args.no_autotune:
+    if args.batch_settings:
+        W, wps, K, Kc = (int(x) for x in args.batch_settings.split(","))
+        kw = dict(tile_cols=W, waves_per_simd=wps, K=K, Kc=Kc)
+    elif not args.no_autotune:
         # tile width, waves per SIMD and checkpoint spacing, chosen on an evenly spaced eighth
         # of the batch (untimed, like the reference's own launch set-up)
         t0 = time.perf_counter()
@@ -294,7 +297,8 @@ def batch_record(args, torch, sw, pairs, sc, steps, warmup, cpu=True):
             "batch_vs_plain": plain and round(gcups / plain, 2),
             "autotune": tuning and {k: (round(v, 3) if isinstance(v, float) else v) for k, v in tuning.items()},
         },
-        "roofline": roofline(cells, r0["ms_forward"], r0["forward_launches"], precision=r0["precision"]),
+        "roofline": roofline(cells, r0["ms_forward"], r0["forward_launches"], 3, precision=r0["precision"],
+                             codes=r0["score_codes"]),
     }
     if cpu:
         rec["cpu_baseline"] = cpu_baseline_pairs(pairs, sc)
@@ -457,6 +461,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="workload 3: default batch settings")
     ap.add_argument("--no-plain", action="store_true", help="workload 3: skip the pair-after-pair comparison")
+    ap.add_argument("--batch-settings", default="", metavar="W,WPS,K,KC",
+                    help="workload 3: tile columns, sweep waves per SIMD and checkpoint rows / columns "
+                         "instead of the autotuner's choice (the profiles' form)")
     ap.add_argument("--skip", default="", help="with --workload all: comma-separated records to leave out "
                                                 "(chromosome, batched, served, served_mixed)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
