@@ -175,42 +175,86 @@ int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions,
     }
     // Spans of every launch: for diagonal d, each region with tiles on it contributes its
     // stripes [max(0, d - G_r + 1), min(S_r - 1, d)].  Regions are visited longest-lived
-    // first so that each diagonal stops at the first region that has ended.
+    // first (a counting sort by the number of diagonals) so that each diagonal stops at the
+    // first region that has ended.
+    const auto t_spans = std::chrono::steady_clock::now();
     std::vector<int64_t> order((size_t)nr);
-    for (int64_t q = 0; q < nr; ++q) order[(size_t)q] = q;
-    std::sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
-        return Sr[(size_t)x] + Gr[(size_t)x] > Sr[(size_t)y] + Gr[(size_t)y];
-    });
-    std::vector<TileSpan>& spans = c->host_spans;  // kept by the context: feeds an asynchronous copy
-    spans.clear();
+    int64_t total = 0;
     {
-        int64_t total = 0;
-        for (int64_t q = 0; q < nr; ++q) total += Sr[(size_t)q] + Gr[(size_t)q] - 1;
-        spans.reserve((size_t)total);
-    }
-    std::vector<int64_t> span_at((size_t)D + 1, 0), tiles_at((size_t)D, 0);
-    for (int64_t d = 0; d < D; ++d) {
-        span_at[(size_t)d] = (int64_t)spans.size();
-        int64_t t = 0;
-        for (int64_t q : order) {
-            const int64_t Sq = Sr[(size_t)q], Gq = Gr[(size_t)q];
-            if (d > Sq + Gq - 2) break;
-            const int64_t s_lo = std::max<int64_t>(0, d - (Gq - 1)), s_hi = std::min(Sq - 1, d);
-            spans.push_back(TileSpan{t, (int32_t)q, (int32_t)s_lo, d});
-            t += s_hi - s_lo + 1;
+        std::vector<int64_t> at((size_t)D + 2, 0);  // regions living longer than each count
+        for (int64_t q = 0; q < nr; ++q) ++at[(size_t)(D - (Sr[(size_t)q] + Gr[(size_t)q] - 1))];
+        int64_t acc = 0;
+        for (auto& v : at) {
+            const int64_t n_here = v;
+            v = acc;
+            acc += n_here;
         }
-        tiles_at[(size_t)d] = t;
+        for (int64_t q = 0; q < nr; ++q) {
+            order[(size_t)at[(size_t)(D - (Sr[(size_t)q] + Gr[(size_t)q] - 1))]++] = q;
+            total += Sr[(size_t)q] + Gr[(size_t)q] - 1;
+        }
     }
-    span_at[(size_t)D] = (int64_t)spans.size();
     HCK(c, ensure(c->regions, sizeof(Region) * std::max<int64_t>(nr, 1)));
     HCK(c, hipMemcpyAsync(c->regions.p, regions.data(), sizeof(Region) * nr, hipMemcpyHostToDevice, c->stream));
-    HCK(c, ensure(c->spans, sizeof(TileSpan) * std::max<size_t>(spans.size(), 1)));
-    HCK(c, hipMemcpyAsync(c->spans.p, spans.data(), sizeof(TileSpan) * spans.size(), hipMemcpyHostToDevice,
-                          c->stream));
+    HCK(c, ensure(c->spans, sizeof(TileSpan) * (size_t)std::max<int64_t>(total, 1)));
+    // pinned, grow-only: the uploads are asynchronous (the caller synchronises the stream
+    // before the next sweep, as for `regions`); a context that may not allocate pinned
+    // memory stages them in host memory (its uploads complete before the call returns)
+    TileSpan* hs;
+    if (c->fixed_memory) {
+        c->host_spans.resize((size_t)std::max<int64_t>(total, 1));
+        hs = c->host_spans.data();
+    } else {
+        if ((size_t)total > c->pinned_spans_cap) {
+            if (c->pinned_spans) (void)hipHostFree(c->pinned_spans);
+            c->pinned_spans = nullptr;
+            c->pinned_spans_cap = 0;
+            HCK(c, hipHostMalloc((void**)&c->pinned_spans, sizeof(TileSpan) * (size_t)std::max<int64_t>(total, 1), 0));
+            c->pinned_spans_cap = (size_t)std::max<int64_t>(total, 1);
+        }
+        hs = c->pinned_spans;
+    }
+    TileSpan* ds = (TileSpan*)c->spans.p;
     p.regions = (const Region*)c->regions.p;
+    // Large span lists (many regions) are built a few diagonals ahead of the launches that
+    // use them, each part uploaded before its launches, so that the device starts at once
+    // and the host builds the rest while it sweeps; small ones are built and uploaded whole.
+    const bool pipelined = total > (int64_t)1 << 16;
+    int64_t pos = 0, d_built = 0;
+    std::vector<int64_t> span_at((size_t)D + 1, 0), tiles_at((size_t)D, 0);
+    auto build_to = [&](int64_t d_end) {
+        const int64_t from = pos;
+        for (int64_t d = d_built; d < d_end; ++d) {
+            span_at[(size_t)d] = pos;
+            int64_t t = 0;
+            for (int64_t q : order) {
+                const int64_t Sq = Sr[(size_t)q], Gq = Gr[(size_t)q];
+                if (d > Sq + Gq - 2) break;
+                const int64_t s_lo = std::max<int64_t>(0, d - (Gq - 1)), s_hi = std::min(Sq - 1, d);
+                hs[pos++] = TileSpan{t, (int32_t)q, (int32_t)s_lo, d};
+                t += s_hi - s_lo + 1;
+            }
+            tiles_at[(size_t)d] = t;
+        }
+        span_at[(size_t)d_end] = pos;
+        d_built = d_end;
+        return hipMemcpyAsync(ds + from, hs + from, sizeof(TileSpan) * (size_t)(pos - from), hipMemcpyHostToDevice,
+                              c->stream);
+    };
+    if (!pipelined) HCK(c, build_to(D));
+    if (c->trace && nr > 1000) {
+        fprintf(stderr, "swalign: sweep set-up: %lld regions, %lld spans%s, %.3f ms before the first launch\n",
+                (long long)nr, (long long)total, pipelined ? " (built along the launches)" : "",
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_spans).count());
+    }
     if (ev_before) HCK(c, hipEventRecord(ev_before, c->stream));
+    int64_t chunk = 2;
     for (int64_t d = 0; d < D; ++d) {
-        p.spans = (const TileSpan*)c->spans.p + span_at[(size_t)d];
+        if (d == d_built) {
+            HCK(c, build_to(std::min(D, d + chunk)));
+            chunk = std::min<int64_t>(2 * chunk, 16);
+        }
+        p.spans = ds + span_at[(size_t)d];
         p.nspans = (int32_t)(span_at[(size_t)d + 1] - span_at[(size_t)d]);
         p.ntiles = tiles_at[(size_t)d];
         if (packed && (mode == kStage1 || mode == kDirs))
@@ -479,6 +523,7 @@ int device_rounds(swh_ctx* c, std::vector<Job>& jobs, std::vector<int64_t>& acti
     int cur = 0;
     int64_t nb = N0;
     int32_t walking = (int32_t)N0;
+    const auto t_start = std::chrono::steady_clock::now();
     while (walking > 0 && fills(walking)) {
         for (int r = 0; r < kRoundsPerSync; ++r) {
             if (++guard > guard_max) return SWH_ERR_INTERNAL;
@@ -502,8 +547,9 @@ int device_rounds(swh_ctx* c, std::vector<Job>& jobs, std::vector<int64_t>& acti
         HCK(c, hipMemcpyAsync(&walking, cnt + cur, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
         HCK(c, hipStreamSynchronize(c->stream));
         if (c->trace)
-            fprintf(stderr, "swalign: device rounds to %lld: %d pairs walking (%lld slots, %lld launches per round)\n",
-                    (long long)guard, walking, (long long)nb, (long long)(D + 3));
+            fprintf(stderr, "swalign: device rounds to %lld: %d pairs walking (%lld slots, %lld launches per round), "
+                    "%.3f ms\n", (long long)guard, walking, (long long)nb, (long long)(D + 3),
+                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count());
         nb = walking;
     }
     HCK(c, hipMemcpyAsync(states.data(), c->tb_states.p, sizeof(WalkState) * (size_t)nj, hipMemcpyDeviceToHost,
@@ -775,7 +821,8 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
     }
     int64_t wmax = 0;
     for (int64_t k : active) wmax = std::max(wmax, std::min(jobs[k].n, jobs[k].pl.Kc));
-    const int64_t Wtb = eo.batch ? W : tile_cols(c, wmax, eo.tile_cols);
+    int64_t Wtb = eo.batch ? W : tile_cols(c, wmax, eo.tile_cols);
+    if (eo.batch && getenv("SWALIGN_TB_W")) Wtb = atoll(getenv("SWALIGN_TB_W"));
     const int64_t tstride = ceil_div(Wtb + 63, 64) * 64;
     const int64_t Gblk = ceil_div(wmax, Wtb);
     const bool tb_packed = eo.batch && packed && Wtb <= w16;
@@ -803,6 +850,7 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
                            guard, guard_max);
         if (rc) return rc;
     }
+    const auto t_rounds = std::chrono::steady_clock::now();
     while (!active.empty()) {
         if (++guard > guard_max) return SWH_ERR_INTERNAL;
         // lookahead and the pairs walked this round, within the direction-code budget
@@ -898,8 +946,9 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
                 tiles += ceil_div(r.r1 - r.r0 + 1, kStripeRows) * ceil_div(r.c1 - r.c0 + 1, Wtb);
             }
             fprintf(stderr, "swalign: round %lld: %lld pairs, lookahead %lld, %zu blocks, %lld tiles, %.3g cells, "
-                    "%lld launches\n", (long long)guard, (long long)nr, (long long)L, regions.size(), (long long)tiles,
-                    (double)cells, (long long)round_launches);
+                    "%lld launches (from %.3f ms)\n", (long long)guard, (long long)nr, (long long)L, regions.size(),
+                    (long long)tiles, (double)cells, (long long)round_launches,
+                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_rounds).count());
         }
         // device arrays of the round; allocated before their addresses go into the walk args
         HCK(c, ensure(c->wregs, sizeof(WalkRegion) * wregs.size()));
@@ -964,27 +1013,60 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
     clk.mark("traceback", guard);
 
     // ---- results: ops (reversed) -> run-length CIGAR, crosspoints start -> end ---------
+    // The CIGAR text is built on the device (k_cigar: a count pass, offsets on the host, a
+    // write pass) and only the text is read back, a fraction of the ops' bytes.  The
+    // allocation-free entry point, whose buffers are fixed slices, reads the ops back into
+    // the caller's memory and builds the text on the host instead.
+    bool any_tb = false;
+    for (int64_t k : scored) any_tb = any_tb || (jobs[k].traceback && jobs[k].status == SWH_OK);
+    if (!any_tb) return SWH_OK;
+    const bool gpu_cigar = !c->fixed_memory && !c->ops_dst;
     uint8_t* ops_h = c->ops_dst;  // the caller's memory in the allocation-free entry point
-    if (ops_h) {
-        if ((size_t)ops_elems > c->ops_dst_cap) return SWH_ERR_INTERNAL;
-    } else {
-        if ((size_t)ops_elems > c->host_ops_cap) {  // pinned, grow-only: the copy runs at full link speed
+    std::vector<CigarJob> cj;
+    std::vector<int64_t> cj_at((size_t)nj, -1);  // job -> index in cj
+    const char* text_h = nullptr;
+    std::vector<XPoint> xps_h(eo.want_xp ? (size_t)xps_elems : 0);
+    if (gpu_cigar) {
+        for (int64_t k : scored) {
+            const Job& j = jobs[k];
+            if (!j.traceback || j.status != SWH_OK) continue;
+            cj_at[(size_t)k] = (int64_t)cj.size();
+            cj.push_back(CigarJob{(const uint8_t*)c->ops.p + j.ops_off, states[(size_t)k].nops, 0, 0});
+        }
+        const int64_t ncj = (int64_t)cj.size();
+        HCK(c, ensure(c->cigjobs, sizeof(CigarJob) * (size_t)std::max<int64_t>(ncj, 1)));
+        HCK(c, hipMemcpyAsync(c->cigjobs.p, cj.data(), sizeof(CigarJob) * ncj, hipMemcpyHostToDevice, c->stream));
+        HCK(c, launch_cigar((CigarJob*)c->cigjobs.p, ncj, nullptr, c->stream));
+        HCK(c, hipMemcpyAsync(cj.data(), c->cigjobs.p, sizeof(CigarJob) * ncj, hipMemcpyDeviceToHost, c->stream));
+        HCK(c, hipStreamSynchronize(c->stream));
+        int64_t total = 0;
+        for (CigarJob& q : cj) {
+            q.at = total;
+            total += q.len;
+        }
+        HCK(c, ensure(c->cigtext, (size_t)std::max<int64_t>(total, 1)));
+        HCK(c, hipMemcpyAsync(c->cigjobs.p, cj.data(), sizeof(CigarJob) * ncj, hipMemcpyHostToDevice, c->stream));
+        HCK(c, launch_cigar((CigarJob*)c->cigjobs.p, ncj, (char*)c->cigtext.p, c->stream));
+        if ((size_t)total > c->host_ops_cap) {  // pinned, grow-only: the copy runs at full link speed
             if (c->host_ops) (void)hipHostFree(c->host_ops);
             c->host_ops = nullptr;
             c->host_ops_cap = 0;
-            HCK(c, hipHostMalloc((void**)&c->host_ops, (size_t)ops_elems, 0));
-            c->host_ops_cap = (size_t)ops_elems;
+            HCK(c, hipHostMalloc((void**)&c->host_ops, (size_t)total, 0));
+            c->host_ops_cap = (size_t)total;
         }
-        ops_h = c->host_ops;
+        HCK(c, hipMemcpyAsync(c->host_ops, c->cigtext.p, (size_t)total, hipMemcpyDeviceToHost, c->stream));
+        text_h = (const char*)c->host_ops;
+        clk.mark("cigar_gpu", total);
+    } else {
+        if (!ops_h || (size_t)ops_elems > c->ops_dst_cap) return SWH_ERR_INTERNAL;
+        HCK(c, hipMemcpyAsync(ops_h, c->ops.p, (size_t)ops_elems, hipMemcpyDeviceToHost, c->stream));
     }
-    std::vector<XPoint> xps_h(eo.want_xp ? (size_t)xps_elems : 0);
-    HCK(c, hipMemcpyAsync(ops_h, c->ops.p, (size_t)ops_elems, hipMemcpyDeviceToHost, c->stream));
     if (eo.want_xp)
         HCK(c, hipMemcpyAsync(xps_h.data(), c->xps.p, sizeof(XPoint) * xps_elems, hipMemcpyDeviceToHost, c->stream));
     HCK(c, hipStreamSynchronize(c->stream));
     const float ms_tb = elapsed(c->ev[4], c->ev[5]);
-    clk.mark("readback", ops_elems);
-    // CIGARs are built on the host threads, pairs split between them
+    clk.mark("readback", gpu_cigar ? (int64_t)c->host_ops_cap : ops_elems);
+    // results on the host threads, pairs split between them
     std::vector<int> fail(16, 0);
     auto build = [&](int tid, int nth) {
         for (size_t x = (size_t)tid; x < scored.size(); x += (size_t)nth) {
@@ -994,8 +1076,11 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
             if (!j.traceback || j.status != SWH_OK) continue;
             r.ms_traceback = ms_tb;
             r.ms_total = r.ms_forward + r.ms_end_search + ms_tb;
-            if (fill_alignment(r, states[(size_t)k], ops_h + j.ops_off, eo.want_xp ? xps_h.data() + j.xps_off : nullptr,
-                               j.xcap, j.best, eo.want_xp, c->cigar_dst, c->cigar_dst_cap) != SWH_OK) {
+            const CigarJob* q = gpu_cigar ? &cj[(size_t)cj_at[(size_t)k]] : nullptr;
+            if (fill_alignment(r, states[(size_t)k], gpu_cigar ? nullptr : ops_h + j.ops_off,
+                               q ? text_h + q->at : nullptr, q ? q->len : 0,
+                               eo.want_xp ? xps_h.data() + j.xps_off : nullptr, j.xcap, j.best, eo.want_xp,
+                               c->cigar_dst, c->cigar_dst_cap) != SWH_OK) {
                 fail[(size_t)tid] = 1;
                 return;
             }
@@ -1018,31 +1103,43 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
 
 void result_init(swh_result* r) { memset(r, 0, sizeof *r); }
 
-int fill_alignment(swh_result& r, const WalkState& st, const uint8_t* ops, const XPoint* xs, int64_t xcap,
-                   int32_t best, bool want_xp, char* dst, int64_t dst_cap) {
+// The result's start cell, CIGAR and crosspoints.  The CIGAR is either the text the device
+// built (text, text_len) or the run-length encoding of the ops read back (ops, written
+// end -> start).
+int fill_alignment(swh_result& r, const WalkState& st, const uint8_t* ops, const char* text, int64_t text_len,
+                   const XPoint* xs, int64_t xcap, int32_t best, bool want_xp, char* dst, int64_t dst_cap) {
     r.start_row = st.start_i;
     r.start_col = st.start_j;
-    // a run of n ops takes at most (digits of n) + 1 <= 2n characters
-    if (dst && dst_cap < 2 * st.nops + 1) return SWH_ERR_INVALID_ARG;
-    char* out = dst ? dst : (char*)malloc((size_t)(2 * st.nops + 1));
-    if (!out) return SWH_ERR_OUT_OF_MEMORY;
-    size_t len = 0;
-    for (int64_t b = st.nops - 1; b >= 0;) {  // ops were written end -> start
-        int64_t e = b;
-        while (e >= 0 && ops[e] == ops[b]) --e;
-        char digits[24];
-        int nd = 0;
-        for (uint64_t v = (uint64_t)(b - e); v; v /= 10) digits[nd++] = (char)('0' + v % 10);
-        while (nd) out[len++] = digits[--nd];
-        out[len++] = (char)ops[b];
-        b = e;
-    }
-    out[len] = 0;
     r.n_ops = st.nops;
-    r.cigar_len = (int64_t)len;
-    if (dst) return SWH_OK;  // the caller's buffer: r.cigar stays NULL, nothing to free
-    char* shrunk = (char*)realloc(out, len + 1);
-    r.cigar = shrunk ? shrunk : out;
+    if (text) {
+        char* out = (char*)malloc((size_t)text_len + 1);
+        if (!out) return SWH_ERR_OUT_OF_MEMORY;
+        memcpy(out, text, (size_t)text_len);
+        out[text_len] = 0;
+        r.cigar = out;
+        r.cigar_len = text_len;
+    } else {
+        // a run of n ops takes at most (digits of n) + 1 <= 2n characters
+        if (dst && dst_cap < 2 * st.nops + 1) return SWH_ERR_INVALID_ARG;
+        char* out = dst ? dst : (char*)malloc((size_t)(2 * st.nops + 1));
+        if (!out) return SWH_ERR_OUT_OF_MEMORY;
+        size_t len = 0;
+        for (int64_t b = st.nops - 1; b >= 0;) {  // ops were written end -> start
+            int64_t e = b;
+            while (e >= 0 && ops[e] == ops[b]) --e;
+            char digits[24];
+            int nd = 0;
+            for (uint64_t v = (uint64_t)(b - e); v; v /= 10) digits[nd++] = (char)('0' + v % 10);
+            while (nd) out[len++] = digits[--nd];
+            out[len++] = (char)ops[b];
+            b = e;
+        }
+        out[len] = 0;
+        r.cigar_len = (int64_t)len;
+        if (dst) return SWH_OK;  // the caller's buffer: r.cigar stays NULL, nothing to free
+        char* shrunk = (char*)realloc(out, len + 1);
+        r.cigar = shrunk ? shrunk : out;
+    }
     if (want_xp) {
         const int64_t nx = std::min<int64_t>(st.nxp, xcap);
         r.xpoints = (swh_xpoint*)malloc(sizeof(swh_xpoint) * (size_t)(nx + 1));
@@ -1158,7 +1255,8 @@ int swh_ctx_destroy(swh_ctx* c) {
     DevBuf* bufs[] = {&c->seqA, &c->seqB, &c->rbuf, &c->cb, &c->cor, &c->rowmax, &c->srow, &c->scol,
                       &c->dirs, &c->regions, &c->key, &c->found, &c->walk, &c->ops, &c->xps, &c->wregs,
                       &c->argjobs, &c->wargs, &c->seqs, &c->spans, &c->alpha, &c->ranges, &c->rowseg,
-                      &c->segs, &c->tb_pairs, &c->tb_states, &c->tb_act, &c->tb_pre};
+                      &c->segs, &c->tb_pairs, &c->tb_states, &c->tb_act, &c->tb_pre, &c->cigjobs,
+                      &c->cigtext};
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (DevBuf* b : bufs)
         if (b->p) (void)hipFree(b->p);
@@ -1167,6 +1265,7 @@ int swh_ctx_destroy(swh_ctx* c) {
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->staging) (void)hipHostFree(c->staging);
     if (c->host_ops) (void)hipHostFree(c->host_ops);
+    if (c->pinned_spans) (void)hipHostFree(c->pinned_spans);
     for (HostBuf* b : {&c->hsrow, &c->hscol})
         if (b->h) (void)hipHostFree(b->h);
     delete c;

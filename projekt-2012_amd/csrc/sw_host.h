@@ -36,6 +36,7 @@ struct swh_ctx {
     DevBuf seqA, seqB, rbuf, cb, cor, rowmax, srow, scol, dirs, regions, key, found, walk, ops, xps, wregs,
         argjobs, wargs, seqs, spans, alpha, ranges, rowseg, segs;
     DevBuf tb_pairs, tb_states, tb_act, tb_pre;  // traceback rounds planned on the device
+    DevBuf cigjobs, cigtext;                      // CIGAR text built on the device
     uint8_t* staging = nullptr;  // pinned host staging of batch inputs
     size_t staging_cap = 0;
     uint8_t* host_ops = nullptr;  // pinned host copy of the traceback ops
@@ -57,7 +58,10 @@ struct swh_ctx {
     char* cigar_dst = nullptr;      // caller's CIGAR buffer (the result's cigar stays NULL)
     int64_t cigar_dst_cap = 0;
     int64_t dirs_cap_fixed = 0;     // direction-code bytes per traceback round, 0 = from free HBM
-    std::vector<swk::TileSpan> host_spans;  // launch spans of the latest sweep (source of its upload)
+    bool fixed_memory = false;      // every device buffer is a fixed slice: nothing may be allocated
+    std::vector<swk::TileSpan> host_spans;  // launch spans of the latest sweep, fixed-memory contexts
+    swk::TileSpan* pinned_spans = nullptr;  // launch spans of the latest sweep (pinned, grow-only)
+    size_t pinned_spans_cap = 0;
 };
 
 namespace swh_host {
@@ -128,10 +132,13 @@ struct Plan {
 int64_t plan_mem(int64_t m, int64_t n, int64_t K, int64_t Kc);
 int choose_plan(int64_t m, int64_t n, const swh_opts* o, int64_t budget, Plan* pl);
 
-// Fills r's start, CIGAR (run-length, from ops written end -> start) and crosspoints.  With
-// dst the CIGAR is written there (dst_cap bytes, NUL included) and r.cigar stays NULL.
-int fill_alignment(swh_result& r, const swk::WalkState& st, const uint8_t* ops, const swk::XPoint* xps,
-                   int64_t xcap, int32_t best, bool want_xp, char* dst = nullptr, int64_t dst_cap = 0);
+// Fills r's start, CIGAR and crosspoints.  The CIGAR is the text built on the device (text,
+// text_len characters) when text is given, else the run-length encoding of ops (written
+// end -> start).  With dst the encoded CIGAR is written there (dst_cap bytes, NUL included)
+// and r.cigar stays NULL.
+int fill_alignment(swh_result& r, const swk::WalkState& st, const uint8_t* ops, const char* text, int64_t text_len,
+                   const swk::XPoint* xps, int64_t xcap, int32_t best, bool want_xp, char* dst = nullptr,
+                   int64_t dst_cap = 0);
 
 // The device's static scratch array of the allocation-free entry point (sw_noalloc.hip):
 // resolves its address on `device` (which must be current) once per device and process.

@@ -151,6 +151,18 @@ struct ByteRange {
 // ORs into mask[8] (zeroed by the caller) bit x for every byte x present in the ranges.
 hipError_t launch_alphabet(const ByteRange* ranges, int64_t nranges, uint32_t* mask, hipStream_t s);
 
+// Run-length CIGAR text of many alignments, one wave per alignment: ops[0 .. nops) hold the
+// alignment's '=', 'X', 'I', 'D' ops end -> start, the text reads start -> end.  The count
+// pass (out == nullptr) writes each job's text length to len; the write pass writes the
+// text (no terminator) at out + at.
+struct CigarJob {
+    const uint8_t* ops;
+    int64_t nops;
+    int64_t at;   // write pass: offset of the job's text in out
+    int64_t len;  // count pass: characters of the job's text
+};
+hipError_t launch_cigar(CigarJob* jobs, int64_t njobs, char* out, hipStream_t s);
+
 // One workgroup per job.  many: more walks than the device holds at once with the large
 // window (walk_lds_bytes() of LDS each), which then run with small windows.
 hipError_t launch_walk(const WalkArgs* jobs, int njobs, bool many, hipStream_t s);

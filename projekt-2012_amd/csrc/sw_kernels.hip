@@ -1188,10 +1188,87 @@ __global__ __launch_bounds__(kWalkThreads) void k_walk(const WalkArgs* jobs) {
     walk_main<R, WS>(a, lds);
 }
 
+// Run-length CIGAR, one wave per alignment (see CigarJob).  Lane l of a 64-op chunk reads the
+// op at reading position r = r0 + l (ops index nops - 1 - r); a run starts where the op
+// differs from the previous one (the previous lane's, or the chunk before's last for lane 0).
+// Each run start after the first closes the run before it, whose length is the distance to
+// the previous start (the nearest start below the lane in the chunk, or the open run carried
+// from earlier chunks) and whose op is the previous position's: "<length><op>".  The last run
+// closes at nops.  The write pass places each closing lane's text after the texts of the
+// closing lanes below it (a wave prefix sum of the text lengths).
+constexpr int kCigarWaves = 4;
+
+__device__ __forceinline__ int dec_digits(int64_t v) {
+    int d = 1;
+    while (v >= 10) {
+        v /= 10;
+        ++d;
+    }
+    return d;
+}
+
+__device__ __forceinline__ void put_run(char* dst, int64_t len, uint32_t op, int nd) {
+    for (int k = nd - 1; k >= 0; --k) {
+        dst[k] = (char)('0' + len % 10);
+        len /= 10;
+    }
+    dst[nd] = (char)op;
+}
+
+template <bool WRITE>
+__global__ __launch_bounds__(64 * kCigarWaves) void k_cigar(CigarJob* jobs, int64_t njobs, char* out) {
+    const int lane = wave_lane();
+    const int64_t x = (int64_t)blockIdx.x * kCigarWaves + (threadIdx.x >> 6);
+    if (x >= njobs) return;
+    const uint8_t* ops = jobs[x].ops;
+    const int64_t nops = jobs[x].nops;
+    char* dst = WRITE ? out + jobs[x].at : nullptr;
+    int64_t run0 = 0;           // reading position where the open run started
+    uint32_t last = 0;          // op at the last position of the previous chunk
+    int64_t written = 0;        // characters of the job's text so far
+    for (int64_t r0 = 0; r0 < nops; r0 += 64) {
+        const int64_t r = r0 + lane;
+        const bool in = r < nops;
+        const uint32_t op = in ? (uint32_t)ops[nops - 1 - r] : 0u;
+        uint32_t prev = (uint32_t)__shfl_up((int)op, 1);
+        if (lane == 0) prev = last;
+        const bool start = in && r > 0 && op != prev;  // closes the run before it
+        const uint64_t sm = __ballot(start);
+        // the nearest start below this lane in the chunk, else the carried one
+        const uint64_t below = sm & ((UINT64_C(1) << lane) - 1);
+        const int64_t pstart = below ? r0 + 63 - __builtin_clzll(below) : run0;
+        int nd = 0;
+        int64_t len = 0;
+        if (start) {
+            len = r - pstart;
+            nd = dec_digits(len);
+        }
+        int chars = start ? nd + 1 : 0;
+        // inclusive prefix sum of chars over the wave
+        int incl = chars;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int v = __shfl_up(incl, off);
+            if (lane >= off) incl += v;
+        }
+        if (WRITE && start) put_run(dst + written + (incl - chars), len, prev, nd);
+        written += __shfl(incl, 63);
+        if (sm) run0 = r0 + 63 - __builtin_clzll(sm);
+        last = (uint32_t)__shfl((int)op, (int)min<int64_t>(63, nops - 1 - r0));
+    }
+    if (nops > 0) {  // the last run ends at nops
+        const int64_t len = nops - run0;
+        const int nd = dec_digits(len);
+        if (WRITE && lane == 0) put_run(dst + written, len, last, nd);
+        written += nd + 1;
+    }
+    if (!WRITE && lane == 0) jobs[x].len = written;
+}
+
 // One thread per round slot: the checkpoint block of the pair's current cell, cut at the
-// cell (rows up to the cell's, columns up to the cell's rounded up to whole tiles within the
-// block, so that the two tiles of a packed wave have equal widths; the walk never visits the
-// columns right of the cell), as the host-planned rounds build it without lookahead.
+// cell's row and, in columns, at the wider of this slot's and its partner slot's cells
+// (slots 2y and 2y + 1 are the two tiles of one packed wave, which sweeps fastest when both
+// have one width; the walk never visits the columns right of the cell).  Sweeping only up to
+// the cells, instead of whole tiles, halves the columns of a block entered from below.
 __global__ __launch_bounds__(256) void k_tb_plan(TbRoundArgs a) {
     const int64_t x = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (x == 0) *a.nact_out = 0;
@@ -1217,6 +1294,13 @@ __global__ __launch_bounds__(256) void k_tb_plan(TbRoundArgs a) {
     const int64_t r0 = brow * P.K, c0 = bcol * P.Kc;
     const int64_t c_end = min(P.n, c0 + P.Kc) - 1;
     const int64_t cols = j - c0 + 1;
+    int64_t want = cols;
+    const int64_t xp = x ^ 1;
+    if (xp < *a.nact_in) {
+        const int32_t kp = a.act_in[xp];
+        const int64_t jp = a.states[kp].j, Kcp = a.pairs[kp].Kc;
+        want = max(want, jp - jp / Kcp * Kcp + 1);
+    }
     uint16_t* dirs = a.dirs + x * a.dirs_stride;
     Region q{};
     q.A = P.A;
@@ -1226,7 +1310,7 @@ __global__ __launch_bounds__(256) void k_tb_plan(TbRoundArgs a) {
     q.r0 = r0;
     q.r1 = i;
     q.c0 = c0;
-    q.c1 = min(c_end, c0 + (cols + a.W - 1) / a.W * a.W - 1);
+    q.c1 = min(c_end, c0 + want - 1);
     q.W = a.W;
     q.G = a.G;
     q.tstride = a.tstride;
@@ -1369,6 +1453,14 @@ hipError_t launch_alphabet(const ByteRange* ranges, int64_t nranges, uint32_t* m
     if (nranges <= 0) return hipSuccess;
     const unsigned grid = (unsigned)(nranges < 4096 ? nranges : 4096);
     hipLaunchKernelGGL(k_alphabet, dim3(grid), dim3(256), 0, s, ranges, nranges, mask);
+    return hipGetLastError();
+}
+
+hipError_t launch_cigar(CigarJob* jobs, int64_t njobs, char* out, hipStream_t s) {
+    if (njobs <= 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((njobs + kCigarWaves - 1) / kCigarWaves);
+    if (out) hipLaunchKernelGGL(k_cigar<true>, dim3(blocks), dim3(64 * kCigarWaves), 0, s, jobs, njobs, out);
+    else hipLaunchKernelGGL(k_cigar<false>, dim3(blocks), dim3(64 * kCigarWaves), 0, s, jobs, njobs, out);
     return hipGetLastError();
 }
 

@@ -201,7 +201,10 @@ extern "C" int swh_align_pair_device_noalloc(swh_ctx* c, const uint8_t* d1, int6
     lc.staging_cap = 0;
     lc.host_ops = nullptr;
     lc.host_ops_cap = 0;
+    lc.pinned_spans = nullptr;
+    lc.pinned_spans_cap = 0;
     lc.dirs_cap_fixed = y.dirs;
+    lc.fixed_memory = true;
     if (traceback) {
         const int64_t ops_bytes = len1 + len2 + 16;
         lc.ops_dst = (uint8_t*)cigar + (cigar_cap - ops_bytes);

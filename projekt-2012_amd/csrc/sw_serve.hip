@@ -724,7 +724,7 @@ int tick(swh_server* s) {
                 break;
             }
             case kFetch: {
-                const int rc = fill_alignment(j->res, j->st, j->ops_h.data(), j->xps_h.data(),
+                const int rc = fill_alignment(j->res, j->st, j->ops_h.data(), nullptr, 0, j->xps_h.data(),
                                               std::min(j->st.nxp, j->xcap), j->best, s->opts.want_xpoints != 0);
                 std::vector<uint8_t>().swap(j->ops_h);
                 std::vector<XPoint>().swap(j->xps_h);
