@@ -821,8 +821,7 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
     }
     int64_t wmax = 0;
     for (int64_t k : active) wmax = std::max(wmax, std::min(jobs[k].n, jobs[k].pl.Kc));
-    int64_t Wtb = eo.batch ? W : tile_cols(c, wmax, eo.tile_cols);
-    if (eo.batch && getenv("SWALIGN_TB_W")) Wtb = atoll(getenv("SWALIGN_TB_W"));
+    const int64_t Wtb = eo.batch ? W : tile_cols(c, wmax, eo.tile_cols);
     const int64_t tstride = ceil_div(Wtb + 63, 64) * 64;
     const int64_t Gblk = ceil_div(wmax, Wtb);
     const bool tb_packed = eo.batch && packed && Wtb <= w16;

@@ -192,6 +192,14 @@ __device__ __forceinline__ void chunk_steps(const SweepParams& p, const TileCtx&
     if (MODE == kDirs) dcol = T.dirs + (64 * c) * 64 + lane;
     int2* ddst = nullptr;
     const int sdump = DUMP ? dump_step(T.C0, T.w, T.Kc, T.scol, T.m, c, row0, ddst) : -1;
+    int svH[R], svE[R];  // DUMP: the state at the lane's checkpoint column, stored after the chunk
+    if (DUMP) {
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            svH[k] = Ho[k];
+            svE[k] = E[k];
+        }
+    }
     const uint32_t col0 = (uint32_t)(64 * c) - (uint32_t)lane;  // the lane's tile column at step 0
     const int s_end = END ? (int)(w - (64 * c - lane)) : 64;  // first step past column w - 1
     SWK_UNROLL(UNROLL_STEPS)
@@ -259,11 +267,19 @@ __device__ __forceinline__ void chunk_steps(const SweepParams& p, const TileCtx&
         s_out.h[s] = Ho_bot;  // every lane stores; only lane 63's slots are read
         s_out.f[s] = Fe_bot;
         if (MODE == kDirs) dcol[s * 64] = (uint16_t)nib;
-        if (DUMP && s == sdump) {
+        if (DUMP) {  // one select per value and step
+            const bool at = s == sdump;
 #pragma unroll
-            for (int k = 0; k < R; ++k)
-                if (row0 + k <= T.r1) ddst[k] = make_int2(Ho[k] - O, E[k]);
+            for (int k = 0; k < R; ++k) {
+                svH[k] = at ? Ho[k] : svH[k];
+                svE[k] = at ? E[k] : svE[k];
+            }
         }
+    }
+    if (DUMP && sdump >= 0) {
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+            if (row0 + k <= T.r1) ddst[k] = make_int2(svH[k] - O, svE[k]);
     }
 }
 
@@ -546,9 +562,15 @@ __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileC
     const int s_end = END ? (int)(T[0].w - (64 * c - lane)) : 64;  // first step past column w - 1
     int2* ddst[2] = {nullptr, nullptr};
     int sdump[2] = {-1, -1};
+    s16x2 svH[R], svE[R];  // DUMP: each half's state at its checkpoint column, stored after the chunk
     if (DUMP) {
 #pragma unroll
         for (int t = 0; t < 2; ++t) sdump[t] = dump_step(T[t].C0, T[t].w, T[t].Kc, T[t].scol, T[t].m, c, row0[t], ddst[t]);
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+            svH[k] = Ho[k];
+            svE[k] = E[k];
+        }
     }
     SWK_UNROLL(UNROLL_STEPS_PK)
     for (int s = 0; s < 64; ++s) {
@@ -621,15 +643,22 @@ __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileC
             dcol[0][s * 64] = (uint16_t)nib;
         }
         if (DUMP) {
+            // one bitwise select per value and step: the halves at their dump step take the state
+            const int dm = (s == sdump[0] ? 0x0000FFFF : 0) | (s == sdump[1] ? (int)0xFFFF0000 : 0);
 #pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                if (s == sdump[t]) {
-#pragma unroll
-                    for (int k = 0; k < R; ++k)
-                        if (row0[t] + k <= T[t].r1)
-                            ddst[t][k] = make_int2(Ho[k][t] - p.sc.O + base[t], E[k][t] + base[t]);
-                }
+            for (int k = 0; k < R; ++k) {
+                svH[k] = pk_sel(dm, Ho[k], svH[k]);
+                svE[k] = pk_sel(dm, E[k], svE[k]);
             }
+        }
+    }
+    if (DUMP) {
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            if (sdump[t] < 0) continue;
+#pragma unroll
+            for (int k = 0; k < R; ++k)
+                if (row0[t] + k <= T[t].r1) ddst[t][k] = make_int2(svH[k][t] - p.sc.O + base[t], svE[k][t] + base[t]);
         }
     }
 }
