@@ -425,6 +425,8 @@ struct StageClock {
 // (the launches shrink to it) and stops when the rest would be better served by host-planned
 // rounds with lookahead.  On return `active` and `states` hold the pairs still walking.
 constexpr int kRoundsPerSync = 4;
+// Alignments from which their CIGARs are built on the device rather than on the host.
+constexpr int64_t kGpuCigarMinJobs = 256;
 
 int device_rounds(swh_ctx* c, std::vector<Job>& jobs, std::vector<int64_t>& active, std::vector<WalkState>& states,
                   SweepParams p, int2* srow, int2* scol, int64_t Wtb, int64_t tstride, int64_t Gblk, bool tb_packed,
@@ -1012,14 +1014,17 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
     clk.mark("traceback", guard);
 
     // ---- results: ops (reversed) -> run-length CIGAR, crosspoints start -> end ---------
-    // The CIGAR text is built on the device (k_cigar: a count pass, offsets on the host, a
-    // write pass) and only the text is read back, a fraction of the ops' bytes.  The
-    // allocation-free entry point, whose buffers are fixed slices, reads the ops back into
-    // the caller's memory and builds the text on the host instead.
-    bool any_tb = false;
-    for (int64_t k : scored) any_tb = any_tb || (jobs[k].traceback && jobs[k].status == SWH_OK);
-    if (!any_tb) return SWH_OK;
-    const bool gpu_cigar = !c->fixed_memory && !c->ops_dst;
+    // Many alignments (a batch): the CIGAR text is built on the device (k_cigar, one wave
+    // per alignment: a count pass, offsets on the host, a write pass) and only the text is
+    // read back, a fraction of the ops' bytes (65,536-pair batch: 105 MB instead of 492 MB,
+    // and 8 ms instead of 38 ms).  Few alignments: each wave would walk a long alignment
+    // alone (2 ms per pass for the 1 Mbp pair's 204k ops), so the ops are read back and
+    // encoded on the host threads (0.2 ms).  The allocation-free entry point, whose buffers
+    // are fixed slices, reads the ops back into the caller's memory.
+    int64_t n_tb = 0;
+    for (int64_t k : scored) n_tb += (jobs[k].traceback && jobs[k].status == SWH_OK) ? 1 : 0;
+    if (n_tb == 0) return SWH_OK;
+    const bool gpu_cigar = !c->fixed_memory && !c->ops_dst && n_tb >= kGpuCigarMinJobs;
     uint8_t* ops_h = c->ops_dst;  // the caller's memory in the allocation-free entry point
     std::vector<CigarJob> cj;
     std::vector<int64_t> cj_at((size_t)nj, -1);  // job -> index in cj
@@ -1057,7 +1062,18 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
         text_h = (const char*)c->host_ops;
         clk.mark("cigar_gpu", total);
     } else {
-        if (!ops_h || (size_t)ops_elems > c->ops_dst_cap) return SWH_ERR_INTERNAL;
+        if (!ops_h) {  // pinned, grow-only: the copy runs at full link speed
+            if ((size_t)ops_elems > c->host_ops_cap) {
+                if (c->host_ops) (void)hipHostFree(c->host_ops);
+                c->host_ops = nullptr;
+                c->host_ops_cap = 0;
+                HCK(c, hipHostMalloc((void**)&c->host_ops, (size_t)ops_elems, 0));
+                c->host_ops_cap = (size_t)ops_elems;
+            }
+            ops_h = c->host_ops;
+        } else if ((size_t)ops_elems > c->ops_dst_cap) {
+            return SWH_ERR_INTERNAL;
+        }
         HCK(c, hipMemcpyAsync(ops_h, c->ops.p, (size_t)ops_elems, hipMemcpyDeviceToHost, c->stream));
     }
     if (eo.want_xp)

@@ -50,6 +50,28 @@ def test_batch_matches_oracle(sw, orc, ctx):
     _check(orc, pairs, res, st)
 
 
+def test_batch_device_cigars_match_oracle(sw, orc, ctx):
+    """A batch of 300 alignments (CIGAR text built on the device) with runs of every length
+    class: single ops, runs of 9/10/99/100/1000+ (identical and near-identical pairs), long
+    gaps, one-base pairs."""
+    rng = random.Random(23)
+    pairs = _mixed_pairs(sw, 19, 260, [1, 2, 17, 64, 300, 1500])
+    for k in range(40):
+        L = rng.choice([9, 10, 11, 99, 100, 101, 1000, 2500])
+        a = bytes(rng.choice(b"ACGT") for _ in range(L))
+        if k % 3 == 0:
+            b = a  # one run of L
+        elif k % 3 == 1:
+            i = rng.randrange(L)
+            b = a[:i] + bytes([a[i] ^ 6]) + a[i + 1:]  # a mismatch inside
+        else:
+            i = rng.randrange(L)
+            b = a[:i] + b"T" * rng.choice([1, 9, 10, 120]) + a[i:]  # a gap inside
+        pairs.append((a, b))
+    res, st = ctx.batch(pairs, sw.scoring(*SC), xpoints=True)
+    _check(orc, pairs, res, st)
+
+
 @pytest.mark.parametrize("kw", [dict(tile_cols=100), dict(lookahead=-1), dict(lookahead=2, tile_cols=37),
                                 dict(K=512, Kc=256), dict(waves_per_simd=1), dict(waves_per_simd=2),
                                 dict(waves_per_simd=3, tile_cols=256)])
