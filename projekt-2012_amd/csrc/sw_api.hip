@@ -103,16 +103,28 @@ int64_t tile_cols(swh_ctx* c, int64_t cols, int64_t want, int wps) {
 // model predicts the measured ratio of the two sweeps (0.978 against 0.977).
 constexpr double kTwoWaveCost = 2.0 / 1.31;
 
+// Stage-1 segment width of a single pair: one tile per resident sweep wave across the
+// columns, or with skewed stripes (whose launches hold every other diagonal's tiles) two,
+// rounded up to a multiple of 64.
+int64_t pair_tile_cols(swh_ctx* c, int64_t cols, int wps, bool skew) {
+    return skew ? ceil_div(tile_cols(c, cols, 0, 2 * wps), 64) * 64 : tile_cols(c, cols, 0, wps);
+}
+
 // Modelled duration (in tile steps) of a sweep over rows x cols with wps waves per SIMD:
-// one launch per tile anti-diagonal, each lasting ceil(tiles / resident waves) tile times.
-double sweep_model(swh_ctx* c, int64_t rows, int64_t cols, int wps) {
-    const int64_t W = tile_cols(c, cols, 0, wps);
+// one launch per tile anti-diagonal, each lasting ceil(tiles / resident waves) tile times.  A
+// rectangular tile takes W + 63 steps; a skewed stripe's segments W (tile (s, g) on diagonal
+// 2s + g; the lanes' fill and drain fall only in a stripe's first and last segments).
+double sweep_model(swh_ctx* c, int64_t rows, int64_t cols, int wps, bool skew) {
+    const int64_t W = pair_tile_cols(c, cols, wps, skew);
     const int64_t S = ceil_div(rows, kStripeRows), G = ceil_div(cols, W);
     const int64_t P = (int64_t)kSweepWaves * wps * std::max(1, c->props.multiProcessorCount);
+    skew = skew && G >= 2;
+    const int64_t D = skew ? 2 * (S - 1) + G : S + G - 1;
     double t = 0;
-    for (int64_t d = 0; d < S + G - 1; ++d) {
-        const int64_t tiles = std::min(d, S - 1) - std::max<int64_t>(0, d - (G - 1)) + 1;
-        t += (double)ceil_div(tiles, P) * (double)(W + 63);
+    for (int64_t d = 0; d < D; ++d) {
+        const int64_t tiles = skew ? std::min(S - 1, d / 2) - std::max<int64_t>(0, (d - G + 2) / 2) + 1
+                                   : std::min(d, S - 1) - std::max<int64_t>(0, d - (G - 1)) + 1;
+        t += (double)ceil_div(tiles, P) * (double)(skew ? W : W + 63);
     }
     return t * (wps == 2 ? kTwoWaveCost : 1.0);
 }
@@ -125,10 +137,10 @@ bool many_walks(const swh_ctx* c, int64_t njobs) {
 }
 
 // Sweep waves per SIMD for a rows x cols sweep: whichever of one or two the model prefers.
-int choose_wps(swh_ctx* c, int64_t rows, int64_t cols) {
+int choose_wps(swh_ctx* c, int64_t rows, int64_t cols, bool skew) {
     if (c->wps_fixed) return c->wps_fixed;
     if (!c->lds_pad[2]) return 1;
-    return sweep_model(c, rows, cols, 2) < sweep_model(c, rows, cols, 1) ? 2 : 1;
+    return sweep_model(c, rows, cols, 2, skew) < sweep_model(c, rows, cols, 1, skew) ? 2 : 1;
 }
 
 // Runs a sweep over `regions` with tiles of kStripeRows x W: one launch per anti-diagonal
@@ -137,19 +149,24 @@ int choose_wps(swh_ctx* c, int64_t rows, int64_t cols) {
 // recorded immediately before the first and after the last launch.
 int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions, int64_t W, int64_t G_fix,
               hipEvent_t ev_before, hipEvent_t ev_after, int wps, int64_t* launches, int64_t tstride, bool packed,
-              bool sync) {
+              bool sync, bool skew) {
     const int64_t nr = (int64_t)regions.size();
-    std::vector<int64_t> Sr((size_t)nr), Gr((size_t)nr);
+    std::vector<int64_t> Sr((size_t)nr), Gr((size_t)nr), Lr((size_t)nr);
     int64_t G = 0, D = 0, cb_elems = 0, rb_elems = 0, cor_elems = 0;
+    // skewed stripes (stage 1, W a multiple of 64, regions of two or more segments): tile
+    // (s, g) on diagonal 2s + g, lane slots as the segments' hand-off
+    skew = skew && mode == kStage1 && W % 64 == 0;
     for (int64_t q = 0; q < nr; ++q) {
-        const Region& r = regions[(size_t)q];
+        Region& r = regions[(size_t)q];
         const int64_t rows = r.r1 - r.r0 + 1, cols = r.c1 - r.c0 + 1;
         if (rows <= 0 || cols <= 0) return SWH_ERR_INTERNAL;
         Sr[(size_t)q] = ceil_div(rows, kStripeRows);
         Gr[(size_t)q] = ceil_div(cols, W);
+        r.skew = skew && Gr[(size_t)q] >= 2;
+        Lr[(size_t)q] = r.skew ? 2 * (Sr[(size_t)q] - 1) + Gr[(size_t)q] : Sr[(size_t)q] + Gr[(size_t)q] - 1;
         G = std::max(G, Gr[(size_t)q]);
-        D = std::max(D, Sr[(size_t)q] + Gr[(size_t)q] - 1);
-        cb_elems += 2 * rows;
+        D = std::max(D, Lr[(size_t)q]);
+        cb_elems += r.skew ? 2 * Sr[(size_t)q] * 64 * (kRowsPerLane + 1) : 2 * rows;
         rb_elems += 2 * cols;
         cor_elems += 2 * Sr[(size_t)q];
     }
@@ -169,7 +186,7 @@ int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions,
         r.cb = (int2*)c->cb.p + cbo;
         r.rb = (int2*)c->rbuf.p + rbo;
         r.cor = (int32_t*)c->cor.p + coro;
-        cbo += 2 * (r.r1 - r.r0 + 1);
+        cbo += r.skew ? 2 * Sr[(size_t)q] * 64 * (kRowsPerLane + 1) : 2 * (r.r1 - r.r0 + 1);
         rbo += 2 * (r.c1 - r.c0 + 1);
         coro += 2 * Sr[(size_t)q];
     }
@@ -182,7 +199,7 @@ int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions,
     int64_t total = 0;
     {
         std::vector<int64_t> at((size_t)D + 2, 0);  // regions living longer than each count
-        for (int64_t q = 0; q < nr; ++q) ++at[(size_t)(D - (Sr[(size_t)q] + Gr[(size_t)q] - 1))];
+        for (int64_t q = 0; q < nr; ++q) ++at[(size_t)(D - Lr[(size_t)q])];
         int64_t acc = 0;
         for (auto& v : at) {
             const int64_t n_here = v;
@@ -190,8 +207,8 @@ int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions,
             acc += n_here;
         }
         for (int64_t q = 0; q < nr; ++q) {
-            order[(size_t)at[(size_t)(D - (Sr[(size_t)q] + Gr[(size_t)q] - 1))]++] = q;
-            total += Sr[(size_t)q] + Gr[(size_t)q] - 1;
+            order[(size_t)at[(size_t)(D - Lr[(size_t)q])]++] = q;
+            total += Lr[(size_t)q];
         }
     }
     HCK(c, ensure(c->regions, sizeof(Region) * std::max<int64_t>(nr, 1)));
@@ -229,8 +246,15 @@ int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions,
             int64_t t = 0;
             for (int64_t q : order) {
                 const int64_t Sq = Sr[(size_t)q], Gq = Gr[(size_t)q];
-                if (d > Sq + Gq - 2) break;
-                const int64_t s_lo = std::max<int64_t>(0, d - (Gq - 1)), s_hi = std::min(Sq - 1, d);
+                if (d >= Lr[(size_t)q]) break;
+                int64_t s_lo, s_hi;
+                if (regions[(size_t)q].skew) {  // g = d - 2s in [0, Gq)
+                    s_lo = std::max<int64_t>(0, (d - Gq + 2) / 2);
+                    s_hi = std::min(Sq - 1, d / 2);
+                } else {
+                    s_lo = std::max<int64_t>(0, d - (Gq - 1));
+                    s_hi = std::min(Sq - 1, d);
+                }
                 hs[pos++] = TileSpan{t, (int32_t)q, (int32_t)s_lo, d};
                 t += s_hi - s_lo + 1;
             }
@@ -660,11 +684,27 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
     bool packed = (eo.precision == 16 || (eo.precision == 0 && eo.batch)) && w16 > 0;
     int wps = 1;
     int64_t W = 0;
+    // skewed stage-1 stripes (no lane fill and drain between segments) need W a multiple of
+    // 64 and, packed, 63 more columns of int16 headroom (a skewed tile spans W + 63 columns)
+    bool skew = eo.skew;
     if (eo.batch) {
         wps = usable_wps(c, eo.wps ? eo.wps : (c->wps_fixed ? c->wps_fixed : kBatchWps));
         W = eo.tile_cols > 0 ? eo.tile_cols : kBatchTileCols;
-        packed = packed && W <= w16;
+        skew = skew && W % 64 == 0 && (!packed || W + 63 <= w16);
+        packed = packed && (skew ? W + 63 : W) <= w16;
+    } else if (skew && !packed &&
+               (eo.skew_forced || eo.tile_cols > 0 ||
+                sweep_model(c, max_m, max_n, choose_wps(c, max_m, max_n, true), true) <
+                    sweep_model(c, max_m, max_n, choose_wps(c, max_m, max_n, false), false))) {
+        // a skewed stripe's launches hold tiles of every other diagonal: segments of half the
+        // width keep the resident waves busy.  Skewed when the model says so: the 1 Mbp pair
+        // (4.58 M against 5.02 M modelled tile steps; 470 against 531 ms measured), not the 10 Mbp
+        // pair (205.0 M against 203.3 M; twice the launches for 1.3% less filling and draining)
+        wps = eo.tile_cols > 0 ? 1 : choose_wps(c, max_m, max_n, true);
+        W = eo.tile_cols > 0 ? eo.tile_cols : pair_tile_cols(c, max_n, wps, true);
+        skew = W % 64 == 0;
     } else {
+        skew = false;
         if (packed) {  // two tiles per wave, one wave per SIMD
             W = tile_cols(c, max_n, eo.tile_cols, 2);
             if (W > w16 && eo.tile_cols > 0) {
@@ -676,7 +716,7 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
             }
         }
         if (!packed) {
-            wps = eo.tile_cols > 0 ? 1 : choose_wps(c, max_m, max_n);
+            wps = eo.tile_cols > 0 ? 1 : choose_wps(c, max_m, max_n, false);
             W = tile_cols(c, max_n, eo.tile_cols, wps);
         }
     }
@@ -692,7 +732,7 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
     }
     clk.mark("setup", (int64_t)live.size());
     int64_t launches = 0;
-    int rc = run_sweep(c, kStage1, p, regions, W, 0, c->ev[0], c->ev[1], wps, &launches, 0, packed);
+    int rc = run_sweep(c, kStage1, p, regions, W, 0, c->ev[0], c->ev[1], wps, &launches, 0, packed, true, skew);
     if (rc) return rc;
     clk.mark("stage1", launches);
     for (int64_t k : live) {
@@ -750,8 +790,14 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
             int64_t q0 = 0, q1 = nq - 1;
             const int64_t g = endseg[(size_t)k];
             if (g >= 0 && g * W < j.n) {
-                q0 = (g * W) / bw;
-                q1 = (std::min(j.n, (g + 1) * W) - 1) / bw;
+                // the segment's columns in the end row: [gW, (g+1)W), or in a skewed stripe
+                // shifted left by the row's lane (see TileCtx in sw_kernels.hip)
+                const int64_t G = ceil_div(j.n, W);
+                const int64_t l = (skew && G >= 2) ? (j.bi % kStripeRows) / kRowsPerLane : 0;
+                const int64_t lo = std::max<int64_t>(0, g * W - l);
+                const int64_t hi = (g + 1 == G) ? j.n - 1 : std::min(j.n, (g + 1) * W - l) - 1;
+                q0 = lo / bw;
+                q1 = hi / bw;
             }
             for (int64_t q = q0; q <= q1; ++q) {
                 Region r{};
@@ -1180,6 +1226,11 @@ EngineOpts engine_opts(const swh_opts* o, bool batch) {
     eo.lookahead = batch ? 0 : 7;
     eo.lookahead_auto = batch && !(o && o->tb_lookahead != 0);
     if (o && o->tb_lookahead != 0) eo.lookahead = std::max(0, o->tb_lookahead);
+    // skewed stage-1 stripes by default (single pairs: where the wavefront model prefers them);
+    // SWALIGN_SKEW=0 / 1 forces rectangular tiles / skewed stripes
+    const char* sk = getenv("SWALIGN_SKEW");
+    eo.skew = sk ? atoi(sk) != 0 : true;
+    eo.skew_forced = sk && atoi(sk) != 0;
     return eo;
 }
 

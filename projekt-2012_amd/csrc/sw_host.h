@@ -84,7 +84,7 @@ void result_init(swh_result* r);
 
 // Columns per tile: `want` if given, else one tile per resident sweep wave.
 int64_t tile_cols(swh_ctx* c, int64_t cols, int64_t want, int wps = 1);
-int choose_wps(swh_ctx* c, int64_t rows, int64_t cols);
+int choose_wps(swh_ctx* c, int64_t rows, int64_t cols, bool skew);
 // w if the context can hold the sweep to w waves per SIMD (1 <= w <= kMaxSweepWps), else 1.
 int usable_wps(const swh_ctx* c, int w);
 // True if njobs walks exceed what the device's LDS holds at once with large walk windows.
@@ -94,7 +94,7 @@ bool many_walks(const swh_ctx* c, int64_t njobs);
 int run_sweep(swh_ctx* c, int mode, swk::SweepParams p, std::vector<swk::Region>& regions, int64_t W,
               int64_t G_fix = 0, hipEvent_t ev_before = nullptr, hipEvent_t ev_after = nullptr, int wps = 1,
               int64_t* launches = nullptr, int64_t tstride = 0, bool packed = false,
-              bool sync = true);
+              bool sync = true, bool skew = false);
 
 // Widest stage-1 tile (columns) whose values stay in int16 relative to the tile's base under
 // scoring sc, for the packed stage 1; 0 if none.
@@ -169,6 +169,8 @@ struct EngineOpts {
     bool want_xp = false;
     bool batch = false;      // many pairs: fixed tile width, two waves per SIMD
     bool host_ckpt = false;  // checkpoints in mapped pinned host memory
+    bool skew = false;       // skewed stage-1 stripes (see run_sweep)
+    bool skew_forced = false;  // single pairs: skewed even where the model prefers rectangles
 };
 
 // Stage 1, end search and traceback of every job, all jobs side by side in each sweep.

@@ -56,7 +56,8 @@ struct Region {
     // kSearch
     int64_t target_row;
     int32_t target;
-    int32_t _pad;
+    int32_t skew;      // kStage1: skewed segments (TileCtx in sw_kernels.hip): tiles (s, g) at diagonal
+                       // d = 2s + g, W a multiple of 64, cb holding lane slots instead of a column
     int32_t* found;    // smallest column of target_row holding the target value
 };
 

@@ -63,12 +63,20 @@ __device__ __forceinline__ int to_vgpr(int v) {
 // Lane within the wavefront (a sweep workgroup holds several independent waves).
 __device__ __forceinline__ int wave_lane() { return (int)(threadIdx.x & 63u); }
 
-// One tile of a sweep: rows [R0, R0 + 64R) clipped to r1, columns [C0, C0 + w).
+// One tile of a sweep: rows [R0, R0 + 64R) clipped to r1.  Lane l takes column C0 + t - l at
+// step t (every tile), over the tile columns x = t - l with, per lane,
+//   first: x >= 0 (the lane starts at step l), else x >= -l (it starts at step 0);
+//   last:  x <  w (it ends at step w - 1 + l), else x < w - l (it ends at step w - 1).
+// A rectangular tile (columns [C0, C0 + w) for every lane) is first and last.  A skewed
+// stage-1 stripe cuts its columns into segments of W columns for lane 0 and W columns shifted
+// left by l for lane l: only its first segment starts lanes late and only its last ends them
+// late, so its other tiles spend no steps filling or draining lanes.
 struct TileCtx {
     const uint8_t* A;   // the pair's rows and columns
     const uint8_t* B;
     int64_t m;          // the pair's rows (stride of the checkpoint columns)
     int64_t R0, r1, C0, w;
+    bool first, last;
     int2* scol;         // stage 1: checkpoint columns (nullable), every Kc columns
     int64_t Kc;
     int32_t* rowmax;    // stage 1: row maxima
@@ -153,16 +161,17 @@ __device__ __forceinline__ void row_symbol(const SweepParams& p, bool in, uint32
 // Checkpoint column of a stage-1 chunk: the step s in [0, 64) at which this lane's column
 // C0 + 64c + s - lane (tile columns [0, w)) is one (column + 1 a multiple of Kc), or -1; dst
 // receives the lane's first row of that checkpoint column.  Kc is a power of two >= 64, so a
-// lane meets at most one checkpoint column in the 64 steps of a chunk.
-__device__ __forceinline__ int dump_step(int64_t C0, int64_t w, int64_t Kc, int2* scol, int64_t m, int64_t c,
-                                         int64_t row0, int2*& dst) {
+// lane meets at most one checkpoint column in the 64 steps of a chunk.  The lane's tile
+// columns are [xlo, xlo + xlen) (see TileCtx).
+__device__ __forceinline__ int dump_step(int64_t C0, int64_t xlo, int64_t xlen, int64_t Kc, int2* scol, int64_t m,
+                                         int64_t c, int64_t row0, int2*& dst) {
     dst = nullptr;
     if (!scol) return -1;
     const int64_t x0 = 64 * c - wave_lane();  // the lane's tile column at step 0
     const int64_t X1 = C0 + x0 + 1;
     const int64_t st = (-X1) & (Kc - 1);      // steps to the next column whose index + 1 is a multiple of Kc
     const int64_t x = x0 + st;
-    if (st >= 64 || x < 0 || x >= w) return -1;
+    if (st >= 64 || x < xlo || x >= xlo + xlen) return -1;
     dst = scol + (((X1 + st) >> __builtin_ctzll((unsigned long long)Kc)) - 1) * m + row0;
     return (int)st;
 }
@@ -190,8 +199,11 @@ __device__ __forceinline__ void chunk_steps(const SweepParams& p, const TileCtx&
     const int64_t w = T.w;
     uint16_t* dcol = nullptr;
     if (MODE == kDirs) dcol = T.dirs + (64 * c) * 64 + lane;
+    // the lane's tile columns: [xlo, xlo + xlen) (see TileCtx)
+    const int xlo = T.first ? 0 : -lane;
+    const int xlen = (int)(T.last ? w : w - lane) - xlo;
     int2* ddst = nullptr;
-    const int sdump = DUMP ? dump_step(T.C0, T.w, T.Kc, T.scol, T.m, c, row0, ddst) : -1;
+    const int sdump = DUMP ? dump_step(T.C0, xlo, xlen, T.Kc, T.scol, T.m, c, row0, ddst) : -1;
     int svH[R], svE[R];  // DUMP: the state at the lane's checkpoint column, stored after the chunk
     if (DUMP) {
 #pragma unroll
@@ -216,10 +228,10 @@ __device__ __forceinline__ void chunk_steps(const SweepParams& p, const TileCtx&
         bool valid = true;
         int col = 0;
         if (EDGE || MODE == kSearch) {
-            // tile column 64c + s - lane, in [-63, w + 126]: mod 2^32, the columns below 0
-            // compare above any width
+            // tile column 64c + s - lane, in [-63, w + 126]: mod 2^32, the columns below the
+            // lane's first compare above any width
             const uint32_t ucol = col0 + (uint32_t)s;
-            valid = ucol < (uint32_t)w;
+            valid = ucol - (uint32_t)xlo < (uint32_t)xlen;
             col = (int)ucol;
         }
         int diag = Ho_up_prev;
@@ -283,17 +295,39 @@ __device__ __forceinline__ void chunk_steps(const SweepParams& p, const TileCtx&
     }
 }
 
-// Sweeps one tile.  left/left_r0: {H,E} of column C0-1 at absolute row r is
-// left[r - left_r0] (nullptr = zero column); corner = H(R0-1, C0-1); right (nullable)
-// receives the tile's last column {H,E} at index row - right_r0.
+// Per-lane hand-off between the segments of a skewed stripe: lane l's state at its last
+// column of segment g (column C0 + W - 1 - l), which it resumes from in segment g + 1.  R + 1
+// int2 per lane: {H, E} of its R rows, then {F + gap_extend of its last row, H of the cell
+// above its first row}: the latter is the lane's diagonal carry, the H above-left of its first
+// cell in segment g + 1.
+template <int R>
+constexpr int lane_slot() { return R + 1; }
+
+// Everything a wave needs to sweep one tile (see make_tile).
+struct TileSetup {
+    TileCtx T;
+    const int2* left;     // rectangular tiles: {H,E} of column C0-1 at row r is left[r - left_r0]
+    int64_t left_r0;      // (nullptr = zero column)
+    int corner;           // H(R0-1, C0-1)
+    int2* right;          // rectangular tiles (nullable): the last column {H,E} at row - right_r0
+    int64_t right_r0;
+    const int2* lane_in;  // skewed tiles after a stripe's first: the previous segment's lane slots
+    int2* lane_out;       // skewed tiles before a stripe's last: this segment's lane slots
+    bool first_segment;
+    int32_t* found;
+};
+
+// Sweeps one tile (see TileSetup for its inputs and outputs).
 template <int R, int MODE, bool PROF>
-__device__ void sweep_tile(const SweepParams& p, const TileCtx& T, const int2* left, int64_t left_r0, int corner,
-                           int2* right, int64_t right_r0, bool first_segment, int32_t* found, SweepLds& lds) {
+__device__ void sweep_tile(const SweepParams& p, const TileSetup& ts, SweepLds& lds) {
+    const TileCtx& T = ts.T;
     const BottomSlots s_slots = bottom_slots(lds);
     const int lane = wave_lane();
     const int O = p.sc.O;
     const int64_t w = T.w;
     const int64_t row0 = T.R0 + (int64_t)lane * R;
+    constexpr int LS = lane_slot<R>();
+    const int2* lin = ts.lane_in ? ts.lane_in + (int64_t)lane * LS : nullptr;
 
     RowSyms<R> a;
     int Ho[R], E[R], rmax[R];
@@ -302,8 +336,12 @@ __device__ void sweep_tile(const SweepParams& p, const TileCtx& T, const int2* l
         const int64_t row = row0 + k;
         const bool in = row <= T.r1;
         row_symbol<PROF>(p, in, in ? (uint32_t)T.A[row] : 0u, a.lo[k], a.hi[k]);
-        if (left && in) {
-            const int2 v = left[row - left_r0];
+        if (lin) {
+            const int2 v = lin[k];
+            Ho[k] = v.x + O;
+            E[k] = v.y;
+        } else if (ts.left && in) {
+            const int2 v = ts.left[row - ts.left_r0];
             Ho[k] = v.x + O;
             E[k] = v.y;
         } else {
@@ -312,39 +350,55 @@ __device__ void sweep_tile(const SweepParams& p, const TileCtx& T, const int2* l
         }
         rmax[k] = 0;
     }
-    // H(row0-1, C0-1) + O: lane 0 takes the corner, the other lanes the left column.
-    int Ho_up_prev = corner + O;
-    if (lane > 0) Ho_up_prev = (left && row0 - 1 <= T.r1) ? left[row0 - 1 - left_r0].x + O : O;
+    // H + O of the cell above-left of the lane's first cell: lane 0 takes the corner, the
+    // other lanes the left column (rectangular) or their slot (skewed).
+    int Ho_up_prev = ts.corner + O;
+    if (lin) Ho_up_prev = lin[R].y + O;
+    else if (lane > 0) Ho_up_prev = (ts.left && row0 - 1 <= T.r1) ? ts.left[row0 - 1 - ts.left_r0].x + O : O;
+    // The bottom row {H+O, F+E} of the lane's previous column, which the lane below takes at
+    // its first step; a skewed tile's lanes all start at step 0, so theirs come from the slots.
     int Ho_bot = O, Fe_bot = p.sc.E;
+    int bl = PROF ? kPadSel : kPadByte;  // symbol of the lane's previous column, likewise
+    if (lin) {
+        Ho_bot = Ho[R - 1];
+        Fe_bot = lin[R].x;
+        bl = col_symbol<PROF>(p, T.B[T.C0 - lane - 1]);
+    }
     int found_col = -1;
 
-    const int64_t nsteps = w + 63;
+    // steps: lanes end at step w - 1 + lane in a last segment, at step w - 1 otherwise
+    const int64_t nsteps = T.last ? w + 63 : w;
     const int64_t nchunks = (nsteps + 63) / 64;
-    const int64_t last_interior = (w - 64) / 64;
+    // chunks where every lane is inside the tile at every step
+    const int64_t first_interior = T.first ? 1 : 0;
+    const int64_t last_interior = T.last ? (w >= 64 ? (w - 64) / 64 : -1) : nchunks - 1;
 
     int2 in_reg;
     int b_reg;
     load_chunk_inputs<PROF>(p, T, 0, in_reg, b_reg);
-    int Qh = in_reg.x, Qf = in_reg.y, Qb = b_reg, bl = PROF ? kPadSel : kPadByte;
+    int Qh = in_reg.x, Qf = in_reg.y, Qb = b_reg;
 
     for (int64_t c = 0; c < nchunks; ++c) {
         if (c + 1 < nchunks) load_chunk_inputs<PROF>(p, T, c + 1, in_reg, b_reg);
-        const bool interior = c >= 1 && c <= last_interior;
+        const bool interior = c >= first_interior && c <= last_interior;
+        const bool starts = T.first && c == 0 && (!T.last || w >= 64);   // lanes start here, none ends
+        const bool ends = c >= first_interior && T.last;                  // lanes end here, none starts
         bool dump = false;
         if (MODE == kStage1 && T.scol) {
-            // checkpoint columns among the chunk's columns but the tile's last, which goes out
-            // with the right edge at the end (no per-step dump when Kc is a multiple of W)
-            const int64_t lo = T.C0 + max(64 * c - 63, (int64_t)0), hi = T.C0 + min(64 * c + 63, w - 2);
+            // checkpoint columns among the columns the chunk's lanes take, but each lane's last,
+            // which goes out after the tile (no per-step dump when Kc is a multiple of W)
+            const int64_t lo = T.C0 + max(64 * c - 63, T.first ? (int64_t)0 : (int64_t)-63);
+            const int64_t hi = T.C0 + min(64 * c + 63, w - 2);
             dump = (lo | (T.Kc - 1)) <= hi;  // smallest x >= lo with x = -1 mod Kc (a power of two)
         }
         if (!dump) {
             if (interior)
                 chunk_steps<R, MODE, PROF, false, false>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
                                                    Qh, Qf, Qb, bl, s_slots, found_col);
-            else if (c == 0 && w >= 64)
+            else if (starts)
                 chunk_steps<R, MODE, PROF, false, false, true>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot,
                                                          Fe_bot, Qh, Qf, Qb, bl, s_slots, found_col);
-            else if (c >= 1)  // no column below 0 from the second chunk on
+            else if (ends)
                 chunk_steps<R, MODE, PROF, false, false, false, true>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev,
                                                                 Ho_bot, Fe_bot, Qh, Qf, Qb, bl, s_slots, found_col);
             else
@@ -353,21 +407,22 @@ __device__ void sweep_tile(const SweepParams& p, const TileCtx& T, const int2* l
         } else if (interior) {
             chunk_steps<R, MODE, PROF, false, true>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot, Qh, Qf,
                                               Qb, bl, s_slots, found_col);
-        } else if (c >= 1) {
+        } else if (ends) {
             chunk_steps<R, MODE, PROF, false, true, false, true>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot,
                                                            Fe_bot, Qh, Qf, Qb, bl, s_slots, found_col);
         } else {
             chunk_steps<R, MODE, PROF, true, true>(p, T, c, row0, a, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot, Qh, Qf,
                                              Qb, bl, s_slots, found_col);
         }
-        // Bottom row of the tile for columns 64c-63+lane.  The LDS hand-off from lane 63 to
-        // the other lanes needs no hardware barrier (one wave, LDS operations in order) but
-        // must not be reordered by the compiler.
+        // Bottom row of the tile: lane 63's outputs of the chunk's steps 64c + lane, tile
+        // column 64c + lane - 63 where lane 63 was inside the tile.  The LDS hand-off from lane
+        // 63 to the other lanes needs no hardware barrier (one wave, LDS operations in order)
+        // but must not be reordered by the compiler.
         __builtin_amdgcn_wave_barrier();
         {
             const int2 o = make_int2(lds.h[lane], lds.f[lane]);
             const int64_t col = 64 * c + lane - 63;
-            if (col >= 0 && col < w) {
+            if (col >= (T.first ? 0 : -63) && col < (T.last ? w : w - 63)) {
                 const int2 v = make_int2(o.x - O, o.y - p.sc.E);
                 if (T.bot) T.bot[col] = v;
                 if (MODE == kStage1 && T.srow) T.srow[col] = v;
@@ -378,17 +433,27 @@ __device__ void sweep_tile(const SweepParams& p, const TileCtx& T, const int2* l
         Qb = b_reg;
         __builtin_amdgcn_wave_barrier();
     }
-    if (right) {
+    if (ts.right) {
 #pragma unroll
         for (int k = 0; k < R; ++k)
-            if (row0 + k <= T.r1) right[row0 + k - right_r0] = make_int2(Ho[k] - O, E[k]);
+            if (row0 + k <= T.r1) ts.right[row0 + k - ts.right_r0] = make_int2(Ho[k] - O, E[k]);
     }
-    if (MODE == kStage1 && T.scol && w > 0 && ((T.C0 + w) & (T.Kc - 1)) == 0) {
-        // the last column is a checkpoint column: every lane's state is that column's
-        int2* dst = T.scol + (((T.C0 + w) >> __builtin_ctzll((unsigned long long)T.Kc)) - 1) * T.m;
+    if (ts.lane_out) {  // a skewed segment's every lane ran to step w - 1
+        int2* lo = ts.lane_out + (int64_t)lane * LS;
 #pragma unroll
-        for (int k = 0; k < R; ++k)
-            if (row0 + k <= T.r1) dst[row0 + k] = make_int2(Ho[k] - O, E[k]);
+        for (int k = 0; k < R; ++k) lo[k] = make_int2(Ho[k] - O, E[k]);
+        lo[R] = make_int2(Fe_bot, Ho_up_prev - O);
+    }
+    if (MODE == kStage1 && T.scol) {
+        // the lane's last column (C0 + w - 1, or C0 + w - 1 - lane in a skewed segment before
+        // the last) is a checkpoint column: the lane's state is that column's
+        const int64_t xl = T.C0 + w - 1 - (T.last ? 0 : lane);
+        if (w > 0 && ((xl + 1) & (T.Kc - 1)) == 0) {
+            int2* dst = T.scol + (((xl + 1) >> __builtin_ctzll((unsigned long long)T.Kc)) - 1) * T.m;
+#pragma unroll
+            for (int k = 0; k < R; ++k)
+                if (row0 + k <= T.r1) dst[row0 + k] = make_int2(Ho[k] - O, E[k]);
+        }
     }
     if (MODE == kStage1) {
         // segments of a stripe run in increasing order (one launch each), so a strictly
@@ -396,34 +461,23 @@ __device__ void sweep_tile(const SweepParams& p, const TileCtx& T, const int2* l
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             const int64_t row = row0 + k;
-            if (row <= T.r1 && (first_segment || rmax[k] > T.rowmax[row])) {
+            if (row <= T.r1 && (ts.first_segment || rmax[k] > T.rowmax[row])) {
                 T.rowmax[row] = rmax[k];
                 if (T.rowseg) T.rowseg[row] = T.seg;
             }
         }
     }
     if (MODE == kSearch) {
-        if (found_col >= 0) atomicMin(found, (int32_t)(T.C0 + found_col));
+        if (found_col >= 0) atomicMin(ts.found, (int32_t)(T.C0 + found_col));
     }
 }
 
-// One wave = one tile (s, g) with s + g = d of its span; tile index x = wave number in the grid,
-// found in the launch's spans by binary search.  Waves of a workgroup are independent.
-// Everything a wave needs to sweep tile x of the launch (see sweep_main).
-struct TileSetup {
-    TileCtx T;
-    const int2* left;
-    int64_t left_r0;
-    int corner;
-    int2* right;
-    int64_t right_r0;
-    bool first_segment;
-    int32_t* found;
-};
-
 // Locates tile x of the launch (binary search over the launch's spans) and its inputs and
-// outputs; false if the tile does not exist.  Hands the next segment of the stripe its
-// corner (the top boundary at this tile's last column).
+// outputs; false if the tile does not exist.  Rectangular tiles: tile (s, g) of a span at
+// diagonal d has g = d - s; it hands the next segment of the stripe its corner (the top
+// boundary at this tile's last column).  Skewed stage-1 tiles (Region.skew): g = d - 2s, since
+// segment g of stripe s needs segments g and g + 1 of stripe s - 1 (the bottom row of the
+// stripe above at its columns [C0, C0 + w) ends 63 columns into the next segment).
 template <int R, int MODE>
 __device__ bool make_tile(const SweepParams& p, int64_t x, TileSetup& ts, int* span_io = nullptr) {
     constexpr int64_t HR = 64 * R;
@@ -439,13 +493,14 @@ __device__ bool make_tile(const SweepParams& p, int64_t x, TileSetup& ts, int* s
     }
     if (span_io) *span_io = lo;
     const TileSpan sp = p.spans[lo];
-    const int64_t s = sp.s_lo + (x - sp.first_tile);
-    const int64_t g = sp.d - s;
     const Region Rg = p.regions[sp.region];
+    const bool skew = MODE == kStage1 && Rg.skew;
+    const int64_t s = sp.s_lo + (x - sp.first_tile);
+    const int64_t g = sp.d - (skew ? 2 * s : s);
     const int64_t rows = Rg.r1 - Rg.r0 + 1, cols = Rg.c1 - Rg.c0 + 1;
     const int64_t W = Rg.W;
     const int64_t nstr = (rows + HR - 1) / HR, nseg = (cols + W - 1) / W;
-    if (s >= nstr || g >= nseg) return false;  // a narrower region has fewer segments
+    if (s >= nstr || g < 0 || g >= nseg) return false;  // a narrower region has fewer segments
     TileCtx& T = ts.T;
     T.A = Rg.A;
     T.B = Rg.B;
@@ -461,6 +516,8 @@ __device__ bool make_tile(const SweepParams& p, int64_t x, TileSetup& ts, int* s
     T.r1 = Rg.r1;
     T.C0 = Rg.c0 + g * W;
     T.w = min(W, Rg.c1 - T.C0 + 1);
+    T.first = !skew || g == 0;
+    T.last = !skew || g + 1 == nseg;
     const int64_t coff = T.C0 - Rg.c0;
     T.top = s == 0 ? (Rg.top ? Rg.top + coff : nullptr) : Rg.rb + (s & 1) * cols + coff;
     T.bot = s + 1 < nstr ? Rg.rb + ((s + 1) & 1) * cols + coff : nullptr;
@@ -470,35 +527,49 @@ __device__ bool make_tile(const SweepParams& p, int64_t x, TileSetup& ts, int* s
         if (brow <= Rg.r1 && (brow + 1) % Rg.K == 0) T.srow = Rg.srow + ((brow + 1) / Rg.K - 1) * Rg.n + T.C0;
     }
     T.dirs = MODE == kDirs ? Rg.dirs + (s * Rg.G + g) * Rg.tstride * 64 : nullptr;
-    // Left column and corner H(R0-1, C0-1).
-    if (g == 0) {
-        ts.left = Rg.left;
-        ts.left_r0 = 0;
-        ts.corner = (Rg.left && T.R0 >= 1) ? Rg.left[T.R0 - 1].x : 0;
-    } else {
-        ts.left = Rg.cb + (g & 1) * rows;
-        ts.left_r0 = Rg.r0;
-        if (s == 0) ts.corner = Rg.top ? Rg.top[coff - 1].x : 0;
-        else ts.corner = Rg.cor[2 * s + (g & 1)];
-    }
-    if (s > 0 && g + 1 < nseg && wave_lane() == 0) Rg.cor[2 * s + ((g + 1) & 1)] = T.top[T.w - 1].x;
-    ts.right = g + 1 < nseg ? Rg.cb + ((g + 1) & 1) * rows : nullptr;
+    ts.lane_in = nullptr;
+    ts.lane_out = nullptr;
+    ts.right = nullptr;
     ts.right_r0 = Rg.r0;
+    if (skew) {
+        // lane slots: [segment parity][stripe][lane]; the bottom row of the stripe above (or
+        // the region's top row) holds the corner H(R0-1, C0-1) one column left of C0
+        ts.left = g == 0 ? Rg.left : nullptr;
+        ts.left_r0 = 0;
+        if (g == 0) ts.corner = (Rg.left && T.R0 >= 1) ? Rg.left[T.R0 - 1].x : 0;
+        else ts.corner = T.top ? T.top[-1].x : 0;
+        constexpr int64_t LS = lane_slot<R>();
+        if (g > 0) ts.lane_in = Rg.cb + ((g & 1) * nstr + s) * 64 * LS;
+        if (g + 1 < nseg) ts.lane_out = Rg.cb + (((g + 1) & 1) * nstr + s) * 64 * LS;
+    } else {
+        // Left column and corner H(R0-1, C0-1).
+        if (g == 0) {
+            ts.left = Rg.left;
+            ts.left_r0 = 0;
+            ts.corner = (Rg.left && T.R0 >= 1) ? Rg.left[T.R0 - 1].x : 0;
+        } else {
+            ts.left = Rg.cb + (g & 1) * rows;
+            ts.left_r0 = Rg.r0;
+            if (s == 0) ts.corner = Rg.top ? Rg.top[coff - 1].x : 0;
+            else ts.corner = Rg.cor[2 * s + (g & 1)];
+        }
+        if (s > 0 && g + 1 < nseg && wave_lane() == 0) Rg.cor[2 * s + ((g + 1) & 1)] = T.top[T.w - 1].x;
+        if (g + 1 < nseg) ts.right = Rg.cb + ((g + 1) & 1) * rows;
+    }
     ts.first_segment = g == 0;
     ts.found = Rg.found;
     return true;
 }
 
-// One wave = one tile (s, g) with s + g = d of its span; tile index x = wave number in the
-// grid.  Waves of a workgroup are independent.
+// One wave = one tile of the launch; tile index x = wave number in the grid.  Waves of a
+// workgroup are independent.
 template <int R, int MODE, bool PROF>
 __device__ void sweep_main(const SweepParams& p, SweepLds& lds) {
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform per wave
     const int64_t x = (int64_t)blockIdx.x * kSweepWaves + wid;
     TileSetup ts;
     if (!make_tile<R, MODE>(p, x, ts)) return;
-    sweep_tile<R, MODE, PROF>(p, ts.T, ts.left, ts.left_r0, ts.corner, ts.right, ts.right_r0, ts.first_segment, ts.found,
-                        lds);
+    sweep_tile<R, MODE, PROF>(p, ts, lds);
 }
 
 // ---- packed 16-bit stage 1: two tiles per wave --------------------------------------
@@ -560,12 +631,20 @@ __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileC
     }
     const uint32_t col0 = (uint32_t)(64 * c) - (uint32_t)lane;  // the lane's tile column at step 0
     const int s_end = END ? (int)(T[0].w - (64 * c - lane)) : 64;  // first step past column w - 1
+    // each tile's columns of the lane: [xlo, xlo + xlen) (see TileCtx)
+    int xlo[2], xlen[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        xlo[t] = T[t].first ? 0 : -lane;
+        xlen[t] = (int)(T[t].last ? T[t].w : T[t].w - lane) - xlo[t];
+    }
     int2* ddst[2] = {nullptr, nullptr};
     int sdump[2] = {-1, -1};
     s16x2 svH[R], svE[R];  // DUMP: each half's state at its checkpoint column, stored after the chunk
     if (DUMP) {
 #pragma unroll
-        for (int t = 0; t < 2; ++t) sdump[t] = dump_step(T[t].C0, T[t].w, T[t].Kc, T[t].scol, T[t].m, c, row0[t], ddst[t]);
+        for (int t = 0; t < 2; ++t)
+            sdump[t] = dump_step(T[t].C0, xlo[t], xlen[t], T[t].Kc, T[t].scol, T[t].m, c, row0[t], ddst[t]);
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             svH[k] = Ho[k];
@@ -584,9 +663,11 @@ __device__ __forceinline__ void chunk_steps_pk(const SweepParams& p, const TileC
         if (END && s >= s_end) continue;  // past the lane's last column (both tiles: equal widths)
         int vmask = -1;
         if (EDGE) {
-            // tile column 64c + s - lane mod 2^32: the columns below 0 compare above any width
+            // tile column 64c + s - lane mod 2^32: the columns below the lane's first compare
+            // above any width
             const uint32_t col = col0 + (uint32_t)s;
-            vmask = (col < (uint32_t)T[0].w ? 0x0000FFFF : 0) | (col < (uint32_t)T[1].w ? (int)0xFFFF0000 : 0);
+            vmask = (col - (uint32_t)xlo[0] < (uint32_t)xlen[0] ? 0x0000FFFF : 0) |
+                    (col - (uint32_t)xlo[1] < (uint32_t)xlen[1] ? (int)0xFFFF0000 : 0);
         }
         s16x2 diag = Ho_up_prev;
         Ho_up_prev = EDGE ? pk_sel(vmask, Ho_up, Ho_up_prev) : Ho_up;
@@ -672,6 +753,10 @@ __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], Sw
     const int O = p.sc.O, Ev = p.sc.E;
     TileCtx T[2] = {ts[0].T, ts[1].T};
     const int64_t row0[2] = {T[0].R0 + (int64_t)lane * R, T[1].R0 + (int64_t)lane * R};
+    constexpr int LS = lane_slot<R>();
+    const int2* lin[2];  // skewed tiles after a stripe's first: this lane's slot (see sweep_tile)
+#pragma unroll
+    for (int t = 0; t < 2; ++t) lin[t] = ts[t].lane_in ? ts[t].lane_in + (int64_t)lane * LS : nullptr;
     int a2[R];  // with symbol codes: tile 1's row profiles
     const int base[2] = {ts[0].corner, ts[1].corner};  // every value is held as value - base
     const int floor16[2] = {max(-base[0], -32768), max(-base[1], -32768)};
@@ -686,7 +771,11 @@ __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], Sw
             const int64_t row = row0[t] + k;
             const bool in = row <= T[t].r1;
             av[t] = in ? (int)T[t].A[row] : 0x200;
-            if (ts[t].left && in) {
+            if (lin[t]) {
+                const int2 v = lin[t][k];
+                hv[t] = v.x + O - base[t];
+                ev[t] = v.y - base[t];
+            } else if (ts[t].left && in) {
                 const int2 v = ts[t].left[row - ts[t].left_r0];
                 hv[t] = v.x + O - base[t];
                 ev[t] = v.y - base[t];
@@ -716,20 +805,49 @@ __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], Sw
         E[k] = pk2(ev[0], ev[1]);
         rmax[k] = zero;
     }
-    int up[2];
+    // per tile as in sweep_tile: the cell above-left of the lane's first cell, the bottom row
+    // and the column symbol of the lane's previous column
+    int up[2], hb[2], fb[2], sym[2];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
         up[t] = ts[t].corner + O - base[t];
-        if (lane > 0)
+        if (lin[t])
+            up[t] = lin[t][R].y + O - base[t];
+        else if (lane > 0)
             up[t] = ((ts[t].left && row0[t] - 1 <= T[t].r1) ? ts[t].left[row0[t] - 1 - ts[t].left_r0].x : 0) + O -
                     base[t];
+        hb[t] = O;
+        fb[t] = Ev;
+        sym[t] = PROF ? 0x0c : 0x100;
+        if (lin[t]) {
+            hb[t] = Ho[R - 1][t];
+            fb[t] = lin[t][R].x - base[t];
+            const uint32_t byte = T[t].B[T[t].C0 - lane - 1];
+            sym[t] = PROF ? (int)(((byte >> (p.code2_shift1 - 1)) & 3u) + 4 * t) : (int)byte;
+        }
     }
     s16x2 Ho_up_prev = pk2(up[0], up[1]);
-    s16x2 Ho_bot = pk2(O, O), Fe_bot = pk2(Ev, Ev);
+    s16x2 Ho_bot = pk2(hb[0], hb[1]), Fe_bot = pk2(fb[0], fb[1]);
 
-    const int64_t w_max = max(T[0].w, T[1].w);
-    const int64_t nchunks = (w_max + 63 + 63) / 64;
-    const int64_t last_interior = (min(T[0].w, T[1].w) - 64) / 64;
+    // steps and interior chunks of each tile (see sweep_tile); the wave runs the longer
+    int64_t nchunks = 0, first_interior = 0, last_interior = INT64_MAX;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int64_t nst = T[t].last ? T[t].w + 63 : T[t].w;
+        nchunks = max(nchunks, (nst + 63) / 64);
+        first_interior = max(first_interior, (int64_t)(T[t].first ? 1 : 0));
+        last_interior = min(last_interior, T[t].last ? (T[t].w >= 64 ? (T[t].w - 64) / 64 : (int64_t)-1)
+                                                     : (nst + 63) / 64 - 1);
+    }
+    // lanes start in chunk 0 of both tiles and end in neither / end in both tiles alike and
+    // start in neither: the EXEC-mask variants; otherwise per-half selects
+    const bool both_start = T[0].first && T[1].first && (!T[0].last || T[0].w >= 64) && (!T[1].last || T[1].w >= 64);
+    const bool same_end = T[0].last && T[1].last && T[0].w == T[1].w;
+    // a skewed tile before its stripe's last ends with the chunk that holds its step w - 1
+    // (w a multiple of 64); its bottom row's F + E then goes to the next segment: kept here,
+    // since a longer second tile keeps the wave stepping (its H, E and diagonal carry stay
+    // as they were: the later chunks' per-half selects leave an ended tile's state alone)
+    s16x2 end_fe = Fe_bot;
 
     auto load = [&](int64_t cc, int& qh, int& qf, int& qb) {
         int h[2], f[2], b[2];
@@ -754,27 +872,31 @@ __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], Sw
         qf = pk_bits(pk2(f[0], f[1]));
         qb = PROF ? (b[0] | 0x0c00 | (b[1] << 16) | 0x0c000000) : ((b[0] & 0xFFFF) | (b[1] << 16));
     };
-    int Qh, Qf, Qb, nh, nf, nb, bl = PROF ? 0x0c0c0c0c : 0x01000100;
+    int Qh, Qf, Qb, nh, nf, nb;
+    int bl = PROF ? (sym[0] | 0x0c00 | (sym[1] << 16) | 0x0c000000) : ((sym[0] & 0xFFFF) | (sym[1] << 16));
     load(0, Qh, Qf, Qb);
     for (int64_t c = 0; c < nchunks; ++c) {
         if (c + 1 < nchunks) load(c + 1, nh, nf, nb);
-        const bool interior = c >= 1 && c <= last_interior;
+        const bool interior = c >= first_interior && c <= last_interior;
+        const bool starts = both_start && c == 0;
+        const bool ends = same_end && c >= first_interior;
         bool dump = false;
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
-            if (MODE == kStage1 && T[t].scol && T[t].w > 0) {
-                const int64_t lo = T[t].C0 + max(64 * c - 63, (int64_t)0), hi = T[t].C0 + min(64 * c + 63, T[t].w - 2);
-                dump = dump || (lo | (T[t].Kc - 1)) <= hi;  // as in sweep_tile (last column: below)
+            if (MODE == kStage1 && T[t].scol && T[t].w > 0) {  // as in sweep_tile (each lane's last column: below)
+                const int64_t lo = T[t].C0 + max(64 * c - 63, T[t].first ? (int64_t)0 : (int64_t)-63);
+                const int64_t hi = T[t].C0 + min(64 * c + 63, T[t].w - 2);
+                dump = dump || (lo | (T[t].Kc - 1)) <= hi;
             }
         }
         if (!dump) {
             if (interior)
                 chunk_steps_pk<R, MODE, PROF, false, false>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
                                                       Qh, Qf, Qb, bl, s_slots, zero, base);
-            else if (c == 0 && min(T[0].w, T[1].w) >= 64)
+            else if (starts)
                 chunk_steps_pk<R, MODE, PROF, false, false, true>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev, Ho_bot,
                                                             Fe_bot, Qh, Qf, Qb, bl, s_slots, zero, base);
-            else if (c >= 1 && T[0].w == T[1].w)  // no column below 0, one last column for both tiles
+            else if (ends)
                 chunk_steps_pk<R, MODE, PROF, false, false, false, true>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev,
                                                                    Ho_bot, Fe_bot, Qh, Qf, Qb, bl, s_slots, zero, base);
             else
@@ -783,7 +905,7 @@ __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], Sw
         } else if (interior) {
             chunk_steps_pk<R, MODE, PROF, false, true>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev, Ho_bot, Fe_bot,
                                                  Qh, Qf, Qb, bl, s_slots, zero, base);
-        } else if (c >= 1 && T[0].w == T[1].w) {
+        } else if (ends) {
             chunk_steps_pk<R, MODE, PROF, false, true, false, true>(p, T, c, row0, a, a2, Ho, E, rmax, Ho_up_prev,
                                                               Ho_bot, Fe_bot, Qh, Qf, Qb, bl, s_slots, zero, base);
         } else {
@@ -797,12 +919,16 @@ __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], Sw
             const s16x2 oh = pk_from(o.x), of = pk_from(o.y);
 #pragma unroll
             for (int t = 0; t < 2; ++t) {
-                if (col >= 0 && col < T[t].w) {
+                if (col >= (T[t].first ? 0 : -63) && col < (T[t].last ? T[t].w : T[t].w - 63)) {
                     const int2 v = make_int2(oh[t] - O + base[t], of[t] - Ev + base[t]);
                     if (T[t].bot) T[t].bot[col] = v;
                     if (MODE == kStage1 && T[t].srow) T[t].srow[col] = v;
                 }
             }
+            // tiles ending with this chunk (skewed, before their stripe's last)
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+                if (!T[t].last && c == T[t].w / 64 - 1) end_fe = pk_sel(t ? (int)0xFFFF0000 : 0x0000FFFF, Fe_bot, end_fe);
         }
         Qh = nh;
         Qf = nf;
@@ -811,14 +937,22 @@ __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], Sw
     }
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
+        if (ts[t].lane_out) {
+            int2* lo = ts[t].lane_out + (int64_t)lane * LS;
+#pragma unroll
+            for (int k = 0; k < R; ++k) lo[k] = make_int2(Ho[k][t] - O + base[t], E[k][t] + base[t]);
+            lo[R] = make_int2(end_fe[t] + base[t], Ho_up_prev[t] - O + base[t]);
+        }
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             const int64_t row = row0[t] + k;
             if (row > T[t].r1) continue;
             if (ts[t].right) ts[t].right[row - ts[t].right_r0] = make_int2(Ho[k][t] - O + base[t], E[k][t] + base[t]);
             if (MODE != kStage1) continue;
-            if (T[t].scol && T[t].w > 0 && ((T[t].C0 + T[t].w) & (T[t].Kc - 1)) == 0)  // as in sweep_tile
-                T[t].scol[(((T[t].C0 + T[t].w) >> __builtin_ctzll((unsigned long long)T[t].Kc)) - 1) * T[t].m + row] =
+            // the lane's last column (see sweep_tile) is a checkpoint column
+            const int64_t xl = T[t].C0 + T[t].w - 1 - (T[t].last ? 0 : lane);
+            if (T[t].scol && T[t].w > 0 && ((xl + 1) & (T[t].Kc - 1)) == 0)
+                T[t].scol[(((xl + 1) >> __builtin_ctzll((unsigned long long)T[t].Kc)) - 1) * T[t].m + row] =
                     make_int2(Ho[k][t] - O + base[t], E[k][t] + base[t]);
             const int r = rmax[k][t] + base[t];
             if (ts[t].first_segment || r > T[t].rowmax[row]) {  // first segment holding the maximum
@@ -843,9 +977,9 @@ __device__ void sweep_main_pk(const SweepParams& p, SweepLds& lds) {
     if (!has0 && !has1) return;
     if (!has0) ts[0] = ts[1];
     if (!has0 || !has1) {
-        // one tile: the second is an empty one (no rows, no columns, nothing stored)
+        // one tile: the second is an empty copy of it (no rows, no inputs, nothing stored),
+        // which steps along with the same columns so that the wave keeps the fast variants
         ts[1] = ts[0];
-        ts[1].T.w = 0;
         ts[1].T.r1 = ts[1].T.R0 - 1;
         ts[1].T.top = nullptr;
         ts[1].T.bot = nullptr;
@@ -855,6 +989,8 @@ __device__ void sweep_main_pk(const SweepParams& p, SweepLds& lds) {
         ts[1].T.dirs = nullptr;
         ts[1].left = nullptr;
         ts[1].right = nullptr;
+        ts[1].lane_in = nullptr;
+        ts[1].lane_out = nullptr;
     }
     sweep_tile_pk<R, MODE, PROF>(p, ts, lds);
 }

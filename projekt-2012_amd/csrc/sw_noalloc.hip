@@ -134,6 +134,7 @@ extern "C" int swh_align_pair_device_noalloc(swh_ctx* c, const uint8_t* d1, int6
     std::vector<Job> jobs{make_job(d1, len1, d2, len2, traceback)};
     Job& j = jobs[0];
     EngineOpts eo = engine_opts(o, false);
+    eo.skew = false;  // the layout below is sized for rectangular stage-1 tiles
     const int64_t L_want = traceback ? std::max(0, eo.lookahead) : 0;
     const int64_t cap = SWH_NOALLOC_SCRATCH_BYTES;
     // checkpoint plan and lookahead: the options' spacing, or the default spacing doubled

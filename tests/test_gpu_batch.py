@@ -246,3 +246,28 @@ def test_batch_device_rounds_equal_host_rounds(sw, orc, ctx, monkeypatch, capfd)
         ref = orc.align_linear(A, B, osc, 512, 1024)
         for k in ("score", "end", "start", "cigar"):
             assert res[q][k] == ref[k], (k, q)
+
+
+@pytest.mark.parametrize("W,K,Kc,precision", [(64, 256, 64, 0), (192, 512, 256, 0), (320, 256, 128, 0),
+                                              (512, 1024, 1024, 0), (256, 256, 64, 32), (448, 512, 512, 32)])
+def test_batch_skewed_stripes(sw, orc, ctx, W, K, Kc, precision):
+    """Batches run stage 1 in skewed stripes (segments shifted left by one column per lane)
+    whenever the tile width is a multiple of 64, packed and int32: pairs of one to many
+    segments, checkpoint columns inside segments and at their ends, give the oracle's
+    alignments."""
+    pairs = _mixed_pairs(sw, 29 + W, 60, [5, 70, 300, 700, 1500, 2600])
+    res, st = ctx.batch(pairs, sw.scoring(*SC), tile_cols=W, K=K, Kc=Kc, precision=precision, xpoints=True)
+    _check(orc, pairs, res, st, K=K, Kc=Kc)
+    assert all(r["precision"] == (precision or 16) for r, (a, b) in zip(res, pairs) if a and b)
+
+
+def test_batch_skewed_equals_rectangular(sw, ctx, monkeypatch):
+    """The same batch with skewed and with rectangular stage-1 stripes: identical results."""
+    pairs = _mixed_pairs(sw, 31, 80, [40, 600, 2000, 3500])
+    skewed, st1 = ctx.batch(pairs, sw.scoring(*SC), xpoints=True)
+    monkeypatch.setenv("SWALIGN_SKEW", "0")
+    rect, st2 = ctx.batch(pairs, sw.scoring(*SC), xpoints=True)
+    assert st1 == st2 == [0] * len(pairs)
+    for x, y in zip(skewed, rect):
+        for k in ("score", "end", "start", "cigar", "xpoints"):
+            assert x[k] == y[k], k

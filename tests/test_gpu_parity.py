@@ -297,3 +297,36 @@ def test_device_inputs_end_at_last_column(sw, orc, ctx):
         finally:
             dx.free()
             dy.free()
+
+
+@pytest.mark.parametrize("m,n,K,Kc", [(3000, 2000, 512, 128), (5000, 700, 256, 64), (2600, 2600, 256, 1024)])
+def test_skewed_forward_checkpoints_bitexact(sw, orc, ctx, monkeypatch, m, n, K, Kc):
+    """Skewed stage-1 stripes (segments shifted left by one column per lane, SWALIGN_SKEW=1
+    on a single pair): score, end, row maxima, checkpoint rows and checkpoint columns equal
+    the oracle's full computation bit for bit."""
+    monkeypatch.setenv("SWALIGN_SKEW", "1")
+    (a, b), sc = sw.synth_pair(m + n + K, m, n, min(m, n) // 2, 0.1, 0.02), sw.scoring(1, -3, -5, -2)
+    A, B = orient(a, b)
+    got = ctx.forward(A, B, sc, K, Kc)
+    osc = orc.scoring(1, -3, -5, -2)
+    (s, i, j), rowmax = orc.forward(A, B, osc, rowmax=True)
+    srow, scol = orc.checkpoints(A, B, osc, K, Kc)
+    assert got["score"] == s and got["end"] == (i, j)
+    assert np.array_equal(got["rowmax"], rowmax)
+    assert np.array_equal(got["srow"], srow)
+    assert np.array_equal(got["scol"], scol)
+
+
+@pytest.mark.parametrize("W,Kc", [(64, 64), (128, 1024), (192, 128), (256, 256), (320, 64)])
+def test_skewed_single_pairs(sw, orc, ctx, monkeypatch, W, Kc):
+    """Single pairs with skewed stage-1 stripes of several segment widths (multiples of 64)
+    and checkpoint-column spacings, so that checkpoint columns fall in a segment's first,
+    interior and last chunks and at each lane's own step: the oracle's alignment."""
+    monkeypatch.setenv("SWALIGN_SKEW", "1")
+    for k, (m, n) in enumerate([(1500, 1400), (900, 2100), (4100, 300)]):
+        (a, b), sc = sw.synth_pair(W * 7 + Kc + k, m, n, min(m, n) // 2, 0.1, 0.02), sw.scoring(1, -3, -5, -2)
+        got = ctx.align(a, b, sc, K=256, Kc=Kc, xpoints=True, tile_cols=W, precision=32)
+        A, B = orient(a, b)
+        ref = orc.align_linear(A, B, orc.scoring(1, -3, -5, -2), 256, Kc)
+        _cmp(got, ref)
+        assert got["xpoints"] == [tuple(x) for x in ref["xpoints"]]
