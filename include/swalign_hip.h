@@ -73,8 +73,8 @@ typedef struct {
                                /* each direction per round (beyond 3, only blocks within 2 of  */
                                /* the diagonal); 0 = auto (7; batches: per round, as many as  */
                                /* the walking pairs' blocks leave room for), -1 = none        */
-    int32_t waves_per_simd;    /* sweep waves resident per SIMD (1 to 4), 0 = auto (batches    */
-                               /* and the request server 4; single pairs 1 or 2 by a model)    */
+    int32_t waves_per_simd;    /* sweep waves resident per SIMD (1 to 4), 0 = auto (batches 3,  */
+                               /* the request server 4; single pairs 1 or 2 by a model)        */
     int32_t autotune;          /* batch: 1 = time candidate settings on a sample first          */
     int32_t host_checkpoints;  /* 1: checkpoint rows and columns live in pinned host memory    */
                                /* mapped into the device (the reference's temp/row_R side      */

@@ -418,10 +418,14 @@ constexpr int64_t kBatchTileCols = 512;
 // spacing that is a multiple of the tile width costs stage 1 nothing per step (65,536-pair
 // batch, W = 512: 256 x 512 723 ms a batch, 512 x 512 744, 256 x 256 751, 1024 x 1024 892).
 constexpr int64_t kBatchCkptRows = 256, kBatchCkptCols = 512;
-// Sweep waves per SIMD of a batch: a batch has tiles to spare, and four waves per SIMD hide
+// Sweep waves per SIMD of a batch: a batch has tiles to spare, and several waves per SIMD hide
 // the per-tile set-up and memory waits that one or two leave exposed (65,536-pair batch,
-// forward sweep: 1210 / 763 / 668 / 650 ms at 1 / 2 / 3 / 4 waves per SIMD).
-constexpr int kBatchWps = 4;
+// forward sweep: 1210 / 763 / 668 / 650 ms at 1 / 2 / 3 / 4 waves per SIMD, rectangular
+// tiles).  The packed stage 1 of skewed stripes holds 139 VGPRs: three waves fit a SIMD.
+constexpr int kBatchWps = 3;
+// A tuned candidate replaces the batch defaults only when it is faster than this fraction of
+// the defaults' time on the sample.
+constexpr double kTuneMargin = 0.98;
 
 // Runs stage 1, the end search and (for jobs that want it) the traceback of every job,
 // all jobs side by side in each sweep.  Fills out[k] (result_init'ed by the caller) and
@@ -1547,6 +1551,16 @@ int tune_jobs(swh_ctx* c, const std::vector<Job>& jobs, const swh_scoring* sc, c
                 }
             }
         }
+    }
+    // A sample times candidates within ~1% of each other that differ by several percent on
+    // the whole batch (its launches hold fewer tiles, so the longest pairs' critical path
+    // weighs more): the default stays unless a candidate beats it clearly.
+    if (out->ms_default > 0 && best > kTuneMargin * out->ms_default) {
+        out->tile_cols = kBatchTileCols;
+        out->waves_per_simd = usable_wps(c, kBatchWps);
+        out->checkpoint_rows = kBatchCkptRows;
+        out->checkpoint_cols = kBatchCkptCols;
+        out->ms_best = out->ms_default;
     }
     out->candidates = cand;
     return cand ? SWH_OK : SWH_ERR_BUDGET;
