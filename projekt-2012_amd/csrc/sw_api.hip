@@ -129,7 +129,71 @@ double sweep_model(swh_ctx* c, int64_t rows, int64_t cols, int wps, bool skew) {
     return t * (wps == 2 ? kTwoWaveCost : 1.0);
 }
 
+// Occupancy model of a single pair's stage 1 in microseconds, used to choose between the
+// int32 sweep and the packed one.  Each launch (tile anti-diagonal) spreads its waves (one
+// per int32 tile, one per two packed tiles) over the SIMDs; a SIMD holding q waves runs
+// their steps at kStepCost[q] times one int32 wave's step, and each round of waves on a
+// SIMD adds a fixed kRoundUs (loads before the first step, stores after the last).  Fitted
+// on MI355X to 38 measured stage-1 times of square pairs of 1, 2, 3 and 10 Mbp (tile widths
+// 64 to 4883, one to four waves per SIMD, both precisions): within 10% for packed widths of
+// 128 and more.
+constexpr double kStepUs = 0.091;
+constexpr double kRoundUs = 5.2;
+constexpr double kStepCostI32[kMaxSweepWps + 1] = {0.0, 1.0, 1.772, 2.753, 3.215};
+// (packed waves hold 139 VGPRs: a fourth on a SIMD waits for one of three to end)
+constexpr double kStepCostPk[kMaxSweepWps + 1] = {0.0, 1.625, 2.550, 3.653, 3.653 * 4 / 3};
+// the packed sweep replaces the int32 one for a single pair only when the model expects it
+// to take less than this fraction of the time (1 Mbp pair: 0.97, measured 1.04; 2 Mbp: 0.85,
+// measured 0.87), and for pairs of at least kPackedPairMinLen on each side
+constexpr double kPackedPairMargin = 0.95;
+constexpr int64_t kPackedPairMinLen = INT64_C(1) << 19;
+
+double occupancy_model(swh_ctx* c, int64_t rows, int64_t cols, int64_t W, int wps, bool skew, bool packed) {
+    const int64_t S = ceil_div(rows, kStripeRows), G = ceil_div(cols, W);
+    const int64_t simds = (int64_t)kSweepWaves * std::max(1, c->props.multiProcessorCount);
+    skew = skew && G >= 2;
+    const int64_t D = skew ? 2 * (S - 1) + G : S + G - 1;
+    const double st = (double)(skew ? W : W + 63) * kStepUs;
+    const double* T = packed ? kStepCostPk : kStepCostI32;
+    double t = 0;
+    for (int64_t d = 0; d < D; ++d) {
+        const int64_t tiles = skew ? std::min(S - 1, d / 2) - std::max<int64_t>(0, (d - G + 2) / 2) + 1
+                                   : std::min(d, S - 1) - std::max<int64_t>(0, d - (G - 1)) + 1;
+        const int64_t q = ceil_div(packed ? ceil_div(tiles, 2) : tiles, simds);
+        const int64_t full = q / wps, rest = q % wps;
+        t += (double)full * (st * T[wps] + kRoundUs) + (rest ? st * T[rest] + kRoundUs : 0.0);
+    }
+    return t;
+}
+
 int usable_wps(const swh_ctx* c, int w) { return w >= 1 && w <= kMaxSweepWps && c->lds_pad[w] ? w : 1; }
+
+// Packed skewed stripes for a single pair: the fastest modelled of segment widths that put
+// about one, one half or two packed waves per SIMD slot on the widest diagonal (two or three
+// waves per SIMD, widths of whole chunks from 128 up to the int16 bound).  Returns the
+// modelled time, or a negative value when no width fits.
+double packed_pair_choice(swh_ctx* c, int64_t rows, int64_t cols, int64_t w16, int wps_req, int64_t* W_out,
+                          int* wps_out) {
+    const int64_t simds = (int64_t)kSweepWaves * std::max(1, c->props.multiProcessorCount);
+    const int64_t wmax = (w16 - 63) / 64 * 64;
+    double best = -1.0;
+    for (int wps = wps_req ? wps_req : 2; wps <= (wps_req ? wps_req : 3); ++wps) {
+        const int wu = usable_wps(c, wps);
+        // the widest skewed diagonal holds G / 2 tiles: G = 4 * simds * wps fills every slot
+        const int64_t base = ceil_div(cols, 4 * simds * wu);
+        for (int64_t f2 : {1, 2, 4}) {  // half, one and two times the filling width
+            const int64_t W = std::min(wmax, std::max<int64_t>(128, ceil_div(base * f2 / 2, 64) * 64));
+            if (W < 128 || ceil_div(cols, W) < 2) continue;
+            const double t = occupancy_model(c, rows, cols, W, wu, true, true);
+            if (best < 0 || t < best) {
+                best = t;
+                *W_out = W;
+                *wps_out = wu;
+            }
+        }
+    }
+    return best;
+}
 
 bool many_walks(const swh_ctx* c, int64_t njobs) {
     const int64_t per_cu = (int64_t)c->props.maxSharedMemoryPerMultiProcessor / (int64_t)walk_lds_bytes();
@@ -679,11 +743,9 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
         live.push_back(k);
     }
     if (live.empty()) return SWH_OK;
-    // packed 16-bit stage 1 when no pair of the sweep can leave int16 (two tiles per wave)
-    // packed 16-bit stage 1 when every tile's values fit int16 around the tile's base: by
-    // default for batches; single pairs only on request (precision 16), since on MI355X the
-    // int32 sweep of one pair is as fast or faster (one wave per SIMD, wider tiles, shorter
-    // wavefront ramp: 715 vs 772 ms at 1 Mbp, 35.5 vs 45.8 s at 10 Mbp)
+    // packed 16-bit stage 1 (two tiles per wave) when every tile's values fit int16 around
+    // the tile's base: by default for batches, for single pairs on request (precision 16) or
+    // where the occupancy model prefers it (below)
     const int64_t w16 = int16_tile_cols(sc);
     bool packed = (eo.precision == 16 || (eo.precision == 0 && eo.batch)) && w16 > 0;
     int wps = 1;
@@ -696,32 +758,66 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
         W = eo.tile_cols > 0 ? eo.tile_cols : kBatchTileCols;
         skew = skew && W % 64 == 0 && (!packed || W + 63 <= w16);
         packed = packed && (skew ? W + 63 : W) <= w16;
-    } else if (skew && !packed &&
-               (eo.skew_forced || eo.tile_cols > 0 ||
-                sweep_model(c, max_m, max_n, choose_wps(c, max_m, max_n, true), true) <
-                    sweep_model(c, max_m, max_n, choose_wps(c, max_m, max_n, false), false))) {
-        // a skewed stripe's launches hold tiles of every other diagonal: segments of half the
-        // width keep the resident waves busy.  Skewed when the model says so: the 1 Mbp pair
-        // (4.58 M against 5.02 M modelled tile steps; 470 against 531 ms measured), not the 10 Mbp
-        // pair (205.0 M against 203.3 M; twice the launches for 1.3% less filling and draining)
-        wps = eo.tile_cols > 0 ? 1 : choose_wps(c, max_m, max_n, true);
-        W = eo.tile_cols > 0 ? eo.tile_cols : pair_tile_cols(c, max_n, wps, true);
-        skew = W % 64 == 0;
     } else {
-        skew = false;
-        if (packed) {  // two tiles per wave, one wave per SIMD
-            W = tile_cols(c, max_n, eo.tile_cols, 2);
-            if (W > w16 && eo.tile_cols > 0) {
-                packed = false;  // the requested width does not fit int16
-            } else if (W > w16) {
-                W = w16;  // narrower tiles: more of them per launch, two waves per SIMD if needed
-                const int64_t slots = (int64_t)kSweepWaves * std::max(1, c->props.multiProcessorCount);
-                if (ceil_div(max_n, W) > 2 * slots && c->lds_pad[2]) wps = 2;
+        // single pairs: packed skewed stripes on request (precision 16), or by default where
+        // the occupancy model expects them to beat the int32 sweep by a margin (long pairs)
+        int64_t Wp = 0;
+        int wp = 1;
+        bool pk_skew = false;
+        if (skew && w16 > 0 && eo.precision != 32) {
+            if (eo.tile_cols > 0) {
+                pk_skew = packed && eo.tile_cols % 64 == 0 && eo.tile_cols + 63 <= w16;
+                Wp = eo.tile_cols;
+                wp = usable_wps(c, eo.wps ? eo.wps : 2);
+            } else if (packed || std::min(max_m, max_n) >= kPackedPairMinLen) {
+                const double tp = packed_pair_choice(c, max_m, max_n, w16, eo.wps, &Wp, &wp);
+                if (tp >= 0 && packed) {
+                    pk_skew = true;
+                } else if (tp >= 0) {
+                    const int ws = eo.wps ? usable_wps(c, eo.wps) : choose_wps(c, max_m, max_n, true);
+                    const int wr = eo.wps ? usable_wps(c, eo.wps) : choose_wps(c, max_m, max_n, false);
+                    const bool s32 = sweep_model(c, max_m, max_n, ws, true) < sweep_model(c, max_m, max_n, wr, false);
+                    const double ti =
+                        s32 ? occupancy_model(c, max_m, max_n, pair_tile_cols(c, max_n, ws, true), ws, true, false)
+                            : occupancy_model(c, max_m, max_n, pair_tile_cols(c, max_n, wr, false), wr, false, false);
+                    pk_skew = tp < kPackedPairMargin * ti;
+                }
             }
         }
-        if (!packed) {
-            wps = eo.tile_cols > 0 ? 1 : choose_wps(c, max_m, max_n, false);
-            W = tile_cols(c, max_n, eo.tile_cols, wps);
+        if (pk_skew) {
+            // two tiles per wave in skewed stripes (10 Mbp pair: 22.4 s against 32.0 s for the
+            // int32 rectangular sweep; 3 Mbp: 2.58 against 3.31 s; 1 Mbp stays int32: 470 ms
+            // against 489 ms)
+            packed = true;
+            W = Wp;
+            wps = wp;
+        } else if (skew && !packed &&
+                   (eo.skew_forced || eo.tile_cols > 0 ||
+                    sweep_model(c, max_m, max_n, choose_wps(c, max_m, max_n, true), true) <
+                        sweep_model(c, max_m, max_n, choose_wps(c, max_m, max_n, false), false))) {
+            // a skewed stripe's launches hold tiles of every other diagonal: segments of half
+            // the width keep the resident waves busy.  Skewed when the model says so: the 1 Mbp
+            // pair (4.58 M against 5.02 M modelled tile steps; 470 against 531 ms measured)
+            wps = eo.wps ? usable_wps(c, eo.wps) : eo.tile_cols > 0 ? 1 : choose_wps(c, max_m, max_n, true);
+            W = eo.tile_cols > 0 ? eo.tile_cols : pair_tile_cols(c, max_n, wps, true);
+            skew = W % 64 == 0;
+        } else {
+            skew = false;
+            if (packed) {  // two tiles per wave, one wave per SIMD
+                W = tile_cols(c, max_n, eo.tile_cols, 2);
+                if (W > w16 && eo.tile_cols > 0) {
+                    packed = false;  // the requested width does not fit int16
+                } else if (W > w16) {
+                    W = w16;  // narrower tiles: more of them per launch, two waves per SIMD if needed
+                    const int64_t slots = (int64_t)kSweepWaves * std::max(1, c->props.multiProcessorCount);
+                    if (ceil_div(max_n, W) > 2 * slots && c->lds_pad[2]) wps = 2;
+                }
+                if (eo.wps) wps = usable_wps(c, eo.wps);
+            }
+            if (!packed) {
+                wps = eo.wps ? usable_wps(c, eo.wps) : eo.tile_cols > 0 ? 1 : choose_wps(c, max_m, max_n, false);
+                W = tile_cols(c, max_n, eo.tile_cols, wps);
+            }
         }
     }
     {  // symbol codes for the int32 sweeps, from the bytes of the pairs
