@@ -73,19 +73,21 @@ TRAFFIC_PROFILES = {1: "k_sweep_stage1_1Mbp_pair", 2: "k_sweep_stage1_10Mbp_pair
 
 def measured_traffic(workload, codes=1, precision=32):
     """HBM bytes per stage-1 launch from the committed FETCH_SIZE / WRITE_SIZE profiles of
-    this workload, each scaled by what the counter reads for a 256 MiB copy."""
+    this workload, each scaled by what the counter reads for a 256 MiB copy: (fetched plus
+    written bytes or None, fetched bytes or None, source files or None)."""
     if workload not in TRAFFIC_PROFILES:
-        return None, None
+        return None, None, None
     name = TRAFFIC_PROFILES[workload]
     k = "k_sweep%s<4, 0, %s>" % ("_pk" if precision == 16 else "", "true" if codes else "false")
     fetch = _counter_avg(name + "_fetch_size.txt", "FETCH_SIZE", k)
     write = _counter_avg(name + "_write_size.txt", "WRITE_SIZE", k)
     cf = _counter_avg("copy_256MiB_fetch_size.txt", "FETCH_SIZE", "copyBuffer")
     cw = _counter_avg("copy_256MiB_write_size.txt", "WRITE_SIZE", "copyBuffer")
-    if None in (fetch, write, cf, cw):
-        return None, None
     kib = 268435456 / 1024.0
-    return int((fetch * kib / cf + write * kib / cw) * 1024), "profiles/%s_{fetch,write}_size.txt" % name
+    fetched = int(fetch * kib / cf * 1024) if None not in (fetch, cf) else None
+    if fetched is None or None in (write, cw):
+        return None, fetched, ("profiles/%s_fetch_size.txt" % name if fetched is not None else None)
+    return fetched + int(write * kib / cw * 1024), fetched, "profiles/%s_{fetch,write}_size.txt" % name
 
 
 def algorithmic_bytes(cells, n_cols_per_tile, K, Kc):
@@ -102,10 +104,12 @@ def roofline(cells, fwd_ms, launches, workload=None, W=None, K=None, Kc=None, pr
     packed int16 pairs at twice the operation rate)."""
     achieved = cells * OPS_PER_CELL / (fwd_ms / 1000.0) / 1e12
     peak = PEAK_INT16X2_TOPS if precision == 16 else PEAK_INT32_TOPS
-    traffic, source = measured_traffic(workload, codes, precision)
+    traffic, fetched, source = measured_traffic(workload, codes, precision)
     extra = {}
     if W and K and Kc:
         extra["algorithmic_bytes_per_launch"] = int(algorithmic_bytes(cells, W, K, Kc) / max(launches, 1))
+    if traffic is None and fetched is not None:
+        extra["fetched_bytes_per_launch"] = fetched  # no WRITE_SIZE profile of this workload
     if source:
         extra["traffic_source"] = source
     return {
@@ -206,17 +210,10 @@ def pair_record(torch, sw, workload, steps, warmup, K=None):
             "device": info["name"], "cus": info["cus"], "clock_mhz": round(clock_ghz * 1000),
         },
         "roofline": roofline(cells, fwd_ms, r0["forward_launches"], workload, precision=r0["precision"],
-                             codes=r0["score_codes"], W=tile_width(m, n, r0["forward_launches"]),
+                             codes=r0["score_codes"], W=r0["forward_tile_cols"],
                              K=r0["checkpoint_rows"] or m, Kc=r0["checkpoint_cols"] or n),
     }
     return rec, (a, b)
-
-
-def tile_width(m, n, launches):
-    """Stage-1 tile width of a single-pair sweep from its launch count: one launch per tile
-    anti-diagonal, S + G - 1 launches for S 256-row stripes and G segments."""
-    G = max(1, launches - (-(-m // 256)) + 1)
-    return -(-n // G)
 
 
 def device_pairs(torch, pairs):

@@ -65,7 +65,8 @@ typedef struct {
     int32_t score_only;        /* 1: score + end cell only (no traceback)                   */
     int32_t precision;         /* stage 1: 16 = packed int16 (values relative to each tile's  */
                                /* base) where a tile's range fits int16, else int32; 32 = int32;*/
-                               /* 0 = 16 for batches, 32 for single pairs (faster there)       */
+                               /* 0 = 16 for batches; single pairs 16 where a model expects it */
+                               /* faster (about 1.5 Mbp and more on each side), else 32        */
     int32_t want_xpoints;      /* 1: return the path nodes on checkpoint rows                */
     int32_t tile_cols;         /* columns per sweep tile (any width >= 1);                     */
                                /* 0 = auto (about four tiles per CU across the row, >= 256)   */
@@ -74,7 +75,7 @@ typedef struct {
                                /* the diagonal); 0 = auto (7; batches: per round, as many as  */
                                /* the walking pairs' blocks leave room for), -1 = none        */
     int32_t waves_per_simd;    /* sweep waves resident per SIMD (1 to 4), 0 = auto (batches 3,  */
-                               /* the request server 4; single pairs 1 or 2 by a model)        */
+                               /* the request server 4; single pairs 1 to 3 by a model)        */
     int32_t autotune;          /* batch: 1 = time candidate settings on a sample first          */
     int32_t host_checkpoints;  /* 1: checkpoint rows and columns live in pinned host memory    */
                                /* mapped into the device (the reference's temp/row_R side      */
@@ -113,7 +114,7 @@ typedef struct {
     int32_t score_codes;      /* 1: the int32 sweeps took substitution scores from
                                  per-row symbol-code profiles; 0: from byte compares
                                  (always 0 from the request server)               */
-    int32_t reserved0;
+    int32_t forward_tile_cols; /* columns per stage-1 tile (segment of a skewed stripe) */
 } swh_result;
 
 typedef struct swh_ctx swh_ctx;

@@ -868,6 +868,7 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
         j.bi = keys[k] ? (int64_t)(0xFFFFFFFFu - (uint32_t)(keys[k] & 0xFFFFFFFFu)) : -1;
         out[k].score = j.best > 0 ? j.best : 0;
         out[k].forward_launches = (int32_t)launches;
+        out[k].forward_tile_cols = (int32_t)W;
         out[k].ms_forward = ms_fwd;
         out[k].ms_total = ms_fwd;
         if (j.best > 0) scored.push_back(k);

@@ -52,7 +52,7 @@ class Result(C.Structure):
                 ("ms_forward", C.c_double), ("ms_end_search", C.c_double),
                 ("ms_traceback", C.c_double), ("ms_total", C.c_double),
                 ("precision_used", C.c_int32), ("forward_launches", C.c_int32),
-                ("score_codes", C.c_int32), ("reserved0", C.c_int32)]
+                ("score_codes", C.c_int32), ("forward_tile_cols", C.c_int32)]
 
 
 class Pair(C.Structure):
@@ -137,7 +137,8 @@ def _result_dict(r):
              checkpoint_rows=r.checkpoint_rows, checkpoint_cols=r.checkpoint_cols,
              ms_forward=r.ms_forward, ms_end_search=r.ms_end_search, ms_traceback=r.ms_traceback,
              ms_total=r.ms_total, precision=r.precision_used,
-             forward_launches=r.forward_launches, score_codes=r.score_codes)
+             forward_launches=r.forward_launches, score_codes=r.score_codes,
+             forward_tile_cols=r.forward_tile_cols)
     if r.xpoints and r.n_xpoints:
         d["xpoints"] = [(r.xpoints[k].row, r.xpoints[k].col, r.xpoints[k].score, r.xpoints[k].layer)
                         for k in range(r.n_xpoints)]
