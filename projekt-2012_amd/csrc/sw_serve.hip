@@ -227,6 +227,10 @@ struct swh_server {
     int64_t sweep_launches = 0, shared_launches = 0;  // sweep launches; those covering two or more jobs
     int64_t fail_tick = -1;         // test hook (SWALIGN_SERVE_FAIL_TICK): tick that reports a failure
     int64_t ticks = 0;
+    // host time of the loop (SWALIGN_TRACE prints it when the server is destroyed): job set-up,
+    // tick planning and launches, waiting for read-backs, advancing the jobs
+    double ms_setup = 0, ms_launch = 0, ms_wait = 0, ms_advance = 0;
+    int64_t nticks = 0, nreadbacks = 0;
     bool loop_exited = false;
     std::vector<double> lat_ms;
     Clock::time_point t_first{}, t_last{};
@@ -502,6 +506,8 @@ constexpr int kDiagonalsPerTick = 8;
 
 int tick(swh_server* s) {
     swh_ctx* c = s->ctx;
+    const auto t_tick = Clock::now();
+    ++s->nticks;
     if (s->fail_tick >= 0 && s->ticks++ == s->fail_tick) return SWH_ERR_INTERNAL;  // injected from the host
     TickSlot& slot = s->ring[s->ring_at];
     s->ring_at = (s->ring_at + 1) % kTickRing;
@@ -662,11 +668,16 @@ int tick(swh_server* s) {
             if (nx) s->rb.push_back(swh_server::Rb{(const uint8_t*)j->xps, nx * (int64_t)sizeof(XPoint), j->xps_h.data()});
         }
     }
+    const auto t_launched = Clock::now();
+    s->ms_launch += std::chrono::duration<double, std::milli>(t_launched - t_tick).count();
     if (readback) {
         const int rc = run_readbacks(s);
         if (rc) return rc;
         for (TickSlot& t : s->ring) t.pending = false;
+        ++s->nreadbacks;
     }
+    const auto t_read = Clock::now();
+    s->ms_wait += std::chrono::duration<double, std::milli>(t_read - t_launched).count();
     // ---- advance ------------------------------------------------------------------------
     for (SJob* j : s->flight) {
         switch (j->phase) {
@@ -737,6 +748,7 @@ int tick(swh_server* s) {
                 break;
         }
     }
+    s->ms_advance += std::chrono::duration<double, std::milli>(Clock::now() - t_read).count();
     return SWH_OK;
 }
 
@@ -788,6 +800,7 @@ void serve_loop(swh_server* s) {
             }
         }
         int rc = SWH_OK;
+        const auto t_setup = Clock::now();
         for (SJob* j : s->flight) {
             if (j->phase != kQueued) continue;
             place(s, *j);
@@ -803,6 +816,7 @@ void serve_loop(swh_server* s) {
             if (j->m == 0 || j->n == 0) finish(s, *j, SWH_OK);
             else setup_stage1(s, *j);
         }
+        s->ms_setup += std::chrono::duration<double, std::milli>(Clock::now() - t_setup).count();
         if (!rc) rc = tick(s);
         std::lock_guard<std::mutex> lk(s->mu);
         if (rc) {
@@ -1014,6 +1028,10 @@ int swh_server_destroy(swh_server* s) {
     }
     s->cv_work.notify_all();
     if (s->loop.joinable()) s->loop.join();
+    if (s->ctx->trace)
+        fprintf(stderr, "swalign: server: %lld ticks (%lld with read-backs); host ms: set-up %.1f, planning and "
+                "launches %.1f, waiting for read-backs %.1f, advancing %.1f\n", (long long)s->nticks,
+                (long long)s->nreadbacks, s->ms_setup, s->ms_launch, s->ms_wait, s->ms_advance);
     for (auto& kv : s->jobs) {
         swh_result_free(&kv.second->res);
         delete kv.second;
