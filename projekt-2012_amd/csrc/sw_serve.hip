@@ -6,13 +6,15 @@
 // blocks) covering the current tile anti-diagonal of every job in that mode -- jobs
 // admitted at different times sit at different diagonals of the same launch -- plus one
 // argmax launch for jobs whose stage 1 just ended and one walk launch for jobs whose
-// traceback blocks are ready.  One synchronisation per tick reads back the few bytes that
-// decide each job's next step.  A job's work therefore overlaps with every other job in
-// flight, and jobs join and leave between ticks.  No kernel waits on another workgroup.
+// traceback blocks are ready.  One read-back per tick brings the few bytes that decide each
+// job's next step; before waiting for it the tick also launches the next tick's diagonals of
+// every job whose sweep continues, so that the device sweeps them while the host waits and
+// plans.  A job's work therefore overlaps with every other job in flight, and jobs join and
+// leave between ticks.  No kernel waits on another workgroup.
 //
-// Each job's device memory (sequences, row maxima, checkpoints, tile hand-off buffers,
-// direction codes, ops) is one block of a pool allocated once at creation; admission is
-// bounded by the pool (the HBM budget).
+// Each job's device memory (sequences, row maxima, checkpoints, its sweeps' regions, tile
+// hand-off buffers, direction codes, ops) is one block of a pool allocated once at creation;
+// admission is bounded by the pool (the HBM budget).
 #include <algorithm>
 #include <chrono>
 #include <climits>
@@ -122,15 +124,22 @@ hipError_t ensure_pin(PinBuf& b, size_t bytes) {
 
 int64_t al16(int64_t x) { return (x + 15) & ~INT64_C(15); }
 
-// Device and host copies of one tick's sweep descriptors.  Ticks that need no read-back
-// are not synchronised, so the host runs up to kTickRing ticks ahead of the device; a
-// slot is reused once the event recorded after its launches has completed.
-struct TickSlot {
-    DevBuf regs[3], spans[3];
-    PinBuf up_h;   // this tick's uploads: data, then the copy list
-    DevBuf up_d;
-    std::vector<Region> hregs[3];
+// Device and host copies of one set of sweep launches' descriptors.
+struct SweepSet {
+    DevBuf spans[3];
     std::vector<TileSpan> hspans[3];
+};
+
+// One tick's descriptors: the sweeps of jobs whose diagonals were not launched ahead, and
+// the sweeps launched ahead for the next tick.  Ticks that need no read-back are not
+// synchronised, so the host runs up to kTickRing ticks ahead of the device; a slot is
+// reused once the event recorded after its launches has completed.
+struct TickSlot {
+    SweepSet now, ahead;
+    // this tick's uploads: data, then the copy list (pageable: on this platform small
+    // pageable copies delayed the tick's sweeps less than copies from pinned memory)
+    std::vector<uint8_t> up_h;
+    DevBuf up_d;
     hipEvent_t done = nullptr;
     bool pending = false;
 };
@@ -159,12 +168,15 @@ struct SJob {
     int64_t xcap = 0;
     uint8_t* work = nullptr;  // per-phase hand-off buffers and direction codes
     int64_t work_bytes = 0;
+    int64_t max_regs = 1;     // regions of the job's largest sweep
+    int64_t reg_at = 0;       // its regions, in the pool seen as an array of Region
     // schedule
     Phase phase = kQueued;
     int mode = kStage1;
     std::vector<Region> regs;
     std::vector<int64_t> Sr, Gr;
     int64_t D = 0, d = 0;
+    int64_t launched_to = 0;  // diagonals of the current sweep launched so far (may run ahead of d)
     int L = 0;                      // traceback: checkpoint blocks recomputed ahead per direction
     std::vector<WalkRegion> wtab;   // the round's (L+1)^2 blocks, rows == 0 where none
     std::vector<int> wtab_reg;      // region of each block (-1 where none)
@@ -211,6 +223,7 @@ struct swh_server {
     };
     std::vector<Rb> rb;
     PinBuf rb_h, rbx_h;
+    hipEvent_t rb_done = nullptr;  // recorded after a tick's read-back transfer
     DevBuf rb_d, rbx_d;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -282,6 +295,10 @@ void layout(swh_server* s, SJob& j) {
     while (j.L > 0 && lookahead_blocks(j.L) * blk > la_cap) --j.L;
     const int64_t tb = lookahead_blocks(j.L) * blk;
     j.work_bytes = std::max({stage1, search, tb});
+    // the regions of the job's sweeps stay in its block (one more Region of room to align
+    // them to a multiple of sizeof(Region) from the pool's start)
+    j.max_regs = std::max<int64_t>({1, nq, lookahead_blocks(j.L)});
+    persist += al((int64_t)sizeof(Region) * (j.max_regs + 1));
     j.bytes = persist + j.work_bytes;
 }
 
@@ -290,6 +307,18 @@ void queue_value(swh_server* s, void* dst, const void* v, int64_t len) {
     s->up_vals.insert(s->up_vals.end(), (const uint8_t*)v, (const uint8_t*)v + len);
     s->up.push_back(swh_server::Up{(uint8_t*)dst, nullptr, len, at});
 }
+void finish(swh_server*, SJob& j, int status);
+
+// The job's current sweep regions, copied to its block at the start of the next tick (its
+// layout holds max_regs of them; a job whose sweep would need more fails instead).
+void upload_regions(swh_server* s, SJob& j) {
+    if ((int64_t)j.regs.size() > j.max_regs) {
+        finish(s, j, SWH_ERR_INTERNAL);
+        return;
+    }
+    queue_value(s, (Region*)s->pool_base + j.reg_at, j.regs.data(), (int64_t)(sizeof(Region) * j.regs.size()));
+}
+
 void queue_ref(swh_server* s, void* dst, const void* host, int64_t len) {
     if (len > 0) s->up.push_back(swh_server::Up{(uint8_t*)dst, (const uint8_t*)host, len, 0});
 }
@@ -309,18 +338,18 @@ int flush_uploads(swh_server* s, TickSlot& slot) {
     int64_t nx = 0;
     for (const auto& u : s->up) nx += ceil_div(u.len, kXferChunk);
     const int64_t total = data + nx * (int64_t)sizeof(Xfer);
-    HCK(c, ensure_pin(slot.up_h, (size_t)total));
+    slot.up_h.resize((size_t)total);
     HCK(c, ensure(slot.up_d, (size_t)total));
     std::vector<Xfer> xs;
     xs.reserve((size_t)nx);
     int64_t off = 0;
     for (const auto& u : s->up) {
-        memcpy(slot.up_h.h + off, u.host ? u.host : s->up_vals.data() + u.val, (size_t)u.len);
+        memcpy(slot.up_h.data() + off, u.host ? u.host : s->up_vals.data() + u.val, (size_t)u.len);
         add_xfer(xs, u.dst, (const uint8_t*)slot.up_d.p + off, u.len);
         off += al16(u.len);
     }
-    memcpy(slot.up_h.h + data, xs.data(), sizeof(Xfer) * xs.size());
-    HCK(c, hipMemcpyAsync(slot.up_d.p, slot.up_h.h, (size_t)total, hipMemcpyHostToDevice, c->stream));
+    memcpy(slot.up_h.data() + data, xs.data(), sizeof(Xfer) * xs.size());
+    HCK(c, hipMemcpyAsync(slot.up_d.p, slot.up_h.data(), (size_t)total, hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(k_copy_list, dim3((unsigned)xs.size()), dim3(256), 0, c->stream,
                        (const Xfer*)((const uint8_t*)slot.up_d.p + data));
     HCK(c, hipGetLastError());
@@ -329,8 +358,10 @@ int flush_uploads(swh_server* s, TickSlot& slot) {
     return SWH_OK;
 }
 
-// Gathers this tick's read-backs (one kernel, one transfer) and waits for them.
-int run_readbacks(swh_server* s) {
+// Gathers this tick's read-backs (one kernel, one transfer) and records an event after
+// them; finish_readbacks waits for it (not for later work on the stream) and copies the
+// values to their destinations.
+int issue_readbacks(swh_server* s) {
     swh_ctx* c = s->ctx;
     int64_t data = 0;
     for (const auto& r : s->rb) data += al16(r.len);
@@ -351,8 +382,14 @@ int run_readbacks(swh_server* s) {
         HCK(c, hipGetLastError());
         HCK(c, hipMemcpyAsync(s->rb_h.h, s->rb_d.p, (size_t)data, hipMemcpyDeviceToHost, c->stream));
     }
-    HCK(c, hipStreamSynchronize(c->stream));
-    off = 0;
+    HCK(c, hipEventRecord(s->rb_done, c->stream));
+    return SWH_OK;
+}
+
+int finish_readbacks(swh_server* s) {
+    swh_ctx* c = s->ctx;
+    HCK(c, hipEventSynchronize(s->rb_done));
+    int64_t off = 0;
     for (const auto& r : s->rb) {
         memcpy(r.host, s->rb_h.h + off, (size_t)r.len);
         off += al16(r.len);
@@ -379,6 +416,10 @@ void place(swh_server* s, SJob& j) {
     j.walk = (WalkState*)take(sizeof(WalkState));
     j.ops = take(j.m + j.n + 16);
     j.xps = (XPoint*)take((int64_t)sizeof(XPoint) * j.xcap);
+    if (j.m > 0 && j.n > 0) {
+        const int64_t at = (int64_t)(take((int64_t)sizeof(Region) * (j.max_regs + 1)) - s->pool_base);
+        j.reg_at = ceil_div(at, (int64_t)sizeof(Region));
+    }
     j.work = take(j.work_bytes);
 }
 
@@ -389,6 +430,7 @@ void start_sweep(SJob& j, int mode, int64_t W, int64_t G_fix, int64_t tstride) {
     j.mode = mode;
     j.d = 0;
     j.D = 0;
+    j.launched_to = 0;
     j.Sr.assign(j.regs.size(), 0);
     j.Gr.assign(j.regs.size(), 0);
     uint8_t* p = j.work;
@@ -438,6 +480,7 @@ void setup_stage1(swh_server* s, SJob& j) {
     r.scol = j.scol;
     j.regs.assign(1, r);
     start_sweep(j, kStage1, server_tile(s, j.n), 0, 0);
+    upload_regions(s, j);
 }
 
 void setup_search(swh_server* s, SJob& j) {
@@ -457,6 +500,7 @@ void setup_search(swh_server* s, SJob& j) {
         j.regs.push_back(r);
     }
     start_sweep(j, kSearch, server_tile(s, bw), 0, 0);
+    upload_regions(s, j);
 }
 
 // A traceback round: the block of the walk's current cell (cut at the cell, region 0) and
@@ -484,6 +528,7 @@ void setup_dirs(swh_server* s, SJob& j) {
         }
     }
     start_sweep(j, kDirs, W, G, tstride);
+    upload_regions(s, j);
     for (size_t k = 0; k < j.wtab.size(); ++k) {
         if (j.wtab_reg[k] < 0) continue;
         const Region& r = j.regs[(size_t)j.wtab_reg[k]];
@@ -504,6 +549,74 @@ const int32_t kNotFound = INT32_MAX;  // initial value of a search's found colum
 // synchronisation) is shared by that many launches per mode.
 constexpr int kDiagonalsPerTick = 8;
 
+// Per mode, kDiagonalsPerTick launches over the sweeping jobs that `from` selects: launch k
+// covers diagonal from(j) + k of every selected job j (below its D); from(j) < 0 skips j.
+// The descriptors go to `set`; each selected job's launched_to moves past its launches.
+template <class From>
+int launch_sweeps(swh_server* s, SweepSet& set, SweepParams p, From from) {
+    swh_ctx* c = s->ctx;
+    std::vector<TileSpan>* spans = set.hspans;
+    for (int mo = 0; mo < 3; ++mo) spans[mo].clear();
+    std::vector<int64_t> span_at[3], tiles_at[3];
+    std::vector<int64_t> d0(s->flight.size(), -1);
+    bool any[3] = {false, false, false};
+    for (size_t x = 0; x < s->flight.size(); ++x) {
+        SJob* j = s->flight[x];
+        if (j->phase != kSweep) continue;
+        d0[x] = from(j);
+        if (d0[x] < 0 || d0[x] >= j->D) {
+            d0[x] = -1;
+            continue;
+        }
+        any[j->mode] = true;
+        j->launched_to = std::min(j->D, d0[x] + kDiagonalsPerTick);
+    }
+    for (int mo = 0; mo < 3; ++mo) {
+        if (!any[mo]) continue;
+        for (int step = 0; step < kDiagonalsPerTick; ++step) {
+            span_at[mo].push_back((int64_t)spans[mo].size());
+            int64_t t = 0;
+            int64_t jobs_here = 0;
+            for (size_t x = 0; x < s->flight.size(); ++x) {
+                SJob* j = s->flight[x];
+                if (d0[x] < 0 || j->mode != mo) continue;
+                const int64_t d = d0[x] + step;
+                if (d >= j->D) continue;
+                ++jobs_here;
+                for (size_t q = 0; q < j->regs.size(); ++q) {
+                    const int64_t Sq = j->Sr[q], Gq = j->Gr[q];
+                    if (d > Sq + Gq - 2) continue;
+                    const int64_t s_lo = std::max<int64_t>(0, d - (Gq - 1)), s_hi = std::min(Sq - 1, d);
+                    spans[mo].push_back(TileSpan{t, (int32_t)(j->reg_at + (int64_t)q), (int32_t)s_lo, d});
+                    t += s_hi - s_lo + 1;
+                }
+            }
+            tiles_at[mo].push_back(t);
+            if (t) {
+                ++s->sweep_launches;
+                if (jobs_here > 1) ++s->shared_launches;
+            }
+        }
+        span_at[mo].push_back((int64_t)spans[mo].size());
+    }
+    p.regions = (const Region*)s->pool_base;  // every job's regions lie in its pool block
+    for (int mo = 0; mo < 3; ++mo) {
+        if (spans[mo].empty()) continue;
+        HCK(c, ensure(set.spans[mo], sizeof(TileSpan) * spans[mo].size()));
+        HCK(c, hipMemcpyAsync(set.spans[mo].p, spans[mo].data(), sizeof(TileSpan) * spans[mo].size(),
+                              hipMemcpyHostToDevice, c->stream));
+        for (int step = 0; step < kDiagonalsPerTick; ++step) {
+            if (!tiles_at[mo][(size_t)step]) continue;
+            p.spans = (const TileSpan*)set.spans[mo].p + span_at[mo][(size_t)step];
+            p.nspans = (int32_t)(span_at[mo][(size_t)step + 1] - span_at[mo][(size_t)step]);
+            p.ntiles = tiles_at[mo][(size_t)step];
+            if (s->packed && mo != kSearch) HCK(c, launch_sweep_pk(mo, p, c->lds_pad[s->wps], c->stream));
+            else HCK(c, launch_sweep(mo, p, c->lds_pad[s->wps], c->stream));
+        }
+    }
+    return SWH_OK;
+}
+
 int tick(swh_server* s) {
     swh_ctx* c = s->ctx;
     const auto t_tick = Clock::now();
@@ -519,49 +632,6 @@ int tick(swh_server* s) {
         const int rc = flush_uploads(s, slot);
         if (rc) return rc;
     }
-    // ---- sweeps: per mode, kDiagonalsPerTick launches, each over every job's next diagonal
-    std::vector<Region>* regs = slot.hregs;
-    std::vector<TileSpan>* spans = slot.hspans;
-    for (int mo = 0; mo < 3; ++mo) {
-        regs[mo].clear();
-        spans[mo].clear();
-    }
-    std::vector<int64_t> span_at[3], tiles_at[3];
-    std::vector<size_t> first_reg(s->flight.size(), 0);
-    for (size_t x = 0; x < s->flight.size(); ++x) {
-        SJob* j = s->flight[x];
-        if (j->phase != kSweep) continue;
-        first_reg[x] = regs[j->mode].size();
-        for (const Region& r : j->regs) regs[j->mode].push_back(r);
-    }
-    for (int mo = 0; mo < 3; ++mo) {
-        if (regs[mo].empty()) continue;
-        for (int step = 0; step < kDiagonalsPerTick; ++step) {
-            span_at[mo].push_back((int64_t)spans[mo].size());
-            int64_t t = 0;
-            int64_t jobs_here = 0;
-            for (size_t x = 0; x < s->flight.size(); ++x) {
-                SJob* j = s->flight[x];
-                if (j->phase != kSweep || j->mode != mo) continue;
-                const int64_t d = j->d + step;
-                if (d >= j->D) continue;
-                ++jobs_here;
-                for (size_t q = 0; q < j->regs.size(); ++q) {
-                    const int64_t Sq = j->Sr[q], Gq = j->Gr[q];
-                    if (d > Sq + Gq - 2) continue;
-                    const int64_t s_lo = std::max<int64_t>(0, d - (Gq - 1)), s_hi = std::min(Sq - 1, d);
-                    spans[mo].push_back(TileSpan{t, (int32_t)(first_reg[x] + q), (int32_t)s_lo, d});
-                    t += s_hi - s_lo + 1;
-                }
-            }
-            tiles_at[mo].push_back(t);
-            if (t) {
-                ++s->sweep_launches;
-                if (jobs_here > 1) ++s->shared_launches;
-            }
-        }
-        span_at[mo].push_back((int64_t)spans[mo].size());
-    }
     SweepParams p{};
     p.sc = Score{s->sc.match, s->sc.mismatch, s->sc.gap_open, s->sc.gap_extend};
     if (!c->codes_off) {  // symbol codes when one mapping separates the bytes of every job in flight
@@ -571,26 +641,12 @@ int tick(swh_server* s) {
         p.code_shift1 = code_shift1_for(mask, &s->sc);
         p.code2_shift1 = code2_shift1_for(mask, &s->sc);
     }
-    for (int mo = 0; mo < 3; ++mo) {
-        if (regs[mo].empty() || spans[mo].empty()) continue;
-        HCK(c, ensure(slot.regs[mo], sizeof(Region) * regs[mo].size()));
-        HCK(c, ensure(slot.spans[mo], sizeof(TileSpan) * spans[mo].size()));
-        HCK(c, hipMemcpyAsync(slot.regs[mo].p, regs[mo].data(), sizeof(Region) * regs[mo].size(),
-                              hipMemcpyHostToDevice, c->stream));
-        HCK(c, hipMemcpyAsync(slot.spans[mo].p, spans[mo].data(), sizeof(TileSpan) * spans[mo].size(),
-                              hipMemcpyHostToDevice, c->stream));
-        p.regions = (const Region*)slot.regs[mo].p;
-        for (int step = 0; step < kDiagonalsPerTick; ++step) {
-            if (!tiles_at[mo][(size_t)step]) continue;
-            p.spans = (const TileSpan*)slot.spans[mo].p + span_at[mo][(size_t)step];
-            p.nspans = (int32_t)(span_at[mo][(size_t)step + 1] - span_at[mo][(size_t)step]);
-            p.ntiles = tiles_at[mo][(size_t)step];
-            if (s->packed && mo != kSearch) HCK(c, launch_sweep_pk(mo, p, c->lds_pad[s->wps], c->stream));
-            else HCK(c, launch_sweep(mo, p, c->lds_pad[s->wps], c->stream));
-        }
+    // ---- sweeps of the jobs whose next diagonals were not launched ahead last tick (jobs
+    // that entered a sweep this tick)
+    {
+        const int rc = launch_sweeps(s, slot.now, p, [](const SJob* j) { return j->launched_to <= j->d ? j->d : -1; });
+        if (rc) return rc;
     }
-    HCK(c, hipEventRecord(slot.done, c->stream));
-    slot.pending = true;
     // ---- argmax for jobs whose stage 1 ended last tick ----------------------------------
     std::vector<ArgmaxJob> aj;
     int64_t max_m = 1;
@@ -668,12 +724,26 @@ int tick(swh_server* s) {
             if (nx) s->rb.push_back(swh_server::Rb{(const uint8_t*)j->xps, nx * (int64_t)sizeof(XPoint), j->xps_h.data()});
         }
     }
+    if (readback) {
+        const int rc = issue_readbacks(s);
+        if (rc) return rc;
+    }
+    // ---- the next tick's diagonals of every job whose sweep continues past this tick,
+    // launched now: the device sweeps them while the host waits for the read-back and
+    // plans the next tick
+    {
+        const int rc = launch_sweeps(s, slot.ahead, p, [](const SJob* j) {
+            return j->d + kDiagonalsPerTick < j->D ? j->d + kDiagonalsPerTick : -1;
+        });
+        if (rc) return rc;
+    }
+    HCK(c, hipEventRecord(slot.done, c->stream));
+    slot.pending = true;
     const auto t_launched = Clock::now();
     s->ms_launch += std::chrono::duration<double, std::milli>(t_launched - t_tick).count();
     if (readback) {
-        const int rc = run_readbacks(s);
+        const int rc = finish_readbacks(s);
         if (rc) return rc;
-        for (TickSlot& t : s->ring) t.pending = false;
         ++s->nreadbacks;
     }
     const auto t_read = Clock::now();
@@ -907,6 +977,11 @@ int swh_server_create(int device, const swh_scoring* sc, const swh_opts* opts, i
             return SWH_ERR_HIP;
         }
     }
+    if (hipEventCreateWithFlags(&s->rb_done, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        swh_server_destroy(s);
+        return SWH_ERR_HIP;
+    }
     s->loop = std::thread(serve_loop, s);
     *out = s;
     return SWH_OK;
@@ -1040,14 +1115,13 @@ int swh_server_destroy(swh_server* s) {
         if (s->jobs.find(j->id) == s->jobs.end()) delete j;
     (void)hipStreamSynchronize(s->ctx->stream);
     for (TickSlot& t : s->ring) {
-        for (int mo = 0; mo < 3; ++mo) {
-            if (t.regs[mo].p) (void)hipFree(t.regs[mo].p);
-            if (t.spans[mo].p) (void)hipFree(t.spans[mo].p);
-        }
+        for (SweepSet* w : {&t.now, &t.ahead})
+            for (int mo = 0; mo < 3; ++mo)
+                if (w->spans[mo].p) (void)hipFree(w->spans[mo].p);
         if (t.up_d.p) (void)hipFree(t.up_d.p);
-        if (t.up_h.h) (void)hipHostFree(t.up_h.h);
         if (t.done) (void)hipEventDestroy(t.done);
     }
+    if (s->rb_done) (void)hipEventDestroy(s->rb_done);
     for (PinBuf* b : {&s->rb_h, &s->rbx_h})
         if (b->h) (void)hipHostFree(b->h);
     DevBuf* bufs[] = {&s->argjobs, &s->wargs, &s->wregs, &s->rb_d, &s->rbx_d};
