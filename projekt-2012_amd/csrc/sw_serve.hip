@@ -963,6 +963,8 @@ int swh_server_create(int device, const swh_scoring* sc, const swh_opts* opts, i
     (void)hipMemGetInfo(&freeb, &totalb);
     int64_t budget = hbm_budget_bytes > 0 ? hbm_budget_bytes : (int64_t)(freeb / 2);
     budget = std::min<int64_t>(budget, (int64_t)(freeb * 0.9));
+    // tile spans address the jobs' regions by an int32 index into the pool
+    budget = std::min<int64_t>(budget, (int64_t)INT32_MAX * (int64_t)sizeof(Region));
     if (hipMalloc((void**)&s->pool_base, (size_t)budget) != hipSuccess) {
         (void)hipGetLastError();
         swh_ctx_destroy(c);
