@@ -143,10 +143,12 @@ constexpr double kStepCostI32[kMaxSweepWps + 1] = {0.0, 1.0, 1.772, 2.753, 3.215
 // (packed waves hold 139 VGPRs: a fourth on a SIMD waits for one of three to end)
 constexpr double kStepCostPk[kMaxSweepWps + 1] = {0.0, 1.625, 2.550, 3.653, 3.653 * 4 / 3};
 // the packed sweep replaces the int32 one for a single pair only when the model expects it
-// to take less than this fraction of the time (1 Mbp pair: 0.97, measured 1.04; 2 Mbp: 0.85,
-// measured 0.87), and for pairs of at least kPackedPairMinLen on each side
+// to take less than this fraction of the time, and for pairs of at least kPackedPairMinLen
+// on each side: with the first and last diagonals in int32 (packed_launch) the packed sweep
+// measured 458 against 469 ms at 1 Mbp (2%, under the margin, which the model overstates
+// there), 898 against 956 ms at 1.5 Mbp, 2538 against 3312 ms at 3 Mbp
 constexpr double kPackedPairMargin = 0.95;
-constexpr int64_t kPackedPairMinLen = INT64_C(1) << 19;
+constexpr int64_t kPackedPairMinLen = INT64_C(1250000);
 
 double occupancy_model(swh_ctx* c, int64_t rows, int64_t cols, int64_t W, int wps, bool skew, bool packed) {
     const int64_t S = ceil_div(rows, kStripeRows), G = ceil_div(cols, W);
@@ -159,14 +161,24 @@ double occupancy_model(swh_ctx* c, int64_t rows, int64_t cols, int64_t W, int wp
     for (int64_t d = 0; d < D; ++d) {
         const int64_t tiles = skew ? std::min(S - 1, d / 2) - std::max<int64_t>(0, (d - G + 2) / 2) + 1
                                    : std::min(d, S - 1) - std::max<int64_t>(0, d - (G - 1)) + 1;
-        const int64_t q = ceil_div(packed ? ceil_div(tiles, 2) : tiles, simds);
+        const bool pk = packed && tiles > simds * 3 / 2;  // as packed_launch
+        const double* Td = pk ? T : kStepCostI32;
+        const int64_t q = ceil_div(pk ? ceil_div(tiles, 2) : tiles, simds);
         const int64_t full = q / wps, rest = q % wps;
-        t += (double)full * (st * T[wps] + kRoundUs) + (rest ? st * T[rest] + kRoundUs : 0.0);
+        t += (double)full * (st * Td[wps] + kRoundUs) + (rest ? st * Td[rest] + kRoundUs : 0.0);
     }
     return t;
 }
 
 int usable_wps(const swh_ctx* c, int w) { return w >= 1 && w <= kMaxSweepWps && c->lds_pad[w] ? w : 1; }
+
+// A packed sweep's launch runs the packed kernel only when its tiles give the SIMDs two waves
+// or more (1.5 tiles per SIMD); fewer tiles (a wavefront's first and last diagonals) run
+// faster one per wave in int32.  Both kernels read and write the same int32 hand-off
+// buffers, lane slots, checkpoints and codes.
+bool packed_launch(const swh_ctx* c, int64_t ntiles) {
+    return ntiles > (int64_t)kSweepWaves * std::max(1, c->props.multiProcessorCount) * 3 / 2;
+}
 
 // Packed skewed stripes for a single pair: the fastest modelled of segment widths that put
 // about one, one half or two packed waves per SIMD slot on the widest diagonal (two or three
@@ -345,7 +357,7 @@ int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions,
         p.spans = ds + span_at[(size_t)d];
         p.nspans = (int32_t)(span_at[(size_t)d + 1] - span_at[(size_t)d]);
         p.ntiles = tiles_at[(size_t)d];
-        if (packed && (mode == kStage1 || mode == kDirs))
+        if (packed && (mode == kStage1 || mode == kDirs) && packed_launch(c, p.ntiles))
             HCK(c, launch_sweep_pk(mode, p, c->lds_pad[usable_wps(c, wps)], c->stream));
         else HCK(c, launch_sweep(mode, p, c->lds_pad[usable_wps(c, wps)], c->stream));
     }
@@ -631,7 +643,7 @@ int device_rounds(swh_ctx* c, std::vector<Job>& jobs, std::vector<int64_t>& acti
                 p.spans = (const TileSpan*)c->spans.p + d * N0;
                 p.nspans = (int32_t)nb;
                 p.ntiles = nb * per_diag[(size_t)d];
-                if (tb_packed) HCK(c, launch_sweep_pk(kDirs, p, lds, c->stream));
+                if (tb_packed && packed_launch(c, p.ntiles)) HCK(c, launch_sweep_pk(kDirs, p, lds, c->stream));
                 else HCK(c, launch_sweep(kDirs, p, lds, c->stream));
             }
             HCK(c, launch_walk((const WalkArgs*)c->wargs.p, (int)nb, many_walks(c, nb), c->stream));

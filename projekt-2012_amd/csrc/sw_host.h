@@ -87,6 +87,9 @@ int64_t tile_cols(swh_ctx* c, int64_t cols, int64_t want, int wps = 1);
 int choose_wps(swh_ctx* c, int64_t rows, int64_t cols, bool skew);
 // w if the context can hold the sweep to w waves per SIMD (1 <= w <= kMaxSweepWps), else 1.
 int usable_wps(const swh_ctx* c, int w);
+// True if a packed sweep's launch of ntiles tiles should run the packed kernel (two waves
+// per SIMD or more), false if the int32 kernel (one tile per wave) is faster for it.
+bool packed_launch(const swh_ctx* c, int64_t ntiles);
 // True if njobs walks exceed what the device's LDS holds at once with large walk windows.
 bool many_walks(const swh_ctx* c, int64_t njobs);
 

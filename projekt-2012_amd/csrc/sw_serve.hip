@@ -600,7 +600,6 @@ int launch_sweeps(swh_server* s, SweepSet& set, SweepParams p, From from) {
         span_at[mo].push_back((int64_t)spans[mo].size());
     }
     p.regions = (const Region*)s->pool_base;  // every job's regions lie in its pool block
-    const int64_t simds = (int64_t)kSweepWaves * std::max(1, c->props.multiProcessorCount);
     for (int mo = 0; mo < 3; ++mo) {
         if (spans[mo].empty()) continue;
         HCK(c, ensure(set.spans[mo], sizeof(TileSpan) * spans[mo].size()));
@@ -612,9 +611,8 @@ int launch_sweeps(swh_server* s, SweepSet& set, SweepParams p, From from) {
             p.nspans = (int32_t)(span_at[mo][(size_t)step + 1] - span_at[mo][(size_t)step]);
             p.ntiles = tiles_at[mo][(size_t)step];
             // packed (two tiles per wave) when the launch has tiles enough to give the SIMDs
-            // two waves each or more; fewer tiles run faster one per wave in int32 (both kernels
-            // read and write the same int32 hand-off buffers, checkpoints and codes)
-            if (s->packed && mo != kSearch && p.ntiles > simds * 3 / 2)
+            // two waves each or more; fewer tiles run faster one per wave in int32
+            if (s->packed && mo != kSearch && packed_launch(c, p.ntiles))
                 HCK(c, launch_sweep_pk(mo, p, c->lds_pad[s->wps], c->stream));
             else HCK(c, launch_sweep(mo, p, c->lds_pad[s->wps], c->stream));
         }
