@@ -177,6 +177,7 @@ int usable_wps(const swh_ctx* c, int w) { return w >= 1 && w <= kMaxSweepWps && 
 // faster one per wave in int32.  Both kernels read and write the same int32 hand-off
 // buffers, lane slots, checkpoints and codes.
 bool packed_launch(const swh_ctx* c, int64_t ntiles) {
+    if (c->packed_min_tiles >= 0) return ntiles >= c->packed_min_tiles;
     return ntiles > (int64_t)kSweepWaves * std::max(1, c->props.multiProcessorCount) * 3 / 2;
 }
 
@@ -684,6 +685,10 @@ int device_rounds(swh_ctx* c, std::vector<Job>& jobs, std::vector<int64_t>& acti
 }
 
 int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const EngineOpts& eo, swh_result* out) {
+    {
+        const char* pm = getenv("SWALIGN_PACKED_MIN_TILES");
+        c->packed_min_tiles = pm && *pm ? std::max<int64_t>(0, atoll(pm)) : -1;
+    }
     const int64_t nj = (int64_t)jobs.size();
     StageClock clk{c->trace};
     int64_t rm_elems = 0, sr_elems = 0, scl_elems = 0, ops_elems = 0, xps_elems = 0;

@@ -47,6 +47,10 @@ struct swh_ctx {
     int wps_fixed = 0;              // sweep waves per SIMD forced by the environment, 0 = model
     bool codes_off = false;         // symbol-code scores disabled by the environment
     bool trace = false;             // SWALIGN_TRACE=1: host wall time of each stage to stderr
+    // tiles a packed sweep's launch needs to run packed (packed_launch); -1 = 1.5 per SIMD.
+    // SWALIGN_PACKED_MIN_TILES=n, read at the start of each alignment call, sets it (tests use
+    // 0 to run every launch of a packed sweep packed)
+    int64_t packed_min_tiles = -1;
     hipError_t last = hipSuccess;
     // The allocation-free entry point runs on a copy of the context whose DevBufs are fixed
     // slices of the device's static scratch array; these then replace the allocations of

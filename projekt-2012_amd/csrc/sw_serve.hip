@@ -957,6 +957,7 @@ int swh_server_create(int device, const swh_scoring* sc, const swh_opts* opts, i
     s->sc = *sc;
     if (opts) s->opts = *opts;
     s->max_in_flight = max_in_flight > 0 ? max_in_flight : 1024;
+    if (const char* pm = getenv("SWALIGN_PACKED_MIN_TILES")) c->packed_min_tiles = *pm ? std::max<int64_t>(0, atoll(pm)) : -1;
     // test hook: SWALIGN_SERVE_FAIL_TICK=t makes tick t report a runtime failure (no device
     // work is affected), to exercise how a failure ends queued and running jobs
     if (const char* ft = getenv("SWALIGN_SERVE_FAIL_TICK")) s->fail_tick = atoll(ft);

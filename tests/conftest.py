@@ -6,6 +6,12 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 
+# Packed sweeps run a launch of few tiles in int32 (one tile per wave) by default; the tests'
+# inputs are small, so they ask for every launch of a packed sweep to run packed, which keeps
+# the packed kernel under test.  Tests of the default choice (test_gpu_configs.py,
+# test_sparse_launches_*) remove this.
+os.environ.setdefault("SWALIGN_PACKED_MIN_TILES", "0")
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs a HIP device (MI355X)")

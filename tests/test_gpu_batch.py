@@ -271,3 +271,24 @@ def test_batch_skewed_equals_rectangular(sw, ctx, monkeypatch):
     for x, y in zip(skewed, rect):
         for k in ("score", "end", "start", "cigar", "xpoints"):
             assert x[k] == y[k], k
+
+
+@pytest.mark.parametrize("min_tiles", ["", "64", "1000000"])
+def test_sparse_launches_int32_in_packed_sweeps(sw, orc, ctx, monkeypatch, min_tiles):
+    """Packed sweeps run a launch of few tiles in int32 (one tile per wave) and the others
+    packed: with the default threshold (""), with launches of at least 64 tiles packed (so
+    one sweep mixes both kernels, packed and int32 tiles handing state to each other through
+    the same buffers), and with every launch in int32, a batch and a packed single pair give
+    the oracle's alignments."""
+    monkeypatch.setenv("SWALIGN_PACKED_MIN_TILES", min_tiles)
+    pairs = _mixed_pairs(sw, 41, 120, [30, 400, 1500, 3000])
+    res, st = ctx.batch(pairs, sw.scoring(*SC), tile_cols=256, K=256, Kc=256, xpoints=True)
+    _check(orc, pairs, res, st, K=256, Kc=256)
+    assert all(r["precision"] == 16 for r, (a, b) in zip(res, pairs) if a and b)
+    (a, b) = sw.synth_pair(43, 9000, 8500, 4000, 0.08, 0.015)
+    got = ctx.align(a, b, sw.scoring(*SC), K=512, Kc=256, xpoints=True, precision=16, tile_cols=128)
+    A, B = orient(a, b)
+    ref = orc.align_linear(A, B, orc.scoring(*SC), 512, 256)
+    assert got["precision"] == 16
+    assert (got["score"], got["end"], got["start"], got["cigar"]) == (ref["score"], tuple(ref["end"]),
+                                                                     tuple(ref["start"]), ref["cigar"])

@@ -30,6 +30,16 @@ from conftest import GOLDEN, orient
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(scope="module", autouse=True)
+def default_launches():
+    """The workloads run with the library's defaults: a packed sweep's launches of few tiles
+    run in int32 (conftest asks other tests to run them packed)."""
+    old = os.environ.pop("SWALIGN_PACKED_MIN_TILES", None)
+    yield
+    if old is not None:
+        os.environ["SWALIGN_PACKED_MIN_TILES"] = old
+
+
 def _digest(r):
     c = r["cigar"] or ""
     return dict(score=r["score"], end=list(r["end"]), start=list(r["start"]), n_ops=r["n_ops"],
