@@ -161,6 +161,8 @@ struct SJob {
     int2* srow = nullptr;
     int2* scol = nullptr;
     unsigned long long* key = nullptr;
+    int32_t* rowseg = nullptr;  // stage 1: each row's first tile segment holding its maximum
+    int32_t* seg = nullptr;     // the end row's, after the argmax
     int32_t* found = nullptr;
     WalkState* walk = nullptr;
     uint8_t* ops = nullptr;
@@ -184,6 +186,7 @@ struct SJob {
     int64_t bi = -1, bj = -1;
     WalkState st{};
     unsigned long long key_h = 0;
+    int32_t seg_h = -1;
     int32_t found_h = -1;
     std::vector<uint8_t> ops_h;      // kFetch: the walk's ops, read back
     std::vector<XPoint> xps_h;
@@ -276,7 +279,7 @@ void layout(swh_server* s, SJob& j) {
     const int64_t m = j.m, n = j.n;
     auto al = [](int64_t x) { return ceil_div(x, 256) * 256; };
     int64_t persist = al(m) + al(n) + al(4 * m) + al(8 * j.pl.nsr * n) + al(8 * j.pl.nsc * m) + al(8) + al(4) +
-                      al(sizeof(WalkState)) + al(m + n + 16);
+                      al(sizeof(WalkState)) + al(m + n + 16) + al(4 * m) + al(4);
     j.xcap = j.pl.nsr + 2;
     persist += al((int64_t)sizeof(XPoint) * j.xcap);
     if (m == 0 || n == 0) {  // nothing to sweep: score 0
@@ -417,6 +420,8 @@ void place(swh_server* s, SJob& j) {
     if (!j.pl.nsr) take(0);
     j.scol = j.pl.nsc ? (int2*)take(8 * j.pl.nsc * j.m) : nullptr;
     j.key = (unsigned long long*)take(8);
+    j.rowseg = (int32_t*)take(4 * j.m);
+    j.seg = (int32_t*)take(4);
     j.found = (int32_t*)take(4);
     j.walk = (WalkState*)take(sizeof(WalkState));
     j.ops = take(j.m + j.n + 16);
@@ -482,6 +487,7 @@ void setup_stage1(swh_server* s, SJob& j) {
     r.c1 = j.n - 1;
     r.rowmax = j.rowmax;
     r.srow = j.srow;
+    r.rowseg = j.rowseg;
     r.scol = j.scol;
     j.regs.assign(1, r);
     start_sweep(j, kStage1, server_tile(s, j.n), 0, 0);
@@ -490,8 +496,16 @@ void setup_stage1(swh_server* s, SJob& j) {
 
 void setup_search(swh_server* s, SJob& j) {
     const int64_t br0 = (j.bi / j.pl.K) * j.pl.K, bw = std::min(j.pl.Kc, j.n), nq = ceil_div(j.n, bw);
+    // only the blocks over the first stage-1 segment holding the end row's maximum (columns
+    // [gW, (g+1)W) of rectangular tiles): no column left of it holds the score in that row
+    int64_t q0 = 0, q1 = nq - 1;
+    const int64_t W1 = server_tile(s, j.n), g = j.seg_h;
+    if (g >= 0 && g * W1 < j.n) {
+        q0 = (g * W1) / bw;
+        q1 = (std::min(j.n, (g + 1) * W1) - 1) / bw;
+    }
     j.regs.clear();
-    for (int64_t q = 0; q < nq; ++q) {
+    for (int64_t q = q0; q <= q1; ++q) {
         Region r = base_region(j);
         r.r0 = br0;
         r.r1 = j.bi;
@@ -660,7 +674,7 @@ int tick(swh_server* s) {
     int64_t max_m = 1;
     for (SJob* j : s->flight)
         if (j->phase == kArgmax) {
-            aj.push_back(ArgmaxJob{j->rowmax, j->m, j->key});
+            aj.push_back(ArgmaxJob{j->rowmax, j->m, j->key, j->rowseg, j->seg});
             max_m = std::max(max_m, j->m);
         }
     if (!aj.empty()) {
@@ -669,6 +683,7 @@ int tick(swh_server* s) {
                               c->stream));
         const int bpj = (int)std::min<int64_t>(1024, std::max<int64_t>(1, ceil_div(max_m, 16384)));
         HCK(c, launch_argmax_rows((const ArgmaxJob*)s->argjobs.p, (int)aj.size(), bpj, c->stream));
+        HCK(c, launch_end_segment((const ArgmaxJob*)s->argjobs.p, (int)aj.size(), c->stream));
     }
     // ---- walks for jobs whose block codes are ready ---------------------------------------
     std::vector<WalkArgs> wa;
@@ -720,6 +735,7 @@ int tick(swh_server* s) {
         if (!readback) break;
         if (j->phase == kArgmax) {
             s->rb.push_back(swh_server::Rb{(const uint8_t*)j->key, 8, &j->key_h});
+            s->rb.push_back(swh_server::Rb{(const uint8_t*)j->seg, 4, &j->seg_h});
         } else if (j->phase == kSweep && j->mode == kSearch && j->d + kDiagonalsPerTick >= j->D) {
             s->rb.push_back(swh_server::Rb{(const uint8_t*)j->found, 4, &j->found_h});
         } else if (j->phase == kWalk) {
@@ -766,6 +782,8 @@ int tick(swh_server* s) {
                     j->phase = kArgmax;
                     const unsigned long long zero = 0;
                     queue_value(s, j->key, &zero, 8);
+                    const int32_t none = -1;
+                    queue_value(s, j->seg, &none, 4);
                 } else if (j->mode == kSearch) {
                     if (j->found_h == INT32_MAX || j->found_h < 0) {
                         finish(s, *j, SWH_ERR_INTERNAL);
