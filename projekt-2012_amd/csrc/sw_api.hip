@@ -357,11 +357,14 @@ int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions,
                 std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_spans).count());
     }
     if (ev_before) HCK(c, hipEventRecord(ev_before, c->stream));
+    // diagonals built per part: doubling from 2, up to about 65,536 spans a part (16 diagonals
+    // of the 65,536-pair batch; a single pair's whole list in a few parts)
+    const int64_t chunk_max = std::max<int64_t>(16, ((int64_t)1 << 16) / std::max<int64_t>(1, total / std::max<int64_t>(1, D)));
     int64_t chunk = 2;
     for (int64_t d = 0; d < D; ++d) {
         if (d == d_built) {
             HCK(c, build_to(std::min(D, d + chunk)));
-            chunk = std::min<int64_t>(2 * chunk, 16);
+            chunk = std::min<int64_t>(2 * chunk, chunk_max);
         }
         p.spans = ds + span_at[(size_t)d];
         p.nspans = (int32_t)(span_at[(size_t)d + 1] - span_at[(size_t)d]);
