@@ -205,9 +205,10 @@ double packed_pair_choice(swh_ctx* c, int64_t rows, int64_t cols, int64_t w16, i
             }
         }
     }
-    // A power-of-two width (from 256, at least half the chosen one) divides the checkpoint-
-    // column spacing, and measured faster than the model's width there: the 10 Mbp pair's
-    // stage 1 took 21.6 s at 512 columns against 22.4 s at 832 (22.0 to 22.4 s at 384 to 640).
+    // The width is rounded down to a power of two (from 256; more tiles per diagonal than the
+    // model credits run well when they oversubscribe the resident waves): the 10 Mbp pair's
+    // stage 1 took 21.6 s at 512 columns against 22.4 s at the model's 832 (22.0 to 22.4 s at
+    // 384 to 640).
     if (best >= 0) {
         int64_t W2 = 256;
         while (W2 * 2 <= *W_out) W2 *= 2;

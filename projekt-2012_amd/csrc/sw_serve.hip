@@ -266,8 +266,8 @@ int64_t server_tile(const swh_server* s, int64_t cols);
 int64_t server_tile(const swh_server* s, int64_t cols) {
     if (s->opts.tile_cols > 0) return s->opts.tile_cols;
     // one tile per resident wave across the row, at least 512 columns, rounded down to a power
-    // of two, which divides the checkpoint-column spacing (a lone 1 Mbp job: 545 ms at 512
-    // columns against 611 ms at 977 and 685 ms at 768)
+    // of two (measured: a lone 1 Mbp job took 545 ms at 512 columns against 611 ms at 977 and
+    // 685 ms at 768)
     const int64_t want = std::max<int64_t>(512, tile_cols(s->ctx, cols, 0));
     int64_t W = 512;
     while (W * 2 <= want) W *= 2;
