@@ -377,9 +377,20 @@ __device__ void sweep_tile(const SweepParams& p, const TileSetup& ts, SweepLds& 
     int b_reg;
     load_chunk_inputs<PROF>(p, T, 0, in_reg, b_reg);
     int Qh = in_reg.x, Qf = in_reg.y, Qb = b_reg;
+    // kStage1: the row maxima of the stripe's earlier segments, read as the last chunk starts
+    // so that the read is in flight during its steps rather than after them (rows belong to
+    // one stripe, whose one tile per launch is the only access to them)
+    int pmax[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) pmax[k] = 0;
 
     for (int64_t c = 0; c < nchunks; ++c) {
         if (c + 1 < nchunks) load_chunk_inputs<PROF>(p, T, c + 1, in_reg, b_reg);
+        if (MODE == kStage1 && c + 1 == nchunks && !ts.first_segment) {
+#pragma unroll
+            for (int k = 0; k < R; ++k)
+                if (row0 + k <= T.r1) pmax[k] = T.rowmax[row0 + k];
+        }
         const bool interior = c >= first_interior && c <= last_interior;
         const bool starts = T.first && c == 0 && (!T.last || w >= 64);   // lanes start here, none ends
         const bool ends = c >= first_interior && T.last;                  // lanes end here, none starts
@@ -461,7 +472,7 @@ __device__ void sweep_tile(const SweepParams& p, const TileSetup& ts, SweepLds& 
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             const int64_t row = row0 + k;
-            if (row <= T.r1 && (ts.first_segment || rmax[k] > T.rowmax[row])) {
+            if (row <= T.r1 && (ts.first_segment || rmax[k] > pmax[k])) {
                 T.rowmax[row] = rmax[k];
                 if (T.rowseg) T.rowseg[row] = T.seg;
             }
@@ -875,6 +886,8 @@ __device__ void sweep_tile_pk(const SweepParams& p, const TileSetup (&ts)[2], Sw
     int Qh, Qf, Qb, nh, nf, nb;
     int bl = PROF ? (sym[0] | 0x0c00 | (sym[1] << 16) | 0x0c000000) : ((sym[0] & 0xFFFF) | (sym[1] << 16));
     load(0, Qh, Qf, Qb);
+    // (the row maxima are read after the steps: reading them ahead, as sweep_tile does, holds
+    // eight more registers across the loop, which cost more than the read's latency)
     for (int64_t c = 0; c < nchunks; ++c) {
         if (c + 1 < nchunks) load(c + 1, nh, nf, nb);
         const bool interior = c >= first_interior && c <= last_interior;
