@@ -1,10 +1,13 @@
 """Summarises rocprofv3 result databases (rocpd SQLite) as short text.
 
     python3 rocpd_summary.py DB [DB ...]
+    python3 rocpd_summary.py --by-waves KERNEL_SUBSTRING --simds N DB
 
 Per kernel: dispatch count and total / average / min / max duration; when the database
 holds counter samples (rocprofv3 --pmc), the per-kernel sum and per-dispatch average of
-each counter.
+each counter.  With --by-waves: the dispatches of the kernels whose names hold the
+substring, grouped by waves per SIMD (the grid's 64-lane waves over N SIMDs: up to 1, 1 to
+2, ...), with each group's count and total / average duration.
 """
 import sqlite3
 import sys
@@ -40,5 +43,37 @@ def summarize(path):
     return "\n".join(out)
 
 
+def by_waves(path, sub, simds):
+    c = sqlite3.connect(path)
+    cols = _cols(c, "kernels")
+    if "grid_x" not in cols:
+        return "# %s: no grid sizes (kernels columns: %s)" % (path, ", ".join(cols))
+    groups = {}
+    for name, dur, gx, gy, gz in c.execute("select name, end-start, grid_x, grid_y, grid_z from kernels"):
+        if sub not in name:
+            continue
+        waves = (gx * gy * gz + 63) // 64
+        key = max(1, -(-waves // simds))  # waves per SIMD, rounded up
+        g = groups.setdefault(key, [0, 0, 0])
+        g[0] += 1
+        g[1] += dur
+        g[2] += waves
+    out = ["# %s: dispatches of kernels named *%s*, by waves per SIMD (%d SIMDs)" % (path, sub, simds),
+           "waves per SIMD | dispatches | avg waves | total_ms | avg_us"]
+    for key in sorted(groups):
+        n, tot, w = groups[key]
+        out.append("%d-%d | %d | %.0f | %.3f | %.1f" % (key - 1, key, n, w / n, tot / 1e6, tot / n / 1e3))
+    n = sum(g[0] for g in groups.values())
+    tot = sum(g[1] for g in groups.values())
+    out.append("all | %d | | %.3f | %.1f" % (n, tot / 1e6, tot / max(1, n) / 1e3))
+    return "\n".join(out)
+
+
 if __name__ == "__main__":
-    print("\n\n".join(summarize(p) for p in sys.argv[1:]))
+    a = sys.argv[1:]
+    if a and a[0] == "--by-waves":
+        if len(a) != 5 or a[2] != "--simds":
+            sys.exit("usage: rocpd_summary.py --by-waves KERNEL_SUBSTRING --simds N DB")
+        print(by_waves(a[4], a[1], int(a[3])))
+    else:
+        print("\n\n".join(summarize(p) for p in a))
