@@ -68,16 +68,34 @@ def _counter_avg(fname, counter, kernel_part):
     return None
 
 
+def _stage1_dispatches(fname, counter):
+    """Stage-1 sweep dispatches (int32 and packed kernels) a committed counter profile covers,
+    or None."""
+    path = os.path.join(PROFILES, fname)
+    if not os.path.exists(path):
+        return None
+    n = 0
+    for line in open(path):
+        f = [x.strip() for x in line.split("|")]
+        if len(f) == 5 and f[0] == counter and ("k_sweep<4, 0," in f[1] or "k_sweep_pk<4, 0," in f[1]):
+            n += int(f[2])
+    return n or None
+
+
 TRAFFIC_PROFILES = {1: "k_sweep_stage1_1Mbp_pair", 2: "k_sweep_stage1_10Mbp_pair", 3: "k_sweep_pk_65536_pair_batch"}
 
 
-def measured_traffic(workload, codes=1, precision=32):
+def measured_traffic(workload, codes=1, precision=32, launches=None):
     """HBM bytes per stage-1 launch from the committed FETCH_SIZE / WRITE_SIZE profiles of
     this workload, each scaled by what the counter reads for a 256 MiB copy: (fetched plus
-    written bytes or None, fetched bytes or None, source files or None)."""
+    written bytes or None, fetched bytes or None, source files or None).  A profile of a
+    sweep with another number of launches (another tile layout) describes other launches
+    and is not used."""
     if workload not in TRAFFIC_PROFILES:
         return None, None, None
     name = TRAFFIC_PROFILES[workload]
+    if launches is not None and _stage1_dispatches(name + "_fetch_size.txt", "FETCH_SIZE") != launches:
+        return None, None, None
     k = "k_sweep%s<4, 0, %s>" % ("_pk" if precision == 16 else "", "true" if codes else "false")
     fetch = _counter_avg(name + "_fetch_size.txt", "FETCH_SIZE", k)
     write = _counter_avg(name + "_write_size.txt", "WRITE_SIZE", k)
@@ -104,7 +122,7 @@ def roofline(cells, fwd_ms, launches, workload=None, W=None, K=None, Kc=None, pr
     packed int16 pairs at twice the operation rate)."""
     achieved = cells * OPS_PER_CELL / (fwd_ms / 1000.0) / 1e12
     peak = PEAK_INT16X2_TOPS if precision == 16 else PEAK_INT32_TOPS
-    traffic, fetched, source = measured_traffic(workload, codes, precision)
+    traffic, fetched, source = measured_traffic(workload, codes, precision, launches)
     extra = {}
     if W and K and Kc:
         extra["algorithmic_bytes_per_launch"] = int(algorithmic_bytes(cells, W, K, Kc) / max(launches, 1))

@@ -66,7 +66,7 @@ typedef struct {
     int32_t precision;         /* stage 1: 16 = packed int16 (values relative to each tile's  */
                                /* base) where a tile's range fits int16, else int32; 32 = int32;*/
                                /* 0 = 16 for batches; single pairs 16 where a model expects it */
-                               /* faster (1.25 Mbp and more on each side), else 32             */
+                               /* faster (1.5 Mbp and more on each side), else 32              */
     int32_t want_xpoints;      /* 1: return the path nodes on checkpoint rows                */
     int32_t tile_cols;         /* columns per sweep tile (any width >= 1);                     */
                                /* 0 = auto (about four tiles per CU across the row, >= 256)   */

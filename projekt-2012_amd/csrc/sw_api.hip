@@ -110,6 +110,16 @@ int64_t pair_tile_cols(swh_ctx* c, int64_t cols, int wps, bool skew) {
     return skew ? ceil_div(tile_cols(c, cols, 0, 2 * wps), 64) * 64 : tile_cols(c, cols, 0, wps);
 }
 
+// Segment width of a single pair's int32 stage 1 in shifted stripes: the widest diagonal
+// holds min(S, cols / (W + 64)) tiles (stripe s starts on diagonal s + floor(64s / W)), two
+// per SIMD at W = cols / (2 x SIMDs) - 64, rounded up to a multiple of 64 (1 Mbp pair: 448
+// columns, stage 1 453 ms; 192 columns, four tiles per SIMD, 454 ms; 256 columns 465 ms,
+// 512 columns 484 ms; skewed stripes of 256 columns 474 ms).
+int64_t shifted_pair_cols(swh_ctx* c, int64_t cols) {
+    const int64_t simds = (int64_t)kSweepWaves * std::max(1, c->props.multiProcessorCount);
+    return std::max<int64_t>(64, ceil_div(cols / (2 * simds) - 64, 64) * 64);
+}
+
 // Modelled duration (in tile steps) of a sweep over rows x cols with wps waves per SIMD:
 // one launch per tile anti-diagonal, each lasting ceil(tiles / resident waves) tile times.  A
 // rectangular tile takes W + 63 steps; a skewed stripe's segments W (tile (s, g) on diagonal
@@ -142,13 +152,13 @@ constexpr double kRoundUs = 5.2;
 constexpr double kStepCostI32[kMaxSweepWps + 1] = {0.0, 1.0, 1.772, 2.753, 3.215};
 // (packed waves hold 139 VGPRs: a fourth on a SIMD waits for one of three to end)
 constexpr double kStepCostPk[kMaxSweepWps + 1] = {0.0, 1.625, 2.550, 3.653, 3.653 * 4 / 3};
-// the packed sweep replaces the int32 one for a single pair only when the model expects it
-// to take less than this fraction of the time, and for pairs of at least kPackedPairMinLen
-// on each side: with the first and last diagonals in int32 (packed_launch) the packed sweep
-// measured 458 against 469 ms at 1 Mbp (2%, under the margin, which the model overstates
-// there), 898 against 956 ms at 1.5 Mbp, 2538 against 3312 ms at 3 Mbp
+// the packed sweep replaces the int32 one for a single pair only when the model (of int32
+// skewed stripes) expects it to take less than this fraction of the time, and for pairs of
+// at least kPackedPairMinLen on each side.  Measured against the int32 sweep in shifted
+// stripes (with the first and last diagonals in int32, packed_launch): 457 against 453 ms at
+// 1 Mbp, 913 against 911 ms at 1.5 Mbp, 1308 against 1527 ms at 2 Mbp
 constexpr double kPackedPairMargin = 0.95;
-constexpr int64_t kPackedPairMinLen = INT64_C(1250000);
+constexpr int64_t kPackedPairMinLen = INT64_C(1500000);
 
 double occupancy_model(swh_ctx* c, int64_t rows, int64_t cols, int64_t W, int wps, bool skew, bool packed) {
     const int64_t S = ceil_div(rows, kStripeRows), G = ceil_div(cols, W);
@@ -235,21 +245,24 @@ int choose_wps(swh_ctx* c, int64_t rows, int64_t cols, bool skew) {
 // recorded immediately before the first and after the last launch.
 int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions, int64_t W, int64_t G_fix,
               hipEvent_t ev_before, hipEvent_t ev_after, int wps, int64_t* launches, int64_t tstride, bool packed,
-              bool sync, bool skew) {
+              bool sync, int skew) {
     const int64_t nr = (int64_t)regions.size();
     std::vector<int64_t> Sr((size_t)nr), Gr((size_t)nr), Lr((size_t)nr);
     int64_t G = 0, D = 0, cb_elems = 0, rb_elems = 0, cor_elems = 0;
     // skewed stripes (stage 1, W a multiple of 64, regions of two or more segments): tile
-    // (s, g) on diagonal 2s + g, lane slots as the segments' hand-off
-    skew = skew && mode == kStage1 && W % 64 == 0;
+    // (s, g) on diagonal 2s + g, or s + g in shifted stripes, lane slots as the segments'
+    // hand-off
+    if (mode != kStage1 || W % 64 != 0) skew = 0;
     for (int64_t q = 0; q < nr; ++q) {
         Region& r = regions[(size_t)q];
         const int64_t rows = r.r1 - r.r0 + 1, cols = r.c1 - r.c0 + 1;
         if (rows <= 0 || cols <= 0) return SWH_ERR_INTERNAL;
         Sr[(size_t)q] = ceil_div(rows, kStripeRows);
         Gr[(size_t)q] = ceil_div(cols, W);
-        r.skew = skew && Gr[(size_t)q] >= 2;
-        Lr[(size_t)q] = r.skew ? 2 * (Sr[(size_t)q] - 1) + Gr[(size_t)q] : Sr[(size_t)q] + Gr[(size_t)q] - 1;
+        r.skew = Gr[(size_t)q] >= 2 ? skew : 0;
+        Lr[(size_t)q] = r.skew == 2 ? shifted_last_diag(Sr[(size_t)q] - 1, cols, W) + 1
+                        : r.skew    ? 2 * (Sr[(size_t)q] - 1) + Gr[(size_t)q]
+                                    : Sr[(size_t)q] + Gr[(size_t)q] - 1;
         G = std::max(G, Gr[(size_t)q]);
         D = std::max(D, Lr[(size_t)q]);
         cb_elems += r.skew ? 2 * Sr[(size_t)q] * 64 * (kRowsPerLane + 1) : 2 * rows;
@@ -325,6 +338,13 @@ int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions,
     const bool pipelined = total > (int64_t)1 << 16;
     int64_t pos = 0, d_built = 0;
     std::vector<int64_t> span_at((size_t)D + 1, 0), tiles_at((size_t)D, 0);
+    // shifted regions: the stripes [lo, hi] on the diagonal last built (stripe 0 alone on
+    // diagonal 0); each region is visited on every diagonal it lives on, in order
+    std::vector<int64_t> sh_lo, sh_hi;
+    if (skew == 2) {
+        sh_lo.assign((size_t)nr, 0);
+        sh_hi.assign((size_t)nr, 0);
+    }
     auto build_to = [&](int64_t d_end) {
         const int64_t from = pos;
         for (int64_t d = d_built; d < d_end; ++d) {
@@ -334,7 +354,18 @@ int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions,
                 const int64_t Sq = Sr[(size_t)q], Gq = Gr[(size_t)q];
                 if (d >= Lr[(size_t)q]) break;
                 int64_t s_lo, s_hi;
-                if (regions[(size_t)q].skew) {  // g = d - 2s in [0, Gq)
+                if (regions[(size_t)q].skew == 2) {
+                    // from diagonal d - 1's stripes: each bound moves by at most one
+                    int64_t& lo = sh_lo[(size_t)q];
+                    int64_t& hi = sh_hi[(size_t)q];
+                    if (d > 0) {
+                        const int64_t cols = regions[(size_t)q].c1 - regions[(size_t)q].c0 + 1;
+                        while (hi + 1 < Sq && shifted_first_diag(hi + 1, W) <= d) ++hi;
+                        while (lo < hi && shifted_last_diag(lo, cols, W) < d) ++lo;
+                    }
+                    s_lo = lo;
+                    s_hi = hi;
+                } else if (regions[(size_t)q].skew) {  // g = d - 2s in [0, Gq)
                     s_lo = std::max<int64_t>(0, (d - Gq + 2) / 2);
                     s_hi = std::min(Sq - 1, d / 2);
                 } else {
@@ -782,6 +813,7 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
     // skewed stage-1 stripes (no lane fill and drain between segments) need W a multiple of
     // 64 and, packed, 63 more columns of int16 headroom (a skewed tile spans W + 63 columns)
     bool skew = eo.skew;
+    bool shift = eo.layout == 2;  // shifted stripes: by default for int32 single pairs (below)
     if (eo.batch) {
         wps = usable_wps(c, eo.wps ? eo.wps : (c->wps_fixed ? c->wps_fixed : kBatchWps));
         W = eo.tile_cols > 0 ? eo.tile_cols : kBatchTileCols;
@@ -827,8 +859,16 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
             // a skewed stripe's launches hold tiles of every other diagonal: segments of half
             // the width keep the resident waves busy.  Skewed when the model says so: the 1 Mbp
             // pair (4.58 M against 5.02 M modelled tile steps; 470 against 531 ms measured)
-            wps = eo.wps ? usable_wps(c, eo.wps) : eo.tile_cols > 0 ? 1 : choose_wps(c, max_m, max_n, true);
-            W = eo.tile_cols > 0 ? eo.tile_cols : pair_tile_cols(c, max_n, wps, true);
+            // Shifted stripes (stripe s's segments 64s columns left) unless skewed ones are
+            // asked for: stripe s starts on diagonal s + floor(64s / W) instead of 2s, so the
+            // wavefront fills the device sooner and takes fewer launches (1 Mbp pair: 8789
+            // launches of 448 columns against 11719 of 256, stage 1 453 against 474 ms)
+            shift = eo.layout != 1;
+            wps = eo.wps          ? usable_wps(c, eo.wps)
+                  : eo.tile_cols > 0 ? 1
+                  : shift         ? usable_wps(c, kMaxSweepWps)
+                                  : choose_wps(c, max_m, max_n, true);
+            W = eo.tile_cols > 0 ? eo.tile_cols : shift ? shifted_pair_cols(c, max_n) : pair_tile_cols(c, max_n, wps, true);
             skew = W % 64 == 0;
         } else {
             skew = false;
@@ -861,7 +901,8 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
     }
     clk.mark("setup", (int64_t)live.size());
     int64_t launches = 0;
-    int rc = run_sweep(c, kStage1, p, regions, W, 0, c->ev[0], c->ev[1], wps, &launches, 0, packed, true, skew);
+    int rc = run_sweep(c, kStage1, p, regions, W, 0, c->ev[0], c->ev[1], wps, &launches, 0, packed, true,
+                       skew ? (shift ? 2 : 1) : 0);
     if (rc) return rc;
     clk.mark("stage1", launches);
     for (int64_t k : live) {
@@ -919,13 +960,15 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
             wmax = std::max(wmax, bw);
             int64_t q0 = 0, q1 = nq - 1;
             const int64_t g = endseg[(size_t)k];
-            if (g >= 0 && g * W < j.n) {
-                // the segment's columns in the end row: [gW, (g+1)W), or in a skewed stripe
-                // shifted left by the row's lane (see TileCtx in sw_kernels.hip)
-                const int64_t G = ceil_div(j.n, W);
-                const int64_t l = (skew && G >= 2) ? (j.bi % kStripeRows) / kRowsPerLane : 0;
-                const int64_t lo = std::max<int64_t>(0, g * W - l);
-                const int64_t hi = (g + 1 == G) ? j.n - 1 : std::min(j.n, (g + 1) * W - l) - 1;
+            // the segment's lane-0 columns: [gW, (g+1)W), in shifted stripes 64 columns left
+            // per stripe above the end row's (see TileCtx in sw_kernels.hip)
+            const bool sk = skew && ceil_div(j.n, W) >= 2;
+            const int64_t x0 = g * W - ((sk && shift) ? 64 * (j.bi / kStripeRows) : 0);
+            if (g >= 0 && x0 < j.n) {
+                // the end row's columns in it, in a skewed stripe shifted left by the row's lane
+                const int64_t l = sk ? (j.bi % kStripeRows) / kRowsPerLane : 0;
+                const int64_t lo = std::max<int64_t>(0, x0 - l);
+                const int64_t hi = (x0 + W >= j.n) ? j.n - 1 : std::min(j.n, x0 + W - l) - 1;
                 q0 = lo / bw;
                 q1 = hi / bw;
             }
@@ -1357,10 +1400,11 @@ EngineOpts engine_opts(const swh_opts* o, bool batch) {
     eo.lookahead_auto = batch && !(o && o->tb_lookahead != 0);
     if (o && o->tb_lookahead != 0) eo.lookahead = std::max(0, o->tb_lookahead);
     // skewed stage-1 stripes by default (single pairs: where the wavefront model prefers them);
-    // SWALIGN_SKEW=0 / 1 forces rectangular tiles / skewed stripes
+    // SWALIGN_SKEW=0 / 1 / 2 forces rectangular tiles / skewed stripes / shifted stripes
     const char* sk = getenv("SWALIGN_SKEW");
     eo.skew = sk ? atoi(sk) != 0 : true;
     eo.skew_forced = sk && atoi(sk) != 0;
+    eo.layout = sk && (atoi(sk) == 1 || atoi(sk) == 2) ? atoi(sk) : 0;
     return eo;
 }
 

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "../../include/swalign_hip.h"
@@ -97,11 +98,31 @@ bool packed_launch(const swh_ctx* c, int64_t ntiles);
 // True if njobs walks exceed what the device's LDS holds at once with large walk windows.
 bool many_walks(const swh_ctx* c, int64_t njobs);
 
-// One sweep over `regions`, one launch per tile anti-diagonal (see sw_api.hip).
+// One sweep over `regions`, one launch per tile anti-diagonal (see sw_api.hip).  skew: 0
+// rectangular tiles, 1 skewed stripes, 2 shifted stripes (stage 1; see Region::skew).
 int run_sweep(swh_ctx* c, int mode, swk::SweepParams p, std::vector<swk::Region>& regions, int64_t W,
               int64_t G_fix = 0, hipEvent_t ev_before = nullptr, hipEvent_t ev_after = nullptr, int wps = 1,
               int64_t* launches = nullptr, int64_t tstride = 0, bool packed = false,
-              bool sync = true, bool skew = false);
+              bool sync = true, int skew = 0);
+
+// Shifted stripes (W a multiple of 64, at least two segments): stripe s's segments g run from
+// floor(64s / W), the first whose lane-0 columns [gW - 64s, gW - 64s + W) reach column 0, to
+// floor((cols - 1 + 64s) / W), the one holding column cols - 1 for lane 0; tile (s, g) lies on
+// diagonal s + g.
+inline int64_t shifted_first_diag(int64_t s, int64_t W) { return s + 64 * s / W; }
+inline int64_t shifted_last_diag(int64_t s, int64_t cols, int64_t W) { return s + (cols - 1 + 64 * s) / W; }
+// Stripes [lo, hi] of a region of S shifted stripes with a tile on diagonal d (none if lo > hi).
+inline void shifted_stripes(int64_t d, int64_t S, int64_t cols, int64_t W, int64_t& lo, int64_t& hi) {
+    const int64_t k = W / 64;  // the first diagonal grows by k + 1 every k stripes
+    int64_t h = std::min(S - 1, d * k / (k + 1));
+    while (h + 1 <= S - 1 && shifted_first_diag(h + 1, W) <= d) ++h;
+    while (h >= 0 && shifted_first_diag(h, W) > d) --h;
+    int64_t l = std::max<int64_t>(0, std::min(h, (d * W - (cols - 1)) * k / ((k + 1) * W)));
+    while (l > 0 && shifted_last_diag(l - 1, cols, W) >= d) --l;
+    while (l <= h && shifted_last_diag(l, cols, W) < d) ++l;
+    lo = l;
+    hi = h;
+}
 
 // Widest stage-1 tile (columns) whose values stay in int16 relative to the tile's base under
 // scoring sc, for the packed stage 1; 0 if none.
@@ -178,6 +199,7 @@ struct EngineOpts {
     bool host_ckpt = false;  // checkpoints in mapped pinned host memory
     bool skew = false;       // skewed stage-1 stripes (see run_sweep)
     bool skew_forced = false;  // single pairs: skewed even where the model prefers rectangles
+    int layout = 0;          // skewed stripes: 0 default, 1 skewed, 2 shifted (Region::skew)
 };
 
 // Stage 1, end search and traceback of every job, all jobs side by side in each sweep.

@@ -56,8 +56,9 @@ struct Region {
     // kSearch
     int64_t target_row;
     int32_t target;
-    int32_t skew;      // kStage1: skewed segments (TileCtx in sw_kernels.hip): tiles (s, g) at diagonal
-                       // d = 2s + g, W a multiple of 64, cb holding lane slots instead of a column
+    int32_t skew;      // kStage1: skewed segments (TileCtx in sw_kernels.hip), W a multiple of 64, cb
+                       // holding lane slots instead of a column: 1 = tiles (s, g) at diagonal d = 2s + g,
+                       // 2 = shifted stripes (stripe s's segment grid 64s columns left), d = s + g
     int32_t* found;    // smallest column of target_row holding the target value
 };
 

@@ -70,7 +70,11 @@ __device__ __forceinline__ int wave_lane() { return (int)(threadIdx.x & 63u); }
 // A rectangular tile (columns [C0, C0 + w) for every lane) is first and last.  A skewed
 // stage-1 stripe cuts its columns into segments of W columns for lane 0 and W columns shifted
 // left by l for lane l: only its first segment starts lanes late and only its last ends them
-// late, so its other tiles spend no steps filling or draining lanes.
+// late, so its other tiles spend no steps filling or draining lanes.  In shifted stripes the
+// segment grid of stripe s starts 64s columns left of column 0 (lane 0 of segment g takes
+// columns [gW - 64s, gW - 64s + W), clipped to the region): the bottom row of stripe s - 1's
+// segment g (lane 63's columns [gW - 64s + 1, ...)) and g - 1 then hold all the top boundary
+// segment g of stripe s reads, so tile (s, g) follows both one diagonal later.
 struct TileCtx {
     const uint8_t* A;   // the pair's rows and columns
     const uint8_t* B;
@@ -506,12 +510,17 @@ __device__ bool make_tile(const SweepParams& p, int64_t x, TileSetup& ts, int* s
     const TileSpan sp = p.spans[lo];
     const Region Rg = p.regions[sp.region];
     const bool skew = MODE == kStage1 && Rg.skew;
+    const bool shifted = skew && Rg.skew == 2;
     const int64_t s = sp.s_lo + (x - sp.first_tile);
-    const int64_t g = sp.d - (skew ? 2 * s : s);
+    // diagonal d: s + g (rectangular, shifted), 2s + g (skewed)
+    const int64_t g = sp.d - (skew && !shifted ? 2 * s : s);
     const int64_t rows = Rg.r1 - Rg.r0 + 1, cols = Rg.c1 - Rg.c0 + 1;
     const int64_t W = Rg.W;
     const int64_t nstr = (rows + HR - 1) / HR, nseg = (cols + W - 1) / W;
-    if (s >= nstr || g < 0 || g >= nseg) return false;  // a narrower region has fewer segments
+    // lane 0's columns [x0, x0 + W) relative to c0, clipped to [0, cols); a segment exists
+    // while they reach the region (a narrower region has fewer segments)
+    const int64_t x0 = g * W - (shifted ? 64 * s : 0);
+    if (s >= nstr || x0 + W <= 0 || x0 >= cols || (!shifted && g < 0)) return false;
     TileCtx& T = ts.T;
     T.A = Rg.A;
     T.B = Rg.B;
@@ -525,10 +534,10 @@ __device__ bool make_tile(const SweepParams& p, int64_t x, TileSetup& ts, int* s
     T.target = Rg.target;
     T.R0 = Rg.r0 + s * HR;
     T.r1 = Rg.r1;
-    T.C0 = Rg.c0 + g * W;
-    T.w = min(W, Rg.c1 - T.C0 + 1);
-    T.first = !skew || g == 0;
-    T.last = !skew || g + 1 == nseg;
+    T.C0 = Rg.c0 + max((int64_t)0, x0);
+    T.w = min(x0 + W, cols) - max((int64_t)0, x0);
+    T.first = !skew || x0 <= 0;
+    T.last = !skew || x0 + W >= cols;
     const int64_t coff = T.C0 - Rg.c0;
     T.top = s == 0 ? (Rg.top ? Rg.top + coff : nullptr) : Rg.rb + (s & 1) * cols + coff;
     T.bot = s + 1 < nstr ? Rg.rb + ((s + 1) & 1) * cols + coff : nullptr;
@@ -545,13 +554,13 @@ __device__ bool make_tile(const SweepParams& p, int64_t x, TileSetup& ts, int* s
     if (skew) {
         // lane slots: [segment parity][stripe][lane]; the bottom row of the stripe above (or
         // the region's top row) holds the corner H(R0-1, C0-1) one column left of C0
-        ts.left = g == 0 ? Rg.left : nullptr;
+        ts.left = T.first ? Rg.left : nullptr;
         ts.left_r0 = 0;
-        if (g == 0) ts.corner = (Rg.left && T.R0 >= 1) ? Rg.left[T.R0 - 1].x : 0;
+        if (T.first) ts.corner = (Rg.left && T.R0 >= 1) ? Rg.left[T.R0 - 1].x : 0;
         else ts.corner = T.top ? T.top[-1].x : 0;
         constexpr int64_t LS = lane_slot<R>();
-        if (g > 0) ts.lane_in = Rg.cb + ((g & 1) * nstr + s) * 64 * LS;
-        if (g + 1 < nseg) ts.lane_out = Rg.cb + (((g + 1) & 1) * nstr + s) * 64 * LS;
+        if (!T.first) ts.lane_in = Rg.cb + ((g & 1) * nstr + s) * 64 * LS;
+        if (!T.last) ts.lane_out = Rg.cb + (((g + 1) & 1) * nstr + s) * 64 * LS;
     } else {
         // Left column and corner H(R0-1, C0-1).
         if (g == 0) {
@@ -567,7 +576,7 @@ __device__ bool make_tile(const SweepParams& p, int64_t x, TileSetup& ts, int* s
         if (s > 0 && g + 1 < nseg && wave_lane() == 0) Rg.cor[2 * s + ((g + 1) & 1)] = T.top[T.w - 1].x;
         if (g + 1 < nseg) ts.right = Rg.cb + ((g + 1) & 1) * rows;
     }
-    ts.first_segment = g == 0;
+    ts.first_segment = x0 <= 0;
     ts.found = Rg.found;
     return true;
 }

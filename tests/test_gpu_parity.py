@@ -299,12 +299,14 @@ def test_device_inputs_end_at_last_column(sw, orc, ctx):
             dy.free()
 
 
+@pytest.mark.parametrize("mode", ["1", "2"])
 @pytest.mark.parametrize("m,n,K,Kc", [(3000, 2000, 512, 128), (5000, 700, 256, 64), (2600, 2600, 256, 1024)])
-def test_skewed_forward_checkpoints_bitexact(sw, orc, ctx, monkeypatch, m, n, K, Kc):
-    """Skewed stage-1 stripes (segments shifted left by one column per lane, SWALIGN_SKEW=1
-    on a single pair): score, end, row maxima, checkpoint rows and checkpoint columns equal
-    the oracle's full computation bit for bit."""
-    monkeypatch.setenv("SWALIGN_SKEW", "1")
+def test_skewed_forward_checkpoints_bitexact(sw, orc, ctx, monkeypatch, m, n, K, Kc, mode):
+    """Skewed stage-1 stripes (segments shifted left by one column per lane) on a single pair,
+    SWALIGN_SKEW=1, and shifted stripes (each stripe's segment grid 64 columns left of the one
+    above), SWALIGN_SKEW=2: score, end, row maxima, checkpoint rows and checkpoint columns
+    equal the oracle's full computation bit for bit."""
+    monkeypatch.setenv("SWALIGN_SKEW", mode)
     (a, b), sc = sw.synth_pair(m + n + K, m, n, min(m, n) // 2, 0.1, 0.02), sw.scoring(1, -3, -5, -2)
     A, B = orient(a, b)
     got = ctx.forward(A, B, sc, K, Kc)
@@ -317,14 +319,16 @@ def test_skewed_forward_checkpoints_bitexact(sw, orc, ctx, monkeypatch, m, n, K,
     assert np.array_equal(got["scol"], scol)
 
 
+@pytest.mark.parametrize("mode", ["1", "2"])
 @pytest.mark.parametrize("prec", [32, 16])
 @pytest.mark.parametrize("W,Kc", [(64, 64), (128, 1024), (192, 128), (256, 256), (320, 64)])
-def test_skewed_single_pairs(sw, orc, ctx, monkeypatch, W, Kc, prec):
-    """Single pairs with skewed stage-1 stripes of several segment widths (multiples of 64)
-    and checkpoint-column spacings, so that checkpoint columns fall in a segment's first,
-    interior and last chunks and at each lane's own step, in int32 and in packed int16 (two
-    segments per wave) at one to three waves per SIMD: the oracle's alignment."""
-    monkeypatch.setenv("SWALIGN_SKEW", "1")
+def test_skewed_single_pairs(sw, orc, ctx, monkeypatch, W, Kc, prec, mode):
+    """Single pairs with skewed (SWALIGN_SKEW=1) and shifted (2) stage-1 stripes of several
+    segment widths (multiples of 64) and checkpoint-column spacings, so that checkpoint
+    columns fall in a segment's first, interior and last chunks and at each lane's own step,
+    in int32 and in packed int16 (two segments per wave) at one to three waves per SIMD: the
+    oracle's alignment."""
+    monkeypatch.setenv("SWALIGN_SKEW", mode)
     for k, (m, n) in enumerate([(1500, 1400), (900, 2100), (4100, 300)]):
         (a, b), sc = sw.synth_pair(W * 7 + Kc + k, m, n, min(m, n) // 2, 0.1, 0.02), sw.scoring(1, -3, -5, -2)
         got = ctx.align(a, b, sc, K=256, Kc=Kc, xpoints=True, tile_cols=W, precision=prec, waves_per_simd=k + 1)
