@@ -253,6 +253,7 @@ int run_sweep(swh_ctx* c, int mode, SweepParams p, std::vector<Region>& regions,
     // (s, g) on diagonal 2s + g, or s + g in shifted stripes, lane slots as the segments'
     // hand-off
     if (mode != kStage1 || W % 64 != 0) skew = 0;
+    if (packed && skew == 2) skew = 1;  // the packed kernel takes skewed stripes only (make_tile)
     for (int64_t q = 0; q < nr; ++q) {
         Region& r = regions[(size_t)q];
         const int64_t rows = r.r1 - r.r0 + 1, cols = r.c1 - r.c0 + 1;
@@ -900,6 +901,7 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
         if (rc) return rc;
     }
     clk.mark("setup", (int64_t)live.size());
+    if (packed) shift = false;  // as run_sweep: packed sweeps run skewed stripes
     int64_t launches = 0;
     int rc = run_sweep(c, kStage1, p, regions, W, 0, c->ev[0], c->ev[1], wps, &launches, 0, packed, true,
                        skew ? (shift ? 2 : 1) : 0);

@@ -493,7 +493,9 @@ __device__ void sweep_tile(const SweepParams& p, const TileSetup& ts, SweepLds& 
 // boundary at this tile's last column).  Skewed stage-1 tiles (Region.skew): g = d - 2s, since
 // segment g of stripe s needs segments g and g + 1 of stripe s - 1 (the bottom row of the
 // stripe above at its columns [C0, C0 + w) ends 63 columns into the next segment).
-template <int R, int MODE>
+// SHIFT: the kernel takes shifted stripes (the int32 sweep; the packed one holds 29 more
+// VGPRs with that path compiled in, and its sweeps run skewed stripes instead).
+template <int R, int MODE, bool SHIFT = true>
 __device__ bool make_tile(const SweepParams& p, int64_t x, TileSetup& ts, int* span_io = nullptr) {
     constexpr int64_t HR = 64 * R;
     if (x >= p.ntiles) return false;
@@ -510,7 +512,7 @@ __device__ bool make_tile(const SweepParams& p, int64_t x, TileSetup& ts, int* s
     const TileSpan sp = p.spans[lo];
     const Region Rg = p.regions[sp.region];
     const bool skew = MODE == kStage1 && Rg.skew;
-    const bool shifted = skew && Rg.skew == 2;
+    const bool shifted = SHIFT && skew && Rg.skew == 2;
     const int64_t s = sp.s_lo + (x - sp.first_tile);
     // diagonal d: s + g (rectangular, shifted), 2s + g (skewed)
     const int64_t g = sp.d - (skew && !shifted ? 2 * s : s);
@@ -519,8 +521,13 @@ __device__ bool make_tile(const SweepParams& p, int64_t x, TileSetup& ts, int* s
     const int64_t nstr = (rows + HR - 1) / HR, nseg = (cols + W - 1) / W;
     // lane 0's columns [x0, x0 + W) relative to c0, clipped to [0, cols); a segment exists
     // while they reach the region (a narrower region has fewer segments)
-    const int64_t x0 = g * W - (shifted ? 64 * s : 0);
-    if (s >= nstr || x0 + W <= 0 || x0 >= cols || (!shifted && g < 0)) return false;
+    int64_t x0 = g * W;
+    if (shifted) {
+        x0 -= 64 * s;
+        if (s >= nstr || x0 + W <= 0 || x0 >= cols) return false;
+    } else if (s >= nstr || g < 0 || g >= nseg) {
+        return false;
+    }
     TileCtx& T = ts.T;
     T.A = Rg.A;
     T.B = Rg.B;
@@ -534,10 +541,17 @@ __device__ bool make_tile(const SweepParams& p, int64_t x, TileSetup& ts, int* s
     T.target = Rg.target;
     T.R0 = Rg.r0 + s * HR;
     T.r1 = Rg.r1;
-    T.C0 = Rg.c0 + max((int64_t)0, x0);
-    T.w = min(x0 + W, cols) - max((int64_t)0, x0);
-    T.first = !skew || x0 <= 0;
-    T.last = !skew || x0 + W >= cols;
+    if (shifted) {  // a first segment starts at column c0, a last one ends at c1
+        T.C0 = Rg.c0 + max((int64_t)0, x0);
+        T.w = min(x0 + W, cols) - max((int64_t)0, x0);
+        T.first = x0 <= 0;
+        T.last = x0 + W >= cols;
+    } else {  // the same values for g >= 0, in the forms that keep the packed kernel's registers
+        T.C0 = Rg.c0 + g * W;
+        T.w = min(W, Rg.c1 - T.C0 + 1);
+        T.first = !skew || g == 0;
+        T.last = !skew || g + 1 == nseg;
+    }
     const int64_t coff = T.C0 - Rg.c0;
     T.top = s == 0 ? (Rg.top ? Rg.top + coff : nullptr) : Rg.rb + (s & 1) * cols + coff;
     T.bot = s + 1 < nstr ? Rg.rb + ((s + 1) & 1) * cols + coff : nullptr;
@@ -994,8 +1008,8 @@ __device__ void sweep_main_pk(const SweepParams& p, SweepLds& lds) {
     int span = -1;
     // Either tile may be missing (spans built on the device keep a fixed count of tiles per
     // region, some of which a smaller region does not have); a lone second tile runs first.
-    const bool has0 = make_tile<R, MODE>(p, 2 * x, ts[0], &span);
-    const bool has1 = make_tile<R, MODE>(p, 2 * x + 1, ts[1], &span);
+    const bool has0 = make_tile<R, MODE, false>(p, 2 * x, ts[0], &span);
+    const bool has1 = make_tile<R, MODE, false>(p, 2 * x + 1, ts[1], &span);
     if (!has0 && !has1) return;
     if (!has0) ts[0] = ts[1];
     if (!has0 || !has1) {

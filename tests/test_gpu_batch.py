@@ -253,9 +253,10 @@ def test_batch_device_rounds_equal_host_rounds(sw, orc, ctx, monkeypatch, capfd)
                                               (512, 1024, 1024, 0), (256, 256, 64, 32), (448, 512, 512, 32)])
 def test_batch_skewed_stripes(sw, orc, ctx, monkeypatch, W, K, Kc, precision, mode):
     """Batches run stage 1 in skewed stripes (segments shifted left by one column per lane)
-    whenever the tile width is a multiple of 64, packed and int32, by default and in shifted
-    stripes (SWALIGN_SKEW=2): pairs of one to many segments, checkpoint columns inside
-    segments and at their ends, give the oracle's alignments."""
+    whenever the tile width is a multiple of 64, packed and int32, by default and with
+    SWALIGN_SKEW=2 (shifted stripes for the int32 batches, skewed ones for the packed): pairs of
+    one to many segments, checkpoint columns inside segments and at their ends, give the
+    oracle's alignments."""
     if mode:
         monkeypatch.setenv("SWALIGN_SKEW", mode)
     pairs = _mixed_pairs(sw, 29 + W, 60, [5, 70, 300, 700, 1500, 2600])

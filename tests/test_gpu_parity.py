@@ -323,11 +323,11 @@ def test_skewed_forward_checkpoints_bitexact(sw, orc, ctx, monkeypatch, m, n, K,
 @pytest.mark.parametrize("prec", [32, 16])
 @pytest.mark.parametrize("W,Kc", [(64, 64), (128, 1024), (192, 128), (256, 256), (320, 64)])
 def test_skewed_single_pairs(sw, orc, ctx, monkeypatch, W, Kc, prec, mode):
-    """Single pairs with skewed (SWALIGN_SKEW=1) and shifted (2) stage-1 stripes of several
-    segment widths (multiples of 64) and checkpoint-column spacings, so that checkpoint
-    columns fall in a segment's first, interior and last chunks and at each lane's own step,
-    in int32 and in packed int16 (two segments per wave) at one to three waves per SIMD: the
-    oracle's alignment."""
+    """Single pairs with skewed (SWALIGN_SKEW=1) and shifted (2; int32 sweeps, packed ones keep
+    skewed stripes) stage-1 stripes of several segment widths (multiples of 64) and
+    checkpoint-column spacings, so that checkpoint columns fall in a segment's first, interior
+    and last chunks and at each lane's own step, in int32 and in packed int16 (two segments per
+    wave) at one to three waves per SIMD: the oracle's alignment."""
     monkeypatch.setenv("SWALIGN_SKEW", mode)
     for k, (m, n) in enumerate([(1500, 1400), (900, 2100), (4100, 300)]):
         (a, b), sc = sw.synth_pair(W * 7 + Kc + k, m, n, min(m, n) // 2, 0.1, 0.02), sw.scoring(1, -3, -5, -2)
