@@ -328,6 +328,7 @@ def serve_record(sw, jobs, sc, workload_name, data, budget=64 << 30, warmup=1, p
     (swh_align_pair): the latency a job has with the device to itself."""
     import numpy as np
     cells = [len(a) * len(b) for a, b in jobs]
+    precisions = set()  # stage-1 precisions the served results report
 
     def run(rate):
         rng = np.random.default_rng(7)
@@ -346,6 +347,7 @@ def serve_record(sw, jobs, sc, workload_name, data, budget=64 << 30, warmup=1, p
                 st, r = srv.poll(j)
                 assert st == 0
                 lat.append(r["ms_total"])
+                precisions.add(r["precision"])
             stats = srv.stats()
         return dict(rate_jobs_per_s=round(rate, 2) if rate else None, gcups=round(stats["gcups"], 2),
                     p50_ms=round(float(np.percentile(lat, 50)), 2), p99_ms=round(float(np.percentile(lat, 99)), 2),
@@ -379,7 +381,8 @@ def serve_record(sw, jobs, sc, workload_name, data, budget=64 << 30, warmup=1, p
     top = rates["95%"]
     return {
         "value": top["gcups"], "unit": "GCUPS", "p50_ms": top["p50_ms"], "p99_ms": top["p99_ms"],
-        "dtype": "int32", "data": data,
+        "dtype": ("int16x2 stage 1 (int32 for launches of few tiles), int32 end search and traceback"
+                  if 16 in precisions else "int32"), "data": data,
         "config": {"workload": workload_name, "jobs": len(jobs), "cells": sum(cells), "hbm_pool_bytes": budget,
                    "saturated": sat, "capacity_jobs_per_s": round(capacity, 2), "rates": rates,
                    "plain": {"gcups": round(plain_gc, 2), "jobs": len(plain),
