@@ -111,13 +111,20 @@ int64_t pair_tile_cols(swh_ctx* c, int64_t cols, int wps, bool skew) {
 }
 
 // Segment width of a single pair's int32 stage 1 in shifted stripes: the widest diagonal
-// holds min(S, cols / (W + 64)) tiles (stripe s starts on diagonal s + floor(64s / W)), two
-// per SIMD at W = cols / (2 x SIMDs) - 64, rounded up to a multiple of 64 (1 Mbp pair: 448
-// columns, stage 1 453 ms; 192 columns, four tiles per SIMD, 454 ms; 256 columns 465 ms,
-// 512 columns 484 ms; skewed stripes of 256 columns 474 ms).
-int64_t shifted_pair_cols(swh_ctx* c, int64_t cols) {
+// holds min(S, cols / (W + 64)) tiles (stripe s starts on diagonal s + floor(64s / W)).  The
+// narrowest W, a multiple of 64, at which that is at most min(S, four tiles per SIMD): fewer
+// launches than narrower segments, one round of four waves per SIMD, and no fewer tiles than
+// the stripes allow; at least 192 (below it a launch's fixed cost outweighs its steps).
+// Measured stage 1 (ms) by width:
+//   300 kbp:  64 97.8, 128 86.3, *192* 80.8, 256 81.1
+//   500 kbp:  128 161.7, *192* 157.2, 256 165.8, 320 172.9
+//   1 Mbp:    *192* 450.5, 384 489.9, 448 451.1, 512 483.7 (skewed stripes of 256: 474)
+//   1.2 Mbp:  *256* 612.7, 448 658.4, 576 631.6, 640 667.8
+int64_t shifted_pair_cols(swh_ctx* c, int64_t rows, int64_t cols) {
     const int64_t simds = (int64_t)kSweepWaves * std::max(1, c->props.multiProcessorCount);
-    return std::max<int64_t>(64, ceil_div(cols / (2 * simds) - 64, 64) * 64);
+    const int64_t S = ceil_div(rows, kStripeRows);
+    const int64_t per_tile = std::max(ceil_div(cols, S), ceil_div(cols, 4 * simds));  // W + 64
+    return std::max<int64_t>(192, ceil_div(per_tile - 64, 64) * 64);
 }
 
 // Modelled duration (in tile steps) of a sweep over rows x cols with wps waves per SIMD:
@@ -854,22 +861,26 @@ int run_jobs(swh_ctx* c, std::vector<Job>& jobs, const swh_scoring* sc, const En
             W = Wp;
             wps = wp;
         } else if (skew && !packed &&
-                   (eo.skew_forced || eo.tile_cols > 0 ||
+                   (eo.layout != 1 || eo.skew_forced || eo.tile_cols > 0 ||
                     sweep_model(c, max_m, max_n, choose_wps(c, max_m, max_n, true), true) <
                         sweep_model(c, max_m, max_n, choose_wps(c, max_m, max_n, false), false))) {
+            // int32 single pairs run shifted stripes unless skewed ones are asked for, which
+            // follow the tile-step model: shifted stage 1 beat rectangular tiles at every size
+            // measured (ms): 3 kbp 0.75 against 0.87, 10 kbp 2.47 / 3.00, 50 kbp 12.4 / 15.1,
+            // 300 kbp 80.8 / 103.1, 1 Mbp x 300 kbp 216 / 233, 1 Mbp 451 / 532
             // a skewed stripe's launches hold tiles of every other diagonal: segments of half
             // the width keep the resident waves busy.  Skewed when the model says so: the 1 Mbp
             // pair (4.58 M against 5.02 M modelled tile steps; 470 against 531 ms measured)
             // Shifted stripes (stripe s's segments 64s columns left) unless skewed ones are
             // asked for: stripe s starts on diagonal s + floor(64s / W) instead of 2s, so the
-            // wavefront fills the device sooner and takes fewer launches (1 Mbp pair: 8789
-            // launches of 448 columns against 11719 of 256, stage 1 453 against 474 ms)
+            // wavefront fills the device sooner and takes fewer launches (1 Mbp pair: 10417
+            // launches of 192 columns against 11719 of 256, stage 1 451 against 474 ms)
             shift = eo.layout != 1;
             wps = eo.wps          ? usable_wps(c, eo.wps)
                   : eo.tile_cols > 0 ? 1
                   : shift         ? usable_wps(c, kMaxSweepWps)
                                   : choose_wps(c, max_m, max_n, true);
-            W = eo.tile_cols > 0 ? eo.tile_cols : shift ? shifted_pair_cols(c, max_n) : pair_tile_cols(c, max_n, wps, true);
+            W = eo.tile_cols > 0 ? eo.tile_cols : shift ? shifted_pair_cols(c, max_m, max_n) : pair_tile_cols(c, max_n, wps, true);
             skew = W % 64 == 0;
         } else {
             skew = false;
