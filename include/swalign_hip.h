@@ -68,14 +68,18 @@ typedef struct {
                                /* 0 = 16 for batches; single pairs 16 where a model expects it */
                                /* faster (1.5 Mbp and more on each side), else 32              */
     int32_t want_xpoints;      /* 1: return the path nodes on checkpoint rows                */
-    int32_t tile_cols;         /* columns per sweep tile (any width >= 1);                     */
-                               /* 0 = auto (about four tiles per CU across the row, >= 256)   */
+    int32_t tile_cols;         /* columns per sweep tile (any width >= 1); 0 = auto: int32      */
+                               /* single pairs max(cols / stripes, cols / (4 x SIMDs)) - 64,   */
+                               /* a multiple of 64 >= 192 (shifted stripes); batches 512;      */
+                               /* packed single pairs by a model; traceback blocks about four  */
+                               /* tiles per CU across the block, >= 256                        */
     int32_t tb_lookahead;      /* traceback: checkpoint blocks recomputed ahead of the walk in  */
                                /* each direction per round (beyond 3, only blocks within 2 of  */
                                /* the diagonal); 0 = auto (7; batches: per round, as many as  */
                                /* the walking pairs' blocks leave room for), -1 = none        */
     int32_t waves_per_simd;    /* sweep waves resident per SIMD (1 to 4), 0 = auto (batches 3,  */
-                               /* the request server 4; single pairs 1 to 3 by a model)        */
+                               /* the request server 4, int32 single pairs 4, packed ones 2   */
+                               /* or 3 by a model)                                             */
     int32_t autotune;          /* batch: 1 = time candidate settings on a sample first          */
     int32_t host_checkpoints;  /* 1: checkpoint rows and columns live in pinned host memory    */
                                /* mapped into the device (the reference's temp/row_R side      */
